@@ -1,0 +1,224 @@
+// Internal C++ types of libelpaso_amd: context, device buffers, the model
+// (host description + finalized device layout) and error handling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <atomic>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/elpaso_amd.h"
+
+namespace elp {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define ELP_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::elp::Error(ELP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define ELP_REQUIRE(cond, code, msg)                   \
+  do {                                                 \
+    if (!(cond)) throw ::elp::Error((code), (msg));    \
+  } while (0)
+
+void set_last_error(const std::string &msg);
+
+struct Ctx {
+  int device = 0;
+  std::atomic<int64_t> bytes{0};  // device bytes held by models of this context
+  hipStream_t stream = nullptr;   // default work stream of the context
+  ~Ctx() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// RAII device allocation, accounted against the owning context.
+template <typename T>
+struct DArr {
+  T *p = nullptr;
+  size_t n = 0;
+  Ctx *ctx = nullptr;
+  DArr() = default;
+  DArr(const DArr &) = delete;
+  DArr &operator=(const DArr &) = delete;
+  DArr(DArr &&o) noexcept { *this = std::move(o); }
+  DArr &operator=(DArr &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; ctx = o.ctx;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
+  ~DArr() { release(); }
+  void alloc(Ctx *c, size_t count) {
+    release();
+    ctx = c;
+    n = count;
+    if (count) {
+      ELP_HIP(hipMalloc(&p, count * sizeof(T)));
+      if (ctx) ctx->bytes += (int64_t)(count * sizeof(T));
+    }
+  }
+  void upload(Ctx *c, const std::vector<T> &h) {
+    alloc(c, h.size());
+    if (n) ELP_HIP(hipMemcpy(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void download(std::vector<T> &h) const {
+    h.resize(n);
+    if (n) ELP_HIP(hipMemcpy(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  void zero(hipStream_t s = nullptr) {
+    if (n) ELP_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  size_t bytes() const { return n * sizeof(T); }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      if (ctx) ctx->bytes -= (int64_t)(n * sizeof(T));
+    }
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Material {
+  bool fluid = true;
+  double rho = 0, c = 0, E = 0, nu = 0, eta = 0;
+  int visco_type = 1;
+};
+
+struct ElemGroupIn {
+  int kind;
+  int64_t mat;
+  std::vector<int64_t> ids, conn;
+};
+
+inline int npe_of(int kind) { return (kind == ELP_FLUID8 || kind == ELP_BRICK8) ? 8 : 20; }
+inline int nd_of(int kind) { return (kind == ELP_FLUID8 || kind == ELP_FLUID20) ? 1 : 3; }
+inline bool is_fluid(int kind) { return kind == ELP_FLUID8 || kind == ELP_FLUID20; }
+
+// One kind of volume element after finalize: elements of the kind in
+// ascending id, their node ranks, per-element material parameters, and the
+// node-side adjacency used by the deterministic gather.
+struct KindBlock {
+  int kind = 0, npe = 0, nd = 0;
+  int64_t n_elem = 0;
+  std::vector<int32_t> conn;      // n_elem x npe node ranks
+  std::vector<double> params;     // n_elem x 4 (fluid: rho, 1/c^2, rho, c ; brick: lambda, mu, rho, E)
+  std::vector<int32_t> nodes;     // node ranks whose adjacent elements are of this kind
+  std::vector<int32_t> adj_ptr;   // per entry of `nodes`: adjacent elements (kind-local, ascending id)
+  std::vector<int32_t> adj_elem;
+  std::vector<uint8_t> adj_local; // local node index in the element
+  DArr<int32_t> d_conn, d_nodes, d_adj_ptr, d_adj_elem;
+  DArr<uint8_t> d_adj_local;
+  DArr<double> d_params;
+  DArr<double> d_Ke, d_Me;        // element matrices (nd npe)^2 and npe^2 per element
+};
+
+struct BoxInfo {
+  bool set = false;
+  double origin[3], length[3];
+  int nel[3], order = 1, robin_sides = 0;
+};
+
+struct Model {
+  // keeps the context alive while any device array of the model exists
+  // (declared first so it is destroyed last)
+  std::shared_ptr<Ctx> ctx_ref;
+  Ctx *ctx = nullptr;
+  // ---- input
+  std::vector<int64_t> node_ids;
+  std::vector<double> xyz;
+  std::map<int64_t, Material> mats;
+  std::vector<ElemGroupIn> groups;
+  std::vector<int64_t> imp_elem, imp_face;
+  std::complex<double> Z{0, 0};
+  bool imp_lumped = true;
+  std::vector<int64_t> if_ids, if_fn, if_sn;
+  std::vector<double> if_ori;
+  std::vector<std::pair<int64_t, double>> sources;
+  std::vector<std::pair<int64_t, std::array<double, 3>>> forces;
+  struct Dir { int64_t node; int dof; std::complex<double> v; };
+  std::vector<Dir> dirichlet;
+  double ray_a = 0, ray_b = 0;
+  bool has_rayleigh = false;
+  BoxInfo box;
+  // ---- finalized
+  bool finalized = false, assembled = false;
+  int64_t n_nodes = 0, n = 0, nnz = 0;
+  std::vector<int64_t> sorted_ids;       // node ids ascending (rank -> id)
+  std::vector<int32_t> rank_to_input;    // rank -> position in input arrays
+  std::vector<int32_t> node_row, node_nd; // first row and #dofs per node rank
+  std::vector<uint32_t> node_dofmask;
+  std::vector<uint8_t> row_field;        // 0 structure, 1 fluid, 2 Dirichlet
+  std::vector<int32_t> rowptr, colind;
+  std::vector<int32_t> nbr_ptr, nbr, nbr_off;  // node graph, column offset of each neighbour
+  std::vector<KindBlock> kinds;
+  // extra entries (C_Gamma and FSI S) in a compact CSR over all rows
+  std::vector<int32_t> x_rowptr, x_col, x_fullpos;
+  std::vector<uint8_t> x_type;           // 0 = C_Gamma, 1 = S
+  std::vector<int32_t> xc_ptr, xc_idx;   // contributions into the face/FSI buffer
+  std::vector<int32_t> face_nodes;       // n_faces x nfn node ranks (face node order)
+  int nfn = 0;
+  int64_t n_faces = 0, n_if = 0;
+  std::vector<int32_t> if_fnodes;        // n_if x 4 fluid node ranks
+  std::vector<double> if_factor;         // -ori per interface
+  // visco
+  double eta = 0;
+  int visco_type = 1;
+  bool has_visco = false;
+  // rhs and Dirichlet
+  std::vector<std::complex<double>> F;   // without Dirichlet values
+  std::vector<int32_t> dir_rows;
+  std::vector<std::complex<double>> dir_vals;
+  // device
+  DArr<double> d_xyz;
+  DArr<int32_t> d_rowptr, d_colind, d_nbr_ptr, d_nbr, d_nbr_off, d_node_row;
+  DArr<double> d_K, d_M, d_C;
+  DArr<int32_t> d_x_rowptr, d_x_col, d_x_fullpos, d_xc_ptr, d_xc_idx, d_face_nodes, d_if_fnodes;
+  DArr<uint8_t> d_x_type, d_row_field;
+  DArr<double> d_x_val, d_fbuf, d_if_factor;
+  // solver-side state (built lazily)
+  struct SolverState;
+  std::shared_ptr<SolverState> solver;
+  std::mutex mu;
+};
+
+// host-side builders (elp_model.cpp)
+void finalize_model(Model &m);
+int64_t model_device_bytes_estimate(const Model &m);
+// assembly driver (elp_assembly.hip)
+void assemble_model(Model &m, hipStream_t s);
+void upload_tables(Ctx *ctx);
+float time_volume_assembly(Model &m, int repeats, hipStream_t s);
+void element_matrices_batch(Ctx *ctx, int kind, int64_t n, const double *xyz, double p0,
+                            double p1, double p2, double *Ke, double *Me);
+void dynamic_matrix(Model &m, double omega, std::complex<double> Z, double *a, double *f);
+
+}  // namespace elp
+
+// opaque C-ABI handles
+struct elp_ctx {
+  std::shared_ptr<elp::Ctx> c;
+};
+struct elp_model {
+  std::unique_ptr<elp::Model> m;
+};
+

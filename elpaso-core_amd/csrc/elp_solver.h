@@ -1,0 +1,20 @@
+// Harmonic sweep solver: fused multi-frequency operator, shifted-Laplacian
+// multigrid / l1-Jacobi preconditioners, batched COCG, FP64 refinement.
+#pragma once
+
+#include <complex>
+
+#include "elp_internal.h"
+
+namespace elp {
+
+void default_opts(elp_solve_opts &o);
+
+// Solve A(w_k) x_k = f for all frequencies (see elp_sweep in elpaso_amd.h).
+void sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z, const elp_solve_opts &o,
+           double *X, double *relres, int32_t *iters, elp_sweep_stats *stats, hipStream_t s);
+
+// device bytes the solver needs for a block of `block` frequencies
+int64_t solver_bytes_estimate(const Model &m, int block, int precision);
+
+}  // namespace elp

@@ -1,0 +1,1029 @@
+// Harmonic sweep solver on one MI355X.
+//
+// The system of one frequency w (elPaSo's K_dyn(w), restructured so the
+// global matrices are assembled once):
+//   fluid rows      A = K/w^2 - M + i/(w Z) C_G       (K, M already / rho)
+//   structure rows  A = (1 + i eta g + i w a) K + (-w^2 + i w b) M
+//   plus the FSI block S; Dirichlet rows are identity rows (their columns are
+//   zeroed in the solver's copies: prescribed values are zero).
+// These operators are complex symmetric, so the Krylov method is COCG
+// (conjugate orthogonal CG: CG with the unconjugated bilinear form), whose
+// short recurrence keeps the memory per frequency at a handful of vectors.
+//
+// Vectors of a block of B frequencies are stored [row][B] (frequency
+// fastest): one gather of a column index fetches the B values of that column
+// for all frequencies in one contiguous segment, and K, M and the column
+// indices are read once per block of frequencies (the fused multi-frequency
+// SpMV).  Mixed precision: the Krylov iteration and the preconditioner run in
+// complex FP32 on FP32 shadows of K and M; an outer FP64 iterative refinement
+// recomputes the true residual with the FP64 matrices until ||r||/||b|| <= rtol.
+//
+// Preconditioners: l1-Jacobi (any mesh), or a complex-shifted-Laplacian
+// multigrid V-cycle for fluid box models: P(w) = K/w^2 - (1 - i beta) M + ...,
+// levels = the model, then (for Hex20) the Hex8 mesh on the same grid, then
+// Hex8 meshes coarsened by 2, each rediscretized with the same assembly kernels
+// and connected by trilinear interpolation; l1-Jacobi smoothing.  The V-cycle
+// is a fixed linear operator and complex symmetric (R = P^T, symmetric
+// smoothing), as COCG requires.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "elp_internal.h"
+#include "elp_solver.h"
+
+namespace elp {
+
+template <typename T> struct VT;
+template <> struct VT<float> { using t = float2; };
+template <> struct VT<double> { using t = double2; };
+
+struct OpCoef {
+  double2 kf, mf, ks, ms, g;       // complex coefficients
+  double akf, amf, aks, ams, ag;   // magnitudes (l1 diagonal bound)
+};
+
+template <typename V>
+__device__ inline V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+template <typename T>
+struct OpArgs {
+  int64_t n;
+  const int32_t *rowptr, *colind;
+  const T *K, *M;
+  const int32_t *xrp, *xcol;
+  const uint8_t *xtype;
+  const T *xval;
+  const uint8_t *field;
+  const double *rK, *rM, *rG;
+};
+
+// Fused multi-frequency operator.  Each row is handled by G = max(B, 8)
+// lanes: lane l works on frequency l % B and on the nonzeros with index
+// (l / B) mod (G / B); partial sums are combined with a fixed xor tree, so the
+// result is deterministic.
+//   MODE 0: y = A x
+//   MODE 1: y = b - A x
+//   MODE 2: y = x + (b - A x) / D      (one l1-Jacobi sweep)
+//   MODE 3: y = b / D                  (first sweep from a zero guess)
+template <typename T, int B, int MODE>
+__global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__restrict__ coef,
+                                                 const typename VT<T>::t *__restrict__ x,
+                                                 const typename VT<T>::t *__restrict__ b,
+                                                 typename VT<T>::t *__restrict__ y) {
+  using V = typename VT<T>::t;
+  constexpr int G = B < 8 ? 8 : B;
+  constexpr int S = G / B;
+  const int lane = threadIdx.x % G;
+  const int64_t r = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+  if (r >= a.n) return;
+  const int f = lane % B, sub = lane / B;
+  const uint8_t fld = a.field[r];
+  const OpCoef &cf = coef[f];
+  if (fld == 2) {  // Dirichlet row: identity
+    if (sub == 0) {
+      const int64_t o = r * B + f;
+      if (MODE == 0) y[o] = x[o];
+      else if (MODE == 1) { V t; t.x = b[o].x - x[o].x; t.y = b[o].y - x[o].y; y[o] = t; }
+      else y[o] = b[o];
+    }
+    return;
+  }
+  V accK = {0, 0}, accM = {0, 0}, accX = {0, 0};
+  if (MODE != 3) {
+    const int32_t p1 = a.rowptr[r + 1];
+    for (int32_t p = a.rowptr[r] + sub; p < p1; p += S) {
+      const int64_t c = a.colind[p];
+      const T kv = a.K[p], mv = a.M[p];
+      const V xv = x[c * B + f];
+      accK.x += kv * xv.x; accK.y += kv * xv.y;
+      accM.x += mv * xv.x; accM.y += mv * xv.y;
+    }
+#pragma unroll
+    for (int o = B; o < G; o <<= 1) {
+      accK.x += __shfl_xor(accK.x, o, G); accK.y += __shfl_xor(accK.y, o, G);
+      accM.x += __shfl_xor(accM.x, o, G); accM.y += __shfl_xor(accM.y, o, G);
+    }
+    if (sub == 0) {
+      const V cg = {(T)cf.g.x, (T)cf.g.y};
+      for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
+        const V xv = x[(int64_t)a.xcol[k] * B + f];
+        const T v = a.xval[k];
+        if (a.xtype[k] == 0) {
+          V t = cmul(cg, xv);
+          accX.x += v * t.x; accX.y += v * t.y;
+        } else {
+          accX.x += v * xv.x; accX.y += v * xv.y;
+        }
+      }
+    }
+  }
+  if (sub != 0) return;
+  const int64_t o = r * B + f;
+  const bool fl = fld == 1;
+  const V ck = fl ? V{(T)cf.kf.x, (T)cf.kf.y} : V{(T)cf.ks.x, (T)cf.ks.y};
+  const V cm = fl ? V{(T)cf.mf.x, (T)cf.mf.y} : V{(T)cf.ms.x, (T)cf.ms.y};
+  V ax = cmul(ck, accK), bx = cmul(cm, accM);
+  V res;
+  res.x = ax.x + bx.x + accX.x;
+  res.y = ax.y + bx.y + accX.y;
+  if (MODE == 0) {
+    y[o] = res;
+    return;
+  }
+  const V bv = b[o];
+  if (MODE == 1) {
+    V t; t.x = bv.x - res.x; t.y = bv.y - res.y;
+    y[o] = t;
+    return;
+  }
+  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+  const T inv = (T)(1.0 / d);
+  if (MODE == 3) {
+    V t; t.x = bv.x * inv; t.y = bv.y * inv;
+    y[o] = t;
+    return;
+  }
+  const V xv = x[o];
+  V t;
+  t.x = xv.x + (bv.x - res.x) * inv;
+  t.y = xv.y + (bv.y - res.y) * inv;
+  y[o] = t;
+}
+
+// y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P
+template <typename T, int B, bool ADD>
+__global__ __launch_bounds__(256) void transfer_kernel(int64_t n, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                                       const typename VT<T>::t *__restrict__ x,
+                                                       typename VT<T>::t *__restrict__ y) {
+  using V = typename VT<T>::t;
+  constexpr int G = B < 8 ? 8 : B;
+  constexpr int S = G / B;
+  const int lane = threadIdx.x % G;
+  const int64_t r = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+  if (r >= n) return;
+  const int f = lane % B, sub = lane / B;
+  V acc = {0, 0};
+  for (int32_t p = rp[r] + sub; p < rp[r + 1]; p += S) {
+    const V xv = x[(int64_t)ci[p] * B + f];
+    acc.x += v[p] * xv.x;
+    acc.y += v[p] * xv.y;
+  }
+#pragma unroll
+  for (int o = B; o < G; o <<= 1) {
+    acc.x += __shfl_xor(acc.x, o, G);
+    acc.y += __shfl_xor(acc.y, o, G);
+  }
+  if (sub != 0) return;
+  const int64_t o = r * B + f;
+  if (ADD) {
+    V t = y[o];
+    t.x += acc.x;
+    t.y += acc.y;
+    y[o] = t;
+  } else {
+    y[o] = acc;
+  }
+}
+
+// Deterministic per-frequency reductions over [n][B] vectors.
+//   mode 0: sum a*b (unconjugated), mode 1: sum |a|^2
+constexpr int kRedBlocks = 512;
+template <typename V>
+__global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__restrict__ a, const V *__restrict__ b,
+                                                  int mode, double2 *__restrict__ partial) {
+  __shared__ double2 sh[256];
+  const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+  double2 acc = {0, 0};
+  for (int64_t r = r0 + rl; r < r1; r += R) {
+    const V av = a[r * B + f];
+    if (mode == 0) {
+      const V bv = b[r * B + f];
+      acc.x += (double)av.x * bv.x - (double)av.y * bv.y;
+      acc.y += (double)av.x * bv.y + (double)av.y * bv.x;
+    } else {
+      acc.x += (double)av.x * av.x + (double)av.y * av.y;
+    }
+  }
+  sh[t] = acc;
+  __syncthreads();
+  for (int s = R / 2; s > 0; s >>= 1) {
+    if (rl < s) {
+      sh[t].x += sh[t + s * B].x;
+      sh[t].y += sh[t + s * B].y;
+    }
+    __syncthreads();
+  }
+  if (rl == 0) partial[blockIdx.x * B + f] = sh[t];
+}
+
+__global__ void dot_final_kernel(int nblk, int B, const double2 *__restrict__ partial, double2 *__restrict__ out) {
+  const int f = threadIdx.x;
+  if (f >= B) return;
+  double2 s = {0, 0};
+  for (int k = 0; k < nblk; k++) {
+    s.x += partial[k * B + f].x;
+    s.y += partial[k * B + f].y;
+  }
+  out[f] = s;
+}
+
+// COCG scalars on the device: alpha = rho / mu (0 if inactive), beta = rho'/rho
+__global__ void cocg_alpha_kernel(int B, const double2 *rho, const double2 *mu, const int *active, double2 *alpha) {
+  const int f = threadIdx.x;
+  if (f >= B) return;
+  double2 a = {0, 0};
+  const double den = mu[f].x * mu[f].x + mu[f].y * mu[f].y;
+  if (active[f] && den > 0 && isfinite(den)) {
+    a.x = (rho[f].x * mu[f].x + rho[f].y * mu[f].y) / den;
+    a.y = (rho[f].y * mu[f].x - rho[f].x * mu[f].y) / den;
+  }
+  alpha[f] = a;
+}
+
+__global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, const int *active, double2 *beta) {
+  const int f = threadIdx.x;
+  if (f >= B) return;
+  double2 b = {0, 0};
+  const double den = rho[f].x * rho[f].x + rho[f].y * rho[f].y;
+  if (active[f] && den > 0 && isfinite(den)) {
+    b.x = (rho_new[f].x * rho[f].x + rho_new[f].y * rho[f].y) / den;
+    b.y = (rho_new[f].y * rho[f].x - rho_new[f].x * rho[f].y) / den;
+  }
+  beta[f] = b;
+  rho[f] = rho_new[f];
+}
+
+// x += alpha p ; r -= alpha q
+template <typename V>
+__global__ void cocg_update_kernel(int64_t nt, int B, const double2 *__restrict__ alpha, const V *__restrict__ p,
+                                   const V *__restrict__ q, V *__restrict__ x, V *__restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double2 a = alpha[i % B];
+  const V pv = p[i], qv = q[i];
+  V xv = x[i], rv = r[i];
+  xv.x += a.x * pv.x - a.y * pv.y;
+  xv.y += a.x * pv.y + a.y * pv.x;
+  rv.x -= a.x * qv.x - a.y * qv.y;
+  rv.y -= a.x * qv.y + a.y * qv.x;
+  x[i] = xv;
+  r[i] = rv;
+}
+
+// p = z + beta p
+template <typename V>
+__global__ void cocg_dir_kernel(int64_t nt, int B, const double2 *__restrict__ beta, const V *__restrict__ z,
+                                V *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double2 bt = beta[i % B];
+  const V zv = z[i];
+  V pv = p[i];
+  V o;
+  o.x = zv.x + bt.x * pv.x - bt.y * pv.y;
+  o.y = zv.y + bt.x * pv.y + bt.y * pv.x;
+  p[i] = o;
+}
+
+// refinement: r_in = r64 * s_f (to the inner precision), and x64 += d * w_f
+template <typename V>
+__global__ void scale_to_kernel(int64_t nt, int B, const double *__restrict__ s, const double2 *__restrict__ r64,
+                                V *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double sc = s[i % B];
+  V o;
+  o.x = r64[i].x * sc;
+  o.y = r64[i].y * sc;
+  out[i] = o;
+}
+
+template <typename V>
+__global__ void add_scaled_kernel(int64_t nt, int B, const double *__restrict__ w, const V *__restrict__ d,
+                                  double2 *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double sc = w[i % B];
+  double2 v = x[i];
+  v.x += sc * (double)d[i].x;
+  v.y += sc * (double)d[i].y;
+  x[i] = v;
+}
+
+__global__ void bcast_rhs_kernel(int64_t n, int B, const double2 *__restrict__ F, double2 *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * B) return;
+  b[i] = F[i / B];
+}
+
+__global__ void unpack_kernel(int64_t n, int B, int nb, const double2 *__restrict__ x, double2 *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nb) return;
+  const int64_t f = i / n, r = i % n;
+  out[i] = x[r * B + f];
+}
+
+// solver shadows: FP32 copies, Dirichlet columns zeroed; row l1 sums
+template <typename T>
+__global__ void shadow_kernel(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                              const uint8_t *__restrict__ field, const double *__restrict__ K,
+                              const double *__restrict__ M, T *__restrict__ Ko, T *__restrict__ Mo,
+                              double *__restrict__ rK, double *__restrict__ rM) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double sk = 0, sm = 0;
+  for (int32_t p = rowptr[r]; p < rowptr[r + 1]; p++) {
+    const bool z = field[colind[p]] == 2 && colind[p] != r;
+    const double k = z ? 0.0 : K[p], m = z ? 0.0 : M[p];
+    if (Ko) {
+      Ko[p] = (T)k;
+      Mo[p] = (T)m;
+    }
+    sk += fabs(k);
+    sm += fabs(m);
+  }
+  rK[r] = sk;
+  rM[r] = sm;
+}
+
+template <typename T>
+__global__ void extra_shadow_kernel(int64_t n, const int32_t *__restrict__ xrp, const int32_t *__restrict__ xcol,
+                                    const uint8_t *__restrict__ xtype, const uint8_t *__restrict__ field,
+                                    const double *__restrict__ xv, T *__restrict__ out, double *__restrict__ rG) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double sg = 0;
+  for (int32_t k = xrp[r]; k < xrp[r + 1]; k++) {
+    const double v = field[xcol[k]] == 2 ? 0.0 : xv[k];
+    out[k] = (T)v;
+    if (xtype[k] == 0) sg += fabs(v);
+  }
+  rG[r] = sg;
+}
+
+static int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+// ---------------------------------------------------------------------------
+// solver state: levels of the hierarchy
+// ---------------------------------------------------------------------------
+struct LevelData {
+  Model *model = nullptr;
+  std::unique_ptr<Model> own;
+  int64_t n = 0, nnz = 0;
+  DArr<float> K32, M32, X32;
+  DArr<double> K64s, M64s, X64s;
+  const double *K64 = nullptr, *M64 = nullptr, *X64 = nullptr;
+  DArr<double> rK, rM, rG;
+  // interpolation from the next coarser level
+  int64_t nc = 0;
+  DArr<int32_t> P_rp, P_ci, R_rp, R_ci;
+  DArr<float> P32, R32;
+  DArr<double> P64, R64;
+  template <typename T> OpArgs<T> args() const;
+};
+
+template <>
+OpArgs<float> LevelData::args<float>() const {
+  Model &m = *model;
+  return {n, m.d_rowptr.p, m.d_colind.p, K32.p, M32.p, m.d_x_rowptr.p, m.d_x_col.p, m.d_x_type.p, X32.p,
+          m.d_row_field.p, rK.p, rM.p, rG.p};
+}
+template <>
+OpArgs<double> LevelData::args<double>() const {
+  Model &m = *model;
+  return {n, m.d_rowptr.p, m.d_colind.p, K64, M64, m.d_x_rowptr.p, m.d_x_col.p, m.d_x_type.p, X64,
+          m.d_row_field.p, rK.p, rM.p, rG.p};
+}
+
+}  // namespace elp
+
+struct elp::Model::SolverState {
+  std::vector<std::unique_ptr<LevelData>> levels;
+  bool mg = false;
+  int mg_levels = 0, mg_min_nodes = 0;
+};
+
+namespace elp {
+
+void default_opts(elp_solve_opts &o) {
+  o.precision = ELP_PREC_MIXED;
+  o.method = ELP_METHOD_COCG;
+  o.precond = ELP_PC_MG;
+  o.block = 32;
+  o.rtol = 1e-8;
+  o.max_iters = 4000;
+  o.inner_rtol = 1e-4;
+  o.restart = 50;
+  o.shift = 0.5;
+  o.smooth = 2;
+  o.coarse_sweeps = 20;
+  o.mg_levels = 0;
+  o.mg_min_nodes = 2000;
+}
+
+static void prepare_level(LevelData &L, hipStream_t s) {
+  Model &m = *L.model;
+  Ctx *c = m.ctx;
+  L.n = m.n;
+  L.nnz = m.nnz;
+  L.K32.alloc(c, m.nnz);
+  L.M32.alloc(c, m.nnz);
+  L.rK.alloc(c, m.n);
+  L.rM.alloc(c, m.n);
+  L.rG.alloc(c, m.n);
+  const bool dir = !m.dir_rows.empty();
+  hipLaunchKernelGGL(shadow_kernel<float>, dim3(nblk(m.n, 256)), dim3(256), 0, s, m.n, m.d_rowptr.p, m.d_colind.p,
+                     m.d_row_field.p, m.d_K.p, m.d_M.p, L.K32.p, L.M32.p, L.rK.p, L.rM.p);
+  ELP_HIP(hipGetLastError());
+  if (dir) {
+    L.K64s.alloc(c, m.nnz);
+    L.M64s.alloc(c, m.nnz);
+    hipLaunchKernelGGL(shadow_kernel<double>, dim3(nblk(m.n, 256)), dim3(256), 0, s, m.n, m.d_rowptr.p,
+                       m.d_colind.p, m.d_row_field.p, m.d_K.p, m.d_M.p, L.K64s.p, L.M64s.p, L.rK.p, L.rM.p);
+    ELP_HIP(hipGetLastError());
+    L.K64 = L.K64s.p;
+    L.M64 = L.M64s.p;
+  } else {
+    L.K64 = m.d_K.p;
+    L.M64 = m.d_M.p;
+  }
+  const int64_t nx = (int64_t)m.x_col.size();
+  L.X32.alloc(c, std::max<int64_t>(nx, 1));
+  L.X64s.alloc(c, std::max<int64_t>(nx, 1));
+  hipLaunchKernelGGL(extra_shadow_kernel<float>, dim3(nblk(m.n, 256)), dim3(256), 0, s, m.n, m.d_x_rowptr.p,
+                     m.d_x_col.p, m.d_x_type.p, m.d_row_field.p, m.d_x_val.p, L.X32.p, L.rG.p);
+  hipLaunchKernelGGL(extra_shadow_kernel<double>, dim3(nblk(m.n, 256)), dim3(256), 0, s, m.n, m.d_x_rowptr.p,
+                     m.d_x_col.p, m.d_x_type.p, m.d_row_field.p, m.d_x_val.p, L.X64s.p, L.rG.p);
+  ELP_HIP(hipGetLastError());
+  L.X64 = L.X64s.p;
+}
+
+// Hex8 fluid box with consistent Robin faces on the requested sides
+static std::unique_ptr<Model> make_box_level(const Model &fine, const int nel[3], std::complex<double> Z) {
+  auto m = std::make_unique<Model>();
+  m->ctx_ref = fine.ctx_ref;
+  m->ctx = fine.ctx;
+  const int px = nel[0] + 1, py = nel[1] + 1, pz = nel[2] + 1;
+  const int64_t nn = (int64_t)px * py * pz;
+  m->node_ids.resize(nn);
+  m->xyz.resize(nn * 3);
+  for (int64_t k = 0; k < nn; k++) {
+    int64_t i = k % px, j = (k / px) % py, l = k / ((int64_t)px * py);
+    m->node_ids[k] = k + 1;
+    m->xyz[3 * k] = fine.box.origin[0] + fine.box.length[0] * i / nel[0];
+    m->xyz[3 * k + 1] = fine.box.origin[1] + fine.box.length[1] * j / nel[1];
+    m->xyz[3 * k + 2] = fine.box.origin[2] + fine.box.length[2] * l / nel[2];
+  }
+  Material mat = fine.mats.begin()->second;
+  m->mats[1] = mat;
+  ElemGroupIn g;
+  g.kind = ELP_FLUID8;
+  g.mat = 1;
+  static const int nat[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  const int64_t ne = (int64_t)nel[0] * nel[1] * nel[2];
+  for (int64_t e = 0; e < ne; e++) {
+    int64_t i = e % nel[0], j = (e / nel[0]) % nel[1], l = e / ((int64_t)nel[0] * nel[1]);
+    g.ids.push_back(e + 1);
+    for (int a = 0; a < 8; a++)
+      g.conn.push_back((i + nat[a][0]) + px * ((j + nat[a][1]) + (int64_t)py * (l + nat[a][2])) + 1);
+    // Robin sides: Hex8 face f lies on the box side f when the element touches it
+    const bool on[6] = {j == 0, j == nel[1] - 1, l == 0, i == nel[0] - 1, l == nel[2] - 1, i == 0};
+    for (int f = 0; f < 6; f++)
+      if ((fine.box.robin_sides >> f & 1) && on[f]) {
+        m->imp_elem.push_back(e + 1);
+        m->imp_face.push_back(f + 1);
+      }
+  }
+  m->groups.push_back(std::move(g));
+  m->Z = Z;
+  m->imp_lumped = false;
+  return m;
+}
+
+// trilinear interpolation from a coarse box lattice to the rows of `fine`
+static void build_interp(LevelData &F, const Model &coarse_like_box, const int nelc[3]) {
+  const Model &fm = *F.model;
+  const BoxInfo &bx = coarse_like_box.box;
+  std::vector<int32_t> rp(1, 0), ci;
+  std::vector<double> v;
+  std::vector<std::vector<std::pair<int32_t, double>>> cols_of(fm.n);
+  const int px = nelc[0] + 1, py = nelc[1] + 1;
+  for (int64_t r = 0; r < fm.n_nodes; r++) {
+    if (fm.node_row[r] < 0) continue;
+    const int64_t in = fm.rank_to_input[r];
+    double t[3];
+    int cell[3];
+    for (int d = 0; d < 3; d++) {
+      const double h = bx.length[d] / nelc[d];
+      double s = (fm.xyz[in * 3 + d] - bx.origin[d]) / h;
+      int c = (int)std::floor(s);
+      c = std::min(std::max(c, 0), nelc[d] - 1);
+      t[d] = std::min(std::max(s - c, 0.0), 1.0);
+      cell[d] = c;
+    }
+    auto &lst = cols_of[fm.node_row[r]];
+    for (int k = 0; k < 2; k++)
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+          double w = (i ? t[0] : 1 - t[0]) * (j ? t[1] : 1 - t[1]) * (k ? t[2] : 1 - t[2]);
+          if (std::fabs(w) < 1e-13) continue;
+          int32_t col = (int32_t)((cell[0] + i) + (int64_t)px * ((cell[1] + j) + (int64_t)py * (cell[2] + k)));
+          lst.push_back({col, w});
+        }
+  }
+  for (int64_t r = 0; r < fm.n; r++) {
+    auto &lst = cols_of[r];
+    std::sort(lst.begin(), lst.end());
+    for (auto &p : lst) {
+      ci.push_back(p.first);
+      v.push_back(p.second);
+    }
+    rp.push_back((int32_t)ci.size());
+  }
+  const int64_t nc = (int64_t)px * py * (nelc[2] + 1);
+  // transpose
+  std::vector<int32_t> trp(nc + 1, 0), tci(ci.size());
+  std::vector<double> tv(ci.size());
+  for (int32_t c : ci) trp[c + 1]++;
+  for (int64_t c = 0; c < nc; c++) trp[c + 1] += trp[c];
+  std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+  for (int64_t r = 0; r < fm.n; r++)
+    for (int32_t p = rp[r]; p < rp[r + 1]; p++) {
+      int32_t q = fill[ci[p]]++;
+      tci[q] = (int32_t)r;
+      tv[q] = v[p];
+    }
+  Ctx *c = fm.ctx;
+  F.nc = nc;
+  F.P_rp.upload(c, rp);
+  F.P_ci.upload(c, ci);
+  F.R_rp.upload(c, trp);
+  F.R_ci.upload(c, tci);
+  F.P64.upload(c, v);
+  F.R64.upload(c, tv);
+  std::vector<float> v32(v.begin(), v.end()), tv32(tv.begin(), tv.end());
+  F.P32.upload(c, v32);
+  F.R32.upload(c, tv32);
+}
+
+static bool mg_possible(const Model &m) {
+  if (!m.box.set || m.mats.size() != 1 || !m.mats.begin()->second.fluid) return false;
+  for (auto v : m.row_field)
+    if (v == 0) return false;
+  return m.dir_rows.empty();
+}
+
+static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::complex<double> Z, hipStream_t s) {
+  const bool want_mg = o.precond == ELP_PC_MG && mg_possible(m);
+  if (m.solver && m.solver->mg == want_mg && m.solver->mg_levels == o.mg_levels &&
+      m.solver->mg_min_nodes == o.mg_min_nodes)
+    return *m.solver;
+  m.solver.reset();
+  auto st = std::make_shared<Model::SolverState>();
+  st->mg = want_mg;
+  st->mg_levels = o.mg_levels;
+  st->mg_min_nodes = o.mg_min_nodes;
+  auto L0 = std::make_unique<LevelData>();
+  L0->model = &m;
+  prepare_level(*L0, s);
+  st->levels.push_back(std::move(L0));
+  if (want_mg) {
+    int nel[3] = {m.box.nel[0], m.box.nel[1], m.box.nel[2]};
+    bool first = true;
+    const int max_levels = o.mg_levels > 0 ? o.mg_levels : 8;
+    while ((int)st->levels.size() < max_levels) {
+      int nc[3] = {nel[0], nel[1], nel[2]};
+      if (!(first && m.box.order == 2)) {
+        // halve (rounding up: the trilinear interpolation does not need nested grids)
+        for (int d = 0; d < 3; d++) nc[d] = std::max(1, (nel[d] + 1) / 2);
+        if ((int64_t)(nc[0] + 1) * (nc[1] + 1) * (nc[2] + 1) < o.mg_min_nodes) break;
+      }
+      first = false;
+      auto cm = make_box_level(m, nc, Z);
+      finalize_model(*cm);
+      assemble_model(*cm, s);
+      LevelData &fine = *st->levels.back();
+      cm->box = m.box;
+      build_interp(fine, *cm, nc);
+      auto Lc = std::make_unique<LevelData>();
+      Lc->own = std::move(cm);
+      Lc->model = Lc->own.get();
+      prepare_level(*Lc, s);
+      st->levels.push_back(std::move(Lc));
+      for (int d = 0; d < 3; d++) nel[d] = nc[d];
+    }
+  }
+  ELP_HIP(hipStreamSynchronize(s));
+  m.solver = st;
+  return *st;
+}
+
+// ---------------------------------------------------------------------------
+// work buffers and the iteration, per precision T and block size B
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LevelWork {
+  using V = typename VT<T>::t;
+  DArr<V> x, b, r, e, ta, tb;
+};
+
+template <typename T, int B>
+struct Solver {
+  using V = typename VT<T>::t;
+  Model &m;
+  Model::SolverState &st;
+  const elp_solve_opts &o;
+  hipStream_t s;
+  Ctx *ctx;
+  std::vector<LevelWork<T>> lw;
+  DArr<OpCoef> coefA, coefP;
+  DArr<double2> red_part, red_out, rho, rho_new, mu, alpha, beta;
+  DArr<int> active;
+  DArr<V> xi, ri, zi, pi, qi;    // inner Krylov vectors
+  double2 *h_red = nullptr;     // pinned
+  // statistics
+  hipEvent_t ev0, ev1;
+  double spmv_ms = 0;
+  int64_t spmv_calls = 0;
+  bool timing = true;
+
+  Solver(Model &mm, Model::SolverState &ss, const elp_solve_opts &oo, hipStream_t ss2)
+      : m(mm), st(ss), o(oo), s(ss2), ctx(mm.ctx) {
+    const int nl = (int)st.levels.size();
+    lw.resize(nl);
+    for (int l = 0; l < nl; l++) {
+      const int64_t n = st.levels[l]->n * B;
+      lw[l].x.alloc(ctx, n);
+      lw[l].b.alloc(ctx, n);
+      lw[l].r.alloc(ctx, n);
+      lw[l].e.alloc(ctx, n);
+      lw[l].ta.alloc(ctx, n);
+      lw[l].tb.alloc(ctx, n);
+    }
+    const int64_t n0 = m.n * B;
+    xi.alloc(ctx, n0); ri.alloc(ctx, n0); zi.alloc(ctx, n0); pi.alloc(ctx, n0); qi.alloc(ctx, n0);
+    coefA.alloc(ctx, B);
+    coefP.alloc(ctx, B);
+    red_part.alloc(ctx, (size_t)kRedBlocks * B);
+    red_out.alloc(ctx, B);
+    rho.alloc(ctx, B); rho_new.alloc(ctx, B); mu.alloc(ctx, B); alpha.alloc(ctx, B); beta.alloc(ctx, B);
+    active.alloc(ctx, B);
+    ELP_HIP(hipHostMalloc((void **)&h_red, sizeof(double2) * B, hipHostMallocDefault));
+    ELP_HIP(hipEventCreate(&ev0));
+    ELP_HIP(hipEventCreate(&ev1));
+  }
+  ~Solver() {
+    if (h_red) (void)hipHostFree(h_red);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+  }
+
+  template <int MODE, typename TT = T>
+  void op(int l, const OpCoef *coef, const typename VT<TT>::t *x, const typename VT<TT>::t *b,
+          typename VT<TT>::t *y) {
+    constexpr int G = B < 8 ? 8 : B;
+    const LevelData &L = *st.levels[l];
+    const int grid = nblk(L.n, 256 / G);
+    hipLaunchKernelGGL((op_kernel<TT, B, MODE>), dim3(grid), dim3(256), 0, s, L.template args<TT>(), coef, x, b, y);
+    ELP_HIP(hipGetLastError());
+  }
+  template <bool ADD>
+  void transfer(int64_t n, const int32_t *rp, const int32_t *ci, const T *v, const V *x, V *y) {
+    constexpr int G = B < 8 ? 8 : B;
+    hipLaunchKernelGGL((transfer_kernel<T, B, ADD>), dim3(nblk(n, 256 / G)), dim3(256), 0, s, n, rp, ci, v, x, y);
+    ELP_HIP(hipGetLastError());
+  }
+  const T *Pv(const LevelData &L) const;
+  const T *Rv(const LevelData &L) const;
+
+  // per-frequency reduction into h_red (host, after sync)
+  template <typename VV>
+  void reduce(int64_t n, const VV *a, const VV *b, int mode, double2 *dev_out) {
+    hipLaunchKernelGGL(dot_kernel<VV>, dim3(kRedBlocks), dim3(256), 0, s, n, B, a, b, mode, red_part.p);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(64), 0, s, kRedBlocks, B, red_part.p, dev_out);
+    ELP_HIP(hipGetLastError());
+  }
+  void fetch(const double2 *dev) {
+    ELP_HIP(hipMemcpyAsync(h_red, dev, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
+    ELP_HIP(hipStreamSynchronize(s));
+  }
+
+  // smoothing sweeps on level l from xin (nullptr = zero) into out
+  void smooth(int l, const V *b, const V *xin, int sweeps, V *out) {
+    LevelWork<T> &W = lw[l];
+    const V *cur = xin;
+    for (int k = 0; k < sweeps; k++) {
+      V *dst = (k == sweeps - 1) ? out : ((cur == W.ta.p) ? W.tb.p : W.ta.p);
+      if (cur == nullptr) op<3>(l, coefP.p, nullptr, b, dst);
+      else op<2>(l, coefP.p, cur, b, dst);
+      cur = dst;
+    }
+  }
+
+  // z = V-cycle(b) on level l
+  void vcycle(int l, const V *b, V *z) {
+    const int nl = (int)st.levels.size();
+    if (l == nl - 1) {
+      smooth(l, b, nullptr, nl == 1 ? std::max(1, o.smooth) : o.coarse_sweeps, z);
+      return;
+    }
+    LevelWork<T> &W = lw[l];
+    const LevelData &L = *st.levels[l];
+    const int nu = std::max(1, o.smooth);
+    smooth(l, b, nullptr, nu, W.x.p);
+    op<1>(l, coefP.p, W.x.p, b, W.r.p);
+    LevelWork<T> &C = lw[l + 1];
+    transfer<false>(L.nc, L.R_rp.p, L.R_ci.p, Rv(L), W.r.p, C.b.p);
+    vcycle(l + 1, C.b.p, C.e.p);
+    transfer<true>(L.n, L.P_rp.p, L.P_ci.p, Pv(L), C.e.p, W.x.p);
+    smooth(l, b, W.x.p, nu, z);
+  }
+
+  void precond(const V *r, V *z) {
+    if (st.levels.size() == 1) smooth(0, r, nullptr, std::max(1, o.smooth), z);
+    else vcycle(0, r, z);
+  }
+
+  // COCG on A (level 0, precision T) for the block: solves A d = rhs (rhs in
+  // ri on entry) from d = 0 until ||r_f|| <= tol * ||rhs_f|| for all active f.
+  // iters[f] accumulates iterations while f is active.
+  void cocg(const std::vector<int> &act0, double tol, int maxit, std::vector<int> &iters) {
+    const int64_t nt = m.n * B;
+    std::vector<int> act = act0;
+    ELP_HIP(hipMemsetAsync(xi.p, 0, nt * sizeof(V), s));
+    reduce(m.n, ri.p, ri.p, 1, red_out.p);
+    fetch(red_out.p);
+    std::vector<double> nr0(B);
+    for (int f = 0; f < B; f++) nr0[f] = std::sqrt(h_red[f].x);
+    ELP_HIP(hipMemcpyAsync(active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    precond(ri.p, zi.p);
+    ELP_HIP(hipMemcpyAsync(pi.p, zi.p, nt * sizeof(V), hipMemcpyDeviceToDevice, s));
+    reduce(m.n, ri.p, zi.p, 0, rho.p);
+    const int tpb = 256;
+    for (int it = 0; it < maxit; it++) {
+      if (timing) ELP_HIP(hipEventRecord(ev0, s));
+      op<0>(0, coefA.p, pi.p, nullptr, qi.p);
+      if (timing) ELP_HIP(hipEventRecord(ev1, s));
+      spmv_calls++;
+      reduce(m.n, pi.p, qi.p, 0, mu.p);
+      hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, rho.p, mu.p, active.p, alpha.p);
+      hipLaunchKernelGGL((cocg_update_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, alpha.p, pi.p, qi.p,
+                         xi.p, ri.p);
+      reduce(m.n, ri.p, ri.p, 1, red_out.p);
+      fetch(red_out.p);
+      if (timing) {
+        float ms = 0;
+        ELP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        spmv_ms += ms;
+      }
+      bool any = false, changed = false;
+      for (int f = 0; f < B; f++) {
+        if (!act[f]) continue;
+        iters[f]++;
+        const double nr = std::sqrt(h_red[f].x);
+        if (!(nr > tol * nr0[f]) || !std::isfinite(nr) || nr > 1e6 * nr0[f]) {
+          act[f] = 0;
+          changed = true;
+        } else {
+          any = true;
+        }
+      }
+      if (!any) break;
+      if (changed) ELP_HIP(hipMemcpyAsync(active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+      precond(ri.p, zi.p);
+      reduce(m.n, ri.p, zi.p, 0, rho_new.p);
+      hipLaunchKernelGGL(cocg_beta_kernel, dim3(1), dim3(64), 0, s, B, rho_new.p, rho.p, active.p, beta.p);
+      hipLaunchKernelGGL((cocg_dir_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, beta.p, zi.p, pi.p);
+      ELP_HIP(hipGetLastError());
+    }
+  }
+};
+
+template <typename T, int B>
+const T *Solver<T, B>::Pv(const LevelData &L) const {
+  if constexpr (std::is_same<T, float>::value) return L.P32.p;
+  else return L.P64.p;
+}
+template <typename T, int B>
+const T *Solver<T, B>::Rv(const LevelData &L) const {
+  if constexpr (std::is_same<T, float>::value) return L.R32.p;
+  else return L.R64.p;
+}
+
+static OpCoef make_coef(const Model &m, double w, std::complex<double> Z, double shift) {
+  const std::complex<double> I(0, 1);
+  std::complex<double> kf = 1.0 / (w * w), mf = -(1.0 - I * shift);
+  std::complex<double> ks = 1.0 + I * m.eta * (m.visco_type == 2 ? w : 1.0);
+  std::complex<double> ms = -w * w * (1.0 - I * shift);
+  if (m.has_rayleigh) {
+    ks += I * w * m.ray_a;
+    ms += I * w * m.ray_b;
+  }
+  std::complex<double> g = (m.n_faces && std::abs(Z) > 0) ? I / (w * Z) : 0.0;
+  OpCoef c;
+  c.kf = {kf.real(), kf.imag()};
+  c.mf = {mf.real(), mf.imag()};
+  c.ks = {ks.real(), ks.imag()};
+  c.ms = {ms.real(), ms.imag()};
+  c.g = {g.real(), g.imag()};
+  c.akf = std::abs(kf);
+  c.amf = std::abs(mf);
+  c.aks = std::abs(ks);
+  c.ams = std::abs(ms);
+  c.ag = std::abs(g);
+  return c;
+}
+
+// One block of up to B frequencies: FP64 refinement around the inner solver.
+template <typename T, int B>
+static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
+                        std::complex<double> Z, double *X, double *relres, int32_t *iters, int &refinements) {
+  Model &m = S.m;
+  const elp_solve_opts &o = S.o;
+  hipStream_t s = S.s;
+  std::vector<OpCoef> cA(B), cP(B);
+  for (int f = 0; f < B; f++) {
+    const double w = 2.0 * M_PI * freqs[std::min(f, nb - 1)];
+    cA[f] = make_coef(m, w, Z, 0.0);
+    cP[f] = make_coef(m, w, Z, o.shift);
+  }
+  ELP_HIP(hipMemcpyAsync(S.coefA.p, cA.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
+  ELP_HIP(hipMemcpyAsync(S.coefP.p, cP.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
+  // FP64 outer vectors
+  const int64_t nt = m.n * B;
+  DArr<double2> F, b64, x64, r64;
+  std::vector<double2> hF(m.n);
+  for (int64_t r = 0; r < m.n; r++) hF[r] = {m.F[r].real(), m.F[r].imag()};
+  for (size_t k = 0; k < m.dir_rows.size(); k++) hF[m.dir_rows[k]] = {m.dir_vals[k].real(), m.dir_vals[k].imag()};
+  F.upload(m.ctx, hF);
+  b64.alloc(m.ctx, nt);
+  x64.alloc(m.ctx, nt);
+  r64.alloc(m.ctx, nt);
+  hipLaunchKernelGGL(bcast_rhs_kernel, dim3(nblk(nt, 256)), dim3(256), 0, s, m.n, B, F.p, b64.p);
+  ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
+  // the FP64 operator lives in the FP64 solver when T = float; otherwise S itself
+  auto op64 = [&](const double2 *x, const double2 *b, double2 *y) { S.template op<1, double>(0, S.coefA.p, x, b, y); };
+  S.reduce(m.n, b64.p, b64.p, 1, S.red_out.p);
+  S.fetch(S.red_out.p);
+  std::vector<double> nb2(B);
+  for (int f = 0; f < B; f++) nb2[f] = std::sqrt(S.h_red[f].x);
+  std::vector<int> it(B, 0);
+  std::vector<double> rel(B, 1.0);
+  DArr<double> scl;
+  scl.alloc(m.ctx, B);
+  const int max_outer = 30;
+  std::vector<double> prev(B, INFINITY), wprev(B, 0.0);
+  std::vector<int> stalled(B, 0), lastact(B, 0);
+  for (int outer = 0; outer <= max_outer; outer++) {
+    op64(x64.p, b64.p, r64.p);
+    S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+    S.fetch(S.red_out.p);
+    for (int f = 0; f < B; f++) rel[f] = nb2[f] > 0 ? std::sqrt(S.h_red[f].x) / nb2[f] : 0.0;
+    // a refinement step that did not reduce the residual is taken back and
+    // the frequency is not refined further
+    std::vector<double> undo(B, 0.0);
+    bool any_undo = false;
+    for (int f = 0; f < B; f++) {
+      if (!lastact[f]) continue;
+      if (!(rel[f] < prev[f])) {
+        undo[f] = -wprev[f];
+        stalled[f] = 1;
+        any_undo = true;
+      } else if (rel[f] > 0.5 * prev[f]) {
+        stalled[f] = 1;  // keep the small gain, stop refining
+      }
+    }
+    if (any_undo) {
+      ELP_HIP(hipMemcpyAsync(scl.p, undo.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL((add_scaled_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B,
+                         scl.p, S.xi.p, x64.p);
+      op64(x64.p, b64.p, r64.p);
+      S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+      S.fetch(S.red_out.p);
+      for (int f = 0; f < B; f++) rel[f] = nb2[f] > 0 ? std::sqrt(S.h_red[f].x) / nb2[f] : 0.0;
+    }
+    std::vector<int> act(B, 0);
+    std::vector<double> inv(B, 0.0), wgt(B, 0.0);
+    bool any = false;
+    for (int f = 0; f < B; f++) {
+      const double nr = rel[f] * nb2[f];
+      if (f < nb && !stalled[f] && rel[f] > o.rtol && std::isfinite(rel[f])) {
+        act[f] = 1;
+        any = true;
+        inv[f] = 1.0 / nr;
+        wgt[f] = nr;
+      }
+    }
+    if (!any || outer == max_outer) break;
+    int total = 0;
+    for (int f = 0; f < nb; f++) total += it[f];
+    if (total >= o.max_iters * nb) break;
+    refinements++;
+    ELP_HIP(hipMemcpyAsync(scl.p, inv.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((scale_to_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B, scl.p,
+                       r64.p, S.ri.p);
+    double tol = o.inner_rtol;
+    if (std::is_same<T, double>::value) {
+      // FP64 inner solve: aim directly below the target
+      double worst = 0;
+      for (int f = 0; f < B; f++)
+        if (act[f]) worst = std::max(worst, rel[f]);
+      tol = std::max(0.3 * o.rtol / worst, 1e-15);
+    }
+    S.cocg(act, tol, o.max_iters, it);
+    ELP_HIP(hipMemcpyAsync(scl.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((add_scaled_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B, scl.p,
+                       S.xi.p, x64.p);
+    ELP_HIP(hipGetLastError());
+    ELP_HIP(hipStreamSynchronize(s));
+    for (int f = 0; f < B; f++) {
+      prev[f] = rel[f];
+      wprev[f] = wgt[f];
+      lastact[f] = act[f];
+    }
+  }
+  for (int f = 0; f < nb; f++) {
+    if (relres) relres[f] = rel[f];
+    if (iters) iters[f] = it[f];
+  }
+  if (X) {
+    DArr<double2> out;
+    out.alloc(m.ctx, m.n * nb);
+    hipLaunchKernelGGL(unpack_kernel, dim3(nblk(m.n * nb, 256)), dim3(256), 0, s, m.n, B, nb, x64.p, out.p);
+    ELP_HIP(hipGetLastError());
+    ELP_HIP(hipMemcpyAsync(X, out.p, sizeof(double2) * m.n * nb, hipMemcpyDeviceToHost, s));
+  }
+  ELP_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T, int B>
+static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double *freqs, std::complex<double> Z,
+                      const elp_solve_opts &o, double *X, double *relres, int32_t *iters, elp_sweep_stats *stats,
+                      hipStream_t s) {
+  Solver<T, B> S(m, st, o, s);
+  int refinements = 0;
+  for (int64_t f0 = 0; f0 < nf; f0 += B) {
+    const int nb = (int)std::min<int64_t>(B, nf - f0);
+    solve_block<T, B>(S, freqs + f0, nb, Z, X ? X + 2 * f0 * m.n : nullptr, relres ? relres + f0 : nullptr,
+                      iters ? iters + f0 : nullptr, refinements);
+  }
+  if (stats) {
+    stats->spmv_calls = S.spmv_calls;
+    stats->spmv_ms = S.spmv_ms;
+    using V = typename VT<T>::t;
+    const int64_t nx = (int64_t)m.x_col.size();
+    stats->spmv_bytes = m.nnz * (int64_t)(4 + 2 * sizeof(T)) + (m.n + 1) * 4 * 2 + m.n +
+                        nx * (int64_t)(4 + 1 + sizeof(T)) + 2 * m.n * B * (int64_t)sizeof(V);
+    stats->refinements = refinements;
+  }
+}
+
+int64_t solver_bytes_estimate(const Model &m, int block, int precision) {
+  const int64_t vb = precision == ELP_PREC_FP64 ? 16 : 8;
+  int64_t b = m.nnz * 8 + m.n * 24;                    // FP32 shadows, l1 sums
+  b += (int64_t)m.n * block * (5 * vb * 2 + 4 * 16);   // level-0 + Krylov + FP64 outer vectors
+  return b + b / 4;                                    // coarse levels (about 1/8 each) and slack
+}
+
+void sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z, const elp_solve_opts &o, double *X,
+           double *relres, int32_t *iters, elp_sweep_stats *stats, hipStream_t s) {
+  ELP_REQUIRE(o.method == ELP_METHOD_COCG, ELP_EINVAL, "only the COCG method is available");
+  ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
+  for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
+  for (auto &d : m.dir_vals)
+    ELP_REQUIRE(d == std::complex<double>(0, 0), ELP_EINVAL, "the sweep solver supports zero prescribed values only");
+  auto t0 = std::chrono::steady_clock::now();
+  Model::SolverState &st = solver_state(m, o, Z, s);
+  auto t1 = std::chrono::steady_clock::now();
+  const int B = o.precision == ELP_PREC_FP64 ? 1 : o.block;
+  if (o.precision == ELP_PREC_FP64) {
+    run_sweep<double, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s);
+  } else {
+    switch (B) {
+      case 1: run_sweep<float, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 4: run_sweep<float, 4>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 8: run_sweep<float, 8>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 16: run_sweep<float, 16>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 32: run_sweep<float, 32>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      default: throw Error(ELP_EINVAL, "block must be 1, 4, 8, 16 or 32");
+    }
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  if (stats) {
+    stats->setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  }
+}
+
+}  // namespace elp

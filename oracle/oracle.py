@@ -1,0 +1,456 @@
+"""Python side of the CPU oracle: elPaSo's global bookkeeping restated with
+numpy on top of the C element/scatter routines in elp_oracle.c.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module; the product never does.
+
+It accepts any object with the attributes of ``elpaso_amd.model.Model``
+(duck-typed; it does not import the product package).
+
+Pinning: DOF numbering and the pattern are checked against brute-force dense
+assembly on tiny meshes and against the exact nnz of structured meshes;
+assembled matrices against closed forms (tests/test_oracle_*.py).  The solve
+is scipy's SuperLU (complex128), standing in for elPaSo's MUMPS LU, which is
+not available here: MUMPS' own bits are parity unpinned; solutions are pinned
+by the FP64 residual ||b - A x|| / ||b||.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FLUID8, BRICK8, FLUID20, BRICK20 = 1, 2, 3, 4
+NPE = {FLUID8: 8, BRICK8: 8, FLUID20: 20, BRICK20: 20}
+NDOF = {FLUID8: 1, BRICK8: 3, FLUID20: 1, BRICK20: 3}
+ELEM_DOFS = {FLUID8: (16,), FLUID20: (16,), BRICK8: (0, 1, 2), BRICK20: (0, 1, 2)}
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_c128p = np.ctypeslib.ndpointer(np.complex128, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class _Cplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def _cx(z):
+    return _Cplx(float(np.real(z)), float(np.imag(z)))
+
+
+def lib():
+    """Load oracle/build/liborc.so, building it with make if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "build", "liborc.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE], check=True, capture_output=True)
+    L = C.CDLL(path)
+    L.orc_gauss_point_1d.restype = C.c_double
+    L.orc_gauss_weight_1d.restype = C.c_double
+    L.orc_gauss_weight_3d.restype = C.c_double
+    L.orc_gauss_weight_2d.restype = C.c_double
+    L.orc_shape_hex.restype = C.c_double
+    L.orc_shape_hex.argtypes = [C.c_int, C.c_int, C.c_int, _f64p]
+    L.orc_shape_quad.restype = C.c_double
+    L.orc_shape_quad.argtypes = [C.c_int, C.c_int, C.c_int, _f64p]
+    L.orc_gauss_point_3d.argtypes = [C.c_int, C.c_int, _f64p]
+    L.orc_gauss_point_2d.argtypes = [C.c_int, C.c_int, _f64p]
+    L.orc_hex8_table.argtypes = [_f64p]
+    L.orc_hex8_face_table.argtypes = [_f64p]
+    L.orc_hex8_face_nodes.argtypes = [C.c_int, _i32p]
+    L.orc_hex20_face_nodes.argtypes = [C.c_int, _i32p]
+    L.orc_jacobian3d.restype = C.c_double
+    L.orc_jacobian3d.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p]
+    L.orc_fluid_mass.argtypes = [C.c_int, _f64p, C.c_double, _f64p]
+    L.orc_fluid_stiffness.argtypes = [C.c_int, _f64p, _f64p]
+    L.orc_surface_integral.argtypes = [C.c_int, _f64p, _f64p]
+    L.orc_brick_stiffness.argtypes = [C.c_int, _f64p, _Cplx, C.c_double, _c128p]
+    L.orc_brick_mass.argtypes = [C.c_int, _f64p, C.c_double, _f64p]
+    L.orc_setup_c.argtypes = [_Cplx, C.c_double, _c128p]
+    L.orc_fluid_dynamic.argtypes = [C.c_int, _f64p, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, _i32p, _Cplx, C.c_int, _c128p]
+    L.orc_brick_dynamic.argtypes = [C.c_int, _f64p, _Cplx, C.c_double, C.c_double,
+                                    C.c_double, _c128p]
+    L.orc_coupling_matrix.argtypes = [_f64p, _f64p]
+    L.orc_pattern.restype = C.c_int64
+    L.orc_pattern.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, C.c_int, _i32p, _i32p,
+                              _i32p, C.c_void_p]
+    L.orc_assemble_km.argtypes = [C.c_int, C.c_int, _i32p, _f64p, _i32p, C.c_double,
+                                  C.c_double, C.c_double, C.c_int, _i32p, _i32p,
+                                  C.c_void_p, C.c_void_p]
+    L.orc_assemble_dynamic.argtypes = [C.c_int, C.c_int, _i32p, _f64p, _i32p, C.c_double,
+                                       C.c_double, C.c_double, _Cplx, C.c_double,
+                                       C.c_void_p, _Cplx, C.c_int, _i32p, _i32p, _c128p]
+    L.orc_assemble_face.argtypes = [C.c_int, C.c_int, _i32p, _f64p, _i32p, C.c_int,
+                                    _i32p, _i32p, _f64p]
+    L.orc_assemble_fsi.argtypes = [C.c_int, _i32p, _f64p, _i32p, _i32p, _f64p, _i32p,
+                                   _i32p, _f64p]
+    _LIB = L
+    return L
+
+
+# ---------------------------------------------------------------------------
+# element-level helpers
+# ---------------------------------------------------------------------------
+def shape_hex(nnod, k, deriv, gp):
+    return lib().orc_shape_hex(nnod, k, deriv, np.ascontiguousarray(gp, dtype=np.float64))
+
+
+def shape_quad(nnod, k, deriv, gp):
+    g = np.zeros(3)
+    g[: len(gp)] = gp
+    return lib().orc_shape_quad(nnod, k, deriv, g)
+
+
+def hex8_table():
+    t = np.zeros(256)
+    lib().orc_hex8_table(t)
+    return t
+
+
+def hex8_face_table():
+    t = np.zeros(48)
+    lib().orc_hex8_face_table(t)
+    return t
+
+
+def face_nodes(nnod, face):
+    idx = np.zeros(8 if nnod == 20 else 4, dtype=np.int32)
+    (lib().orc_hex20_face_nodes if nnod == 20 else lib().orc_hex8_face_nodes)(face, idx)
+    return idx
+
+
+def fluid_km(xyz, c=343.0):
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = len(x)
+    K, M = np.zeros(n * n), np.zeros(n * n)
+    lib().orc_fluid_stiffness(n, x.ravel(), K)
+    lib().orc_fluid_mass(n, x.ravel(), c, M)
+    return K.reshape(n, n), M.reshape(n, n)
+
+
+def brick_km(xyz, E=210e9, nu=0.3, rho=7850.0):
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = len(x)
+    K = np.zeros(9 * n * n, dtype=np.complex128)
+    M = np.zeros(9 * n * n)
+    lib().orc_brick_stiffness(n, x.ravel(), _cx(E), nu, K)
+    lib().orc_brick_mass(n, x.ravel(), rho, M)
+    return K.reshape(3 * n, 3 * n), M.reshape(3 * n, 3 * n)
+
+
+def setup_c(E, nu):
+    Cm = np.zeros(36, dtype=np.complex128)
+    lib().orc_setup_c(_cx(E), nu, Cm)
+    return Cm.reshape(6, 6)
+
+
+def surface_integral(xyz_face):
+    x = np.ascontiguousarray(xyz_face, dtype=np.float64)
+    n = len(x)
+    Cm = np.zeros(n * n)
+    rc = lib().orc_surface_integral(n, x.ravel(), Cm)
+    if rc:
+        raise ValueError("detJ <= 0 on face")
+    return Cm.reshape(n, n)
+
+
+def coupling_matrix(xyz4):
+    Cm = np.zeros(48)
+    lib().orc_coupling_matrix(np.ascontiguousarray(xyz4, dtype=np.float64).ravel(), Cm)
+    return Cm.reshape(12, 4)
+
+
+# ---------------------------------------------------------------------------
+# global bookkeeping (analysis/analysis.cpp)
+# ---------------------------------------------------------------------------
+class Numbering:
+    """activateDofsAtNodes (analysis.cpp:475-520): nodes in ascending id, DOFs
+    by enum value, each active DOF gets the next row."""
+
+    def __init__(self, model):
+        ids = np.asarray(model.node_ids, dtype=np.int64)
+        order = np.argsort(ids, kind="stable")
+        self.sorted_ids = ids[order]
+        self.node_index = order  # position in model arrays of the k-th node by id
+        nn = len(ids)
+        active = np.zeros((nn, 21), dtype=bool)  # indexed by rank of node id
+        for g in model.groups:
+            r = self.rank(g.conn.ravel())
+            for d in ELEM_DOFS[g.kind]:
+                active[r, d] = True
+        self.active = active
+        rows = -np.ones((nn, 21), dtype=np.int64)
+        flat = active.ravel()
+        rows.ravel()[flat] = np.arange(int(flat.sum()))
+        self.rows = rows
+        self.n = int(flat.sum())
+
+    def rank(self, node_ids):
+        """rank (0-based position in ascending id order) of node ids"""
+        r = np.searchsorted(self.sorted_ids, node_ids)
+        assert np.all(self.sorted_ids[r] == node_ids), "unknown node id"
+        return r
+
+    def row(self, node_ids, dof):
+        return self.rows[self.rank(np.asarray(node_ids)), dof]
+
+    def xyz_by_rank(self, model):
+        return np.ascontiguousarray(model.xyz[self.node_index], dtype=np.float64)
+
+
+def _group_arrays(model, num, g):
+    order = np.argsort(g.elem_ids, kind="stable")
+    conn = num.rank(g.conn[order].ravel()).reshape(len(order), -1)
+    dofs = ELEM_DOFS[g.kind]
+    edofs = num.rows[conn][:, :, list(dofs)].reshape(len(order), -1)
+    return order, conn.astype(np.int32), edofs.astype(np.int32)
+
+
+def _elements_in_id_order(model, num):
+    """All elements of all groups in ascending element id (map order)."""
+    items = []
+    for gi, g in enumerate(model.groups):
+        order, conn, edofs = _group_arrays(model, num, g)
+        items.append((gi, g.elem_ids[order], conn, edofs))
+    return items
+
+
+def _fsi_arrays(model, num):
+    if model.if_ids is None or len(model.if_ids) == 0:
+        return None
+    order = np.argsort(model.if_ids, kind="stable")
+    fn = num.rank(model.if_fluid_nodes[order].ravel()).reshape(-1, 4)
+    sn = num.rank(model.if_struct_nodes[order].ravel()).reshape(-1, 4)
+    frows = num.rows[fn, 16].astype(np.int32)
+    srows = num.rows[sn][:, :, [0, 1, 2]].reshape(-1, 12).astype(np.int32)
+    ori = np.ascontiguousarray(model.if_ori[order], dtype=np.float64)
+    return fn.astype(np.int32), srows, frows, ori
+
+
+def pattern(model, num=None):
+    """getLocalSparsityPattern + preallocateMatrix (analysis.cpp:783-867)"""
+    num = num or Numbering(model)
+    L = lib()
+    rows_list, cols_list = [], []
+    fsi = _fsi_arrays(model, num)
+    if fsi is not None:
+        _, srows, frows, _ = fsi
+        for e in range(len(srows)):
+            r, c = np.meshgrid(srows[e], frows[e], indexing="ij")
+            rows_list += [r.ravel(), c.ravel()]
+            cols_list += [c.ravel(), r.ravel()]
+    er = np.concatenate(rows_list).astype(np.int32) if rows_list else np.zeros(0, np.int32)
+    ec = np.concatenate(cols_list).astype(np.int32) if cols_list else np.zeros(0, np.int32)
+    # element cliques of all groups: concatenate groups of equal clique size
+    groups = {}
+    for g in model.groups:
+        _, _, edofs = _group_arrays(model, num, g)
+        groups.setdefault(edofs.shape[1], []).append(edofs)
+    if len(groups) == 1 and er.size == 0:
+        (ndpe, lst), = groups.items()
+        ed = np.ascontiguousarray(np.concatenate(lst))
+        rp = np.zeros(num.n + 1, dtype=np.int32)
+        nnz = L.orc_pattern(num.n, len(ed), ndpe, ed, 0, er, ec, rp, None)
+        ci = np.zeros(nnz, dtype=np.int32)
+        L.orc_pattern(num.n, len(ed), ndpe, ed, 0, er, ec, rp, ci.ctypes.data)
+        return rp, ci
+    # mixed clique sizes: expand cliques into explicit (row, col) pairs
+    for ndpe, lst in groups.items():
+        ed = np.concatenate(lst)
+        r = np.repeat(ed, ndpe, axis=1).ravel()
+        c = np.tile(ed, (1, ndpe)).ravel()
+        er = np.concatenate([er, r.astype(np.int32)])
+        ec = np.concatenate([ec, c.astype(np.int32)])
+    rp = np.zeros(num.n + 1, dtype=np.int32)
+    dummy = np.zeros(1, dtype=np.int32)
+    nnz = L.orc_pattern(num.n, 0, 1, dummy, len(er), er, ec, rp, None)
+    ci = np.zeros(nnz, dtype=np.int32)
+    L.orc_pattern(num.n, 0, 1, dummy, len(er), er, ec, rp, ci.ctypes.data)
+    return rp, ci
+
+
+def _params(model, g):
+    m = model.materials[g.material_id]
+    if g.kind in (FLUID8, FLUID20):
+        return m.rho, m.c, 0.0
+    return m.E, m.nu, m.rho
+
+
+class Assembled:
+    """Frequency-independent global matrices (the restructured form the GPU
+    path stores): K, M (fluid rows pre-divided by rho), the Robin face matrix
+    C_G, the FSI block S (with -ori), Rayleigh C = a K + b M."""
+
+
+def assemble_km(model, num=None, pat=None):
+    num = num or Numbering(model)
+    rp, ci = pat if pat is not None else pattern(model, num)
+    L = lib()
+    xyz = num.xyz_by_rank(model)
+    K = np.zeros(len(ci))
+    M = np.zeros(len(ci))
+    # elements must be added in ascending element id across groups; groups
+    # here hold disjoint, increasing id ranges, so group order == id order
+    for g in sorted(model.groups, key=lambda g: int(np.min(g.elem_ids))):
+        _, conn, edofs = _group_arrays(model, num, g)
+        p0, p1, p2 = _params(model, g)
+        rc = L.orc_assemble_km(g.kind, len(conn), conn, xyz, edofs, p0, p1, p2, 1, rp, ci,
+                               K.ctypes.data, M.ctypes.data)
+        if rc:
+            raise ValueError(f"detJ <= 0 in element index {-rc - 1}")
+    out = Assembled()
+    out.num, out.rowptr, out.colind, out.K, out.M = num, rp, ci, K, M
+    a, b = getattr(model, "rayleigh", (0.0, 0.0))
+    out.C = a * K + b * M
+    out.CG = face_matrix(model, num, rp, ci)
+    out.S = fsi_matrix(model, num, rp, ci)
+    return out
+
+
+def _imp_faces(model, num):
+    """(nfn, face node ranks [F, nfn], face rows [F, nfn])"""
+    if model.imp_elem_ids is None or len(model.imp_elem_ids) == 0:
+        return None
+    eid_to = {}
+    for g in model.groups:
+        for k, e in enumerate(g.elem_ids):
+            eid_to[int(e)] = (g, k)
+    fn_list = []
+    nfn = None
+    for e, f in zip(model.imp_elem_ids, model.imp_faces):
+        g, k = eid_to[int(e)]
+        nnod = NPE[g.kind]
+        idx = face_nodes(nnod, int(f) - 1)
+        nfn = len(idx)
+        fn_list.append(num.rank(g.conn[k][idx]))
+    fn = np.array(fn_list, dtype=np.int32)
+    rows = num.rows[fn, 16].astype(np.int32)
+    return nfn, fn, rows
+
+
+def face_matrix(model, num, rp, ci):
+    CG = np.zeros(len(ci))
+    f = _imp_faces(model, num)
+    if f is None:
+        return CG
+    nfn, fn, rows = f
+    rc = lib().orc_assemble_face(nfn, len(fn), np.ascontiguousarray(fn), num.xyz_by_rank(model),
+                                 np.ascontiguousarray(rows), int(bool(model.imp_lumped)), rp, ci, CG)
+    if rc:
+        raise ValueError("face detJ <= 0")
+    return CG
+
+
+def fsi_matrix(model, num, rp, ci):
+    S = np.zeros(len(ci))
+    fsi = _fsi_arrays(model, num)
+    if fsi is None:
+        return S
+    fn, srows, frows, ori = fsi
+    rc = lib().orc_assemble_fsi(len(fn), np.ascontiguousarray(fn), num.xyz_by_rank(model),
+                                srows, frows, ori, rp, ci, S)
+    if rc:
+        raise ValueError("FSI entry outside the pattern")
+    return S
+
+
+# ---------------------------------------------------------------------------
+# elPaSo's per-frequency system (analysisfrequencybasic.cpp:101-182)
+# ---------------------------------------------------------------------------
+def dynamic_matrix(model, omega, num=None, pat=None, Z=None):
+    """K_dyn(w) as elPaSo builds it: every element re-integrated at w
+    (assembleGlobalTensors), FSI added (addContributionFrequencyDomain), rows
+    of Dirichlet DOFs zeroed with 1 on the diagonal (MatZeroRows,
+    analysis.cpp:419-454).  Returns (rowptr, colind, complex values)."""
+    num = num or Numbering(model)
+    rp, ci = pat if pat is not None else pattern(model, num)
+    L = lib()
+    xyz = num.xyz_by_rank(model)
+    A = np.zeros(len(ci), dtype=np.complex128)
+    Zv = model.impedance if Z is None else Z
+    imp = None
+    if model.imp_elem_ids is not None and len(model.imp_elem_ids):
+        imp = {}
+        for e, f in zip(model.imp_elem_ids, model.imp_faces):
+            imp[int(e)] = imp.get(int(e), 0) | (1 << (int(f) - 1))
+    for g in sorted(model.groups, key=lambda g: int(np.min(g.elem_ids))):
+        order, conn, edofs = _group_arrays(model, num, g)
+        p0, p1, p2 = _params(model, g)
+        mat = model.materials[g.material_id]
+        Evis = 0.0
+        if g.kind in (BRICK8, BRICK20):
+            Evis = mat.E * (1 + 1j * (omega if mat.visco_type == 2 else 1.0) * mat.eta) if mat.eta else mat.E
+        mask = None
+        if imp is not None and g.kind in (FLUID8, FLUID20):
+            ids = g.elem_ids[order]
+            mask = np.array([imp.get(int(e), 0) for e in ids], dtype=np.uint8)
+        rc = L.orc_assemble_dynamic(g.kind, len(conn), conn, xyz, edofs, p0, p1, p2, _cx(Evis),
+                                    omega, None if mask is None else mask.ctypes.data,
+                                    _cx(Zv), int(bool(model.imp_lumped)), rp, ci, A)
+        if rc:
+            raise ValueError(f"detJ <= 0 in element index {-rc - 1}")
+    S = fsi_matrix(model, num, rp, ci)
+    A += S
+    F = rhs(model, num)
+    apply_dirichlet(model, num, rp, ci, A, F)
+    return rp, ci, A, F
+
+
+def rhs(model, num):
+    """applyNodalForces (analysis.cpp:171-279): F += F_i (structure),
+    F += 4 pi Q factor^2 (fluid, factor = 1)."""
+    F = np.zeros(num.n, dtype=np.complex128)
+    for nid, Q in model.sources:
+        F[num.row([nid], 16)[0]] += 4.0 * np.pi * Q * 1.0 * 1.0
+    for nid, fx, fy, fz in model.forces:
+        for d, v in ((0, fx), (1, fy), (2, fz)):
+            r = num.row([nid], d)[0]
+            if r >= 0:
+                F[r] += v
+    return F
+
+
+def apply_dirichlet(model, num, rp, ci, A, F):
+    """MatZeroRows(diag = 1) + F[row] = value (analysis.cpp:419-454)"""
+    for nid, dof, val in model.dirichlet:
+        r = num.row([nid], dof)[0]
+        if r < 0:
+            continue
+        A[rp[r]:rp[r + 1]] = 0.0
+        A[rp[r] + np.searchsorted(ci[rp[r]:rp[r + 1]], r)] = 1.0
+        F[r] = val
+
+
+def csr(rp, ci, vals, n):
+    return sp.csr_matrix((vals, ci, rp), shape=(n, n))
+
+
+def solve_direct(model, freqs, num=None, pat=None, Z=None, timings=None):
+    """Per-frequency elPaSo flow with SuperLU standing in for MUMPS:
+    re-integrate + scatter + BCs, then a direct LU solve.  Returns X [nf, n]."""
+    import time
+    from scipy.sparse.linalg import splu
+
+    num = num or Numbering(model)
+    pat = pat if pat is not None else pattern(model, num)
+    X = np.zeros((len(freqs), num.n), dtype=np.complex128)
+    for k, f in enumerate(freqs):
+        t0 = time.perf_counter()
+        rp, ci, A, F = dynamic_matrix(model, 2 * np.pi * f, num, pat, Z)
+        t1 = time.perf_counter()
+        lu = splu(csr(rp, ci, A, num.n).tocsc())
+        X[k] = lu.solve(F)
+        t2 = time.perf_counter()
+        if timings is not None:
+            timings.append((t1 - t0, t2 - t1))
+    return X

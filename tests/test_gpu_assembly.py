@@ -1,0 +1,152 @@
+"""GPU parity of the assembly path against the CPU oracle (elPaSo restatement).
+
+Bar: FP64 matrices agree to 1e-12 relative (normwise and per entry relative to
+the largest entry of its row); patterns identical; repeated runs bit-identical.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from elpaso_amd import BRICK8, BRICK20, FLUID8, FLUID20, GpuModel
+from elpaso_amd import _lib
+from elpaso_amd import meshgen as MG
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _check_close(name, got, ref, rp):
+    scale = np.abs(ref).max()
+    assert np.linalg.norm(got - ref) <= TOL * np.linalg.norm(ref) + 1e-300, name
+    # per entry relative to the row's largest entry
+    rowmax = np.maximum.reduceat(np.abs(ref), rp[:-1]) if len(ref) else np.zeros(0)
+    rowmax = np.repeat(rowmax, np.diff(rp))
+    bad = np.abs(got - ref) > TOL * np.maximum(rowmax, 1e-300 * scale)
+    assert not bad.any(), f"{name}: {bad.sum()} entries off, max {np.abs(got - ref).max()}"
+
+
+def _parity(ctx, model, check=(_lib.MAT_K, _lib.MAT_M)):
+    print(f"[parity] {model.name}", flush=True)
+    g = GpuModel(ctx, model)
+    g.assemble()
+    num = O.Numbering(model)
+    rp, ci = O.pattern(model, num)
+    grp, gci = g.pattern()
+    assert np.array_equal(rp, grp) and np.array_equal(ci, gci), "pattern differs"
+    ref = O.assemble_km(model, num, (rp, ci))
+    refs = {_lib.MAT_K: ref.K, _lib.MAT_M: ref.M, _lib.MAT_C: ref.C, _lib.MAT_G: ref.CG, _lib.MAT_S: ref.S}
+    for w in check:
+        _check_close(f"mat{w}", g.values(w), refs[w], rp)
+    return g, num, (rp, ci), ref
+
+
+def test_fluid8_cavity_config0(ctx):
+    m = MG.acoustic_cavity()
+    g, num, pat, ref = _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_G))
+    ids, dofs = g.dof_map()
+    assert (dofs == 16).all() and np.all(np.diff(ids) > 0)
+
+
+def test_fluid20_cavity_small(ctx):
+    m = MG.acoustic_cavity(6, 5, 4, quadratic=True)
+    _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_G))
+
+
+def test_fluid8_consistent_face(ctx):
+    m = MG.acoustic_cavity(5, 4, 3, lumped=False)
+    _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_G))
+
+
+def test_brick8_jittered_rayleigh(ctx):
+    m = MG.steel_block(8, 7, 3, L=(0.04, 0.035, 0.015), jitter=0.1, seed=3)
+    _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_C))
+
+
+def test_brick20_block(ctx):
+    nid, xyz, eid, conn, _ = MG.box(3, 2, 2, (0.03, 0.02, 0.02), quadratic=True, jitter=0.05, seed=1)
+    m = MG.Model(node_ids=nid, xyz=xyz, name="brick20")
+    m.materials[1] = MG.STEEL
+    m.groups.append(MG.ElementGroup(BRICK20, 1, eid, conn))
+    _parity(ctx, m)
+
+
+def test_coupled_small(ctx):
+    m = MG.coupled_plate_cavity(n_xy=4, n_plate_z=2, n_cav_z=3)
+    _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_S))
+
+
+def test_shuffled_ids_and_groups(ctx):
+    """Node/element ids out of order and split groups: numbering follows ids."""
+    m = MG.acoustic_cavity(4, 3, 3)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(len(m.node_ids))
+    newid = rng.choice(np.arange(1, 10 * len(m.node_ids)), size=len(m.node_ids), replace=False)
+    idmap = dict(zip(m.node_ids.tolist(), newid.tolist()))
+    m.node_ids = newid[perm]
+    m.xyz = m.xyz[perm]
+    g0 = m.groups[0]
+    conn = np.vectorize(idmap.get)(g0.conn)
+    half = len(g0.elem_ids) // 2
+    eids = rng.permutation(np.arange(100, 100 + len(g0.elem_ids)))
+    emap = dict(zip(g0.elem_ids.tolist(), eids.tolist()))
+    m.groups = [MG.ElementGroup(FLUID8, 1, eids[half:], conn[half:]),
+                MG.ElementGroup(FLUID8, 1, eids[:half], conn[:half])]
+    m.imp_elem_ids = np.array([emap[int(e)] for e in m.imp_elem_ids])
+    m.sources = [(idmap[m.sources[0][0]], 1.0)]
+    m.box = None
+    _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_G))
+
+
+def test_repeat_bit_identical(ctx):
+    m = MG.steel_block(10, 9, 3, L=(0.05, 0.045, 0.015), jitter=0.1, seed=2)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    k1, m1 = g.values(_lib.MAT_K), g.values(_lib.MAT_M)
+    g.assemble()
+    k2, m2 = g.values(_lib.MAT_K), g.values(_lib.MAT_M)
+    assert np.array_equal(k1.view(np.uint64), k2.view(np.uint64))
+    assert np.array_equal(m1.view(np.uint64), m2.view(np.uint64))
+
+
+def test_element_operators_match_oracle(ctx):
+    rng = np.random.default_rng(7)
+    for kind, npe in ((FLUID8, 8), (FLUID20, 20), (BRICK8, 8), (BRICK20, 20)):
+        nat = MG.HEX8_NAT if npe == 8 else MG.HEX20_NAT
+        xyz = (nat[None] * 0.5 + 0.5) * np.array([0.1, 0.07, 0.05]) + rng.uniform(-0.004, 0.004, (5, npe, 3))
+        if kind in (FLUID8, FLUID20):
+            Ke, Me = ctx.element_matrices(kind, xyz, 1.21, 343.0)
+            for e in range(5):
+                K, M = O.fluid_km(xyz[e], 343.0)
+                assert np.abs(Ke[e] - K).max() <= TOL * np.abs(K).max()
+                assert np.abs(Me[e] - M).max() <= TOL * np.abs(M).max()
+        else:
+            Ke, Me = ctx.element_matrices(kind, xyz, 210e9, 0.3, 7850.0)
+            for e in range(5):
+                K, M = O.brick_km(xyz[e])
+                assert np.abs(Ke[e] - K.real).max() <= TOL * np.abs(K).max()
+                M3 = np.kron(Me[e], np.eye(3))
+                assert np.abs(M3 - M).max() <= TOL * np.abs(M).max()
+
+
+def test_inverted_element_rejected(ctx):
+    m = MG.acoustic_cavity(2, 2, 2)
+    m.groups[0].conn[0] = m.groups[0].conn[0][[4, 5, 6, 7, 0, 1, 2, 3]]  # mirrored
+    g = GpuModel(ctx, m)
+    with pytest.raises(_lib.ElpError) as e:
+        g.assemble()
+    assert e.value.code == -4
+
+
+def test_dynamic_matrix_matches_elpaso_flow(ctx):
+    """A(w) in PETSc AIJ layout vs the oracle's per-frequency re-integration."""
+    for m in (MG.acoustic_cavity(), MG.coupled_plate_cavity(n_xy=4, n_plate_z=2, n_cav_z=3)):
+        g = GpuModel(ctx, m)
+        g.assemble()
+        num = O.Numbering(m)
+        pat = O.pattern(m, num)
+        for f in (10.0, 237.3, 500.0):
+            w = 2 * np.pi * f
+            a, fv = g.dynamic_matrix(w)
+            rp, ci, A, F = O.dynamic_matrix(m, w, num, pat)
+            assert np.abs(a - A).max() <= 1e-12 * np.abs(A).max()
+            assert np.array_equal(fv, F)
