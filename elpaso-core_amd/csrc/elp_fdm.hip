@@ -1,0 +1,230 @@
+// Fast-diagonalization solve of the Hex8 box operator (see elp_fdm.h).
+//
+// 1-D uniform linear elements with natural (rigid-wall) ends are diagonalized
+// by cosine modes: K v = lambda M v with v_m(i) = cos(pi m i / N),
+// lambda_m = (6/h^2)(1 - cos t)/(2 + cos t), t = pi m / N.  The modes are
+// normalized so V^T M V = I.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "elp_fdm.h"
+
+namespace elp {
+
+struct FdmBox {
+  int dims[3];  // px, py, pz (nodes per direction)
+  double h[3];
+  double rho, c;
+  int robin_lo, robin_hi;  // Robin on z = z0 / z = z1
+  DArr<float> fx32, ix32, fy32, iy32;   // forward (V^T) and inverse (V) transforms
+  DArr<double> fx64, ix64, fy64, iy64;
+  DArr<double> lamx, lamy;
+};
+
+static void fdm_free(FdmBox *f) { delete f; }
+
+int64_t fdm_nodes(const FdmBox &f) { return (int64_t)f.dims[0] * f.dims[1] * f.dims[2]; }
+const int *fdm_dims(const FdmBox &f) { return f.dims; }
+double fdm_rho(const FdmBox &f) { return f.rho; }
+double fdm_c(const FdmBox &f) { return f.c; }
+
+// M-orthonormal cosine modes of a uniform 1-D mesh of n elements of size h
+static void modes_1d(int n, double h, std::vector<double> &V, std::vector<double> &lam) {
+  const int p = n + 1;
+  V.assign((size_t)p * p, 0.0);
+  lam.assign(p, 0.0);
+  for (int m = 0; m < p; m++) {
+    const double t = M_PI * m / n;
+    lam[m] = 6.0 / (h * h) * (1.0 - std::cos(t)) / (2.0 + std::cos(t));
+    std::vector<double> v(p);
+    for (int i = 0; i < p; i++) v[i] = std::cos(t * i);
+    // v^T M v with the consistent mass (h/6)[2 1; 1 2] per element
+    double nrm = 0.0;
+    for (int e = 0; e < n; e++) nrm += h / 6.0 * (2 * v[e] * v[e] + 2 * v[e] * v[e + 1] + 2 * v[e + 1] * v[e + 1]);
+    const double s = 1.0 / std::sqrt(nrm);
+    for (int i = 0; i < p; i++) V[(size_t)i * p + m] = v[i] * s;  // V[i][m]
+  }
+}
+
+std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m) {
+  std::unique_ptr<FdmBox, void (*)(FdmBox *)> none(nullptr, fdm_free);
+  if (!m.box.set || m.mats.size() != 1 || !m.mats.begin()->second.fluid) return none;
+  if (!m.dir_rows.empty()) return none;
+  for (auto v : m.row_field)
+    if (v != 1) return none;
+  if (m.box.robin_sides & ~((1 << 2) | (1 << 4))) return none;  // Robin only on z sides
+  // the model's corner nodes must form the uniform lattice of the box
+  auto f = std::unique_ptr<FdmBox, void (*)(FdmBox *)>(new FdmBox(), fdm_free);
+  for (int d = 0; d < 3; d++) {
+    f->dims[d] = m.box.nel[d] + 1;
+    f->h[d] = m.box.length[d] / m.box.nel[d];
+  }
+  f->rho = m.mats.begin()->second.rho;
+  f->c = m.mats.begin()->second.c;
+  f->robin_lo = (m.box.robin_sides >> 2) & 1;
+  f->robin_hi = (m.box.robin_sides >> 4) & 1;
+  Ctx *c = m.ctx;
+  auto upload = [&](int d, DArr<float> &fw32, DArr<float> &iv32, DArr<double> &fw64, DArr<double> &iv64,
+                    DArr<double> *lam) {
+    std::vector<double> V, L;
+    modes_1d(m.box.nel[d], f->h[d], V, L);
+    const int p = f->dims[d];
+    std::vector<double> Vt((size_t)p * p);
+    for (int i = 0; i < p; i++)
+      for (int k = 0; k < p; k++) Vt[(size_t)k * p + i] = V[(size_t)i * p + k];
+    iv64.upload(c, V);
+    fw64.upload(c, Vt);
+    std::vector<float> a(V.begin(), V.end()), b(Vt.begin(), Vt.end());
+    iv32.upload(c, a);
+    fw32.upload(c, b);
+    if (lam) lam->upload(c, L);
+  };
+  upload(0, f->fx32, f->ix32, f->fx64, f->ix64, &f->lamx);
+  upload(1, f->fy32, f->iy32, f->fy64, f->iy64, &f->lamy);
+  return f;
+}
+
+// out[o][p][c] = sum_l Mat[p][l] in[o][l][c], the tile in[o][:][c0:c0+TW] staged in LDS
+template <typename T>
+__global__ __launch_bounds__(256) void axis_transform_kernel(int L, int64_t inner, int tw, const T *__restrict__ Mat,
+                                                             const T *__restrict__ in, T *__restrict__ out) {
+  extern __shared__ unsigned char smem[];
+  T *tile = reinterpret_cast<T *>(smem);
+  const int64_t o = blockIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.y * tw;
+  const int w = (int)min((int64_t)tw, inner - c0);
+  const T *src = in + o * L * inner + c0;
+  for (int idx = threadIdx.x; idx < L * w; idx += blockDim.x) {
+    const int l = idx / w, cc = idx % w;
+    tile[l * tw + cc] = src[(int64_t)l * inner + cc];
+  }
+  __syncthreads();
+  T *dst = out + o * L * inner + c0;
+  for (int idx = threadIdx.x; idx < L * w; idx += blockDim.x) {
+    const int p = idx / w, cc = idx % w;
+    const T *mrow = Mat + (int64_t)p * L;
+    T acc = 0;
+    for (int l = 0; l < L; l++) acc += mrow[l] * tile[l * tw + cc];
+    dst[(int64_t)p * inner + cc] = acc;
+  }
+}
+
+template <typename V>
+__device__ inline V cdiv(V a, V b) {
+  const auto d = b.x * b.x + b.y * b.y;
+  V r;
+  r.x = (a.x * b.x + a.y * b.y) / d;
+  r.y = (a.y * b.x - a.x * b.y) / d;
+  return r;
+}
+
+// one complex tridiagonal z-system per (x-mode p, y-mode q, frequency f):
+//   T = (alpha/w^2 - sM) Mz + Kz/w^2 + tG Ez,  alpha = lamx[p] + lamy[q];
+// x = rho T^{-1} r (Thomas elimination; c' kept in cs)
+template <typename T, typename V>
+__global__ void zsolve_kernel(int px, int py, int pz, int B, double hz, int rlo, int rhi,
+                              const FdmCoef *__restrict__ coef, const double *__restrict__ lamx,
+                              const double *__restrict__ lamy, const V *__restrict__ r, V *__restrict__ x,
+                              V *__restrict__ cs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t plane = (int64_t)px * py * B;
+  if (t >= plane) return;
+  const int f = (int)(t % B);
+  const int64_t pq = t / B;
+  const int p = (int)(pq % px), q = (int)(pq / px);
+  const FdmCoef cf = coef[f];
+  const double a = (lamx[p] + lamy[q]) * cf.w2inv;
+  // coefficient of Mz: (a - sM) complex; of Kz: w2inv real
+  const double cmr = a - cf.sM.x, cmi = -cf.sM.y;
+  const double kd = 1.0 / hz, md = hz / 6.0;
+  // off-diagonal (same for all k): (cm * md) + w2inv * (-kd)
+  V off;
+  off.x = (T)(cmr * md - cf.w2inv * kd);
+  off.y = (T)(cmi * md);
+  V cprev = {0, 0}, dprev = {0, 0};
+  for (int k = 0; k < pz; k++) {
+    const bool end = (k == 0 || k == pz - 1);
+    const double mk = md * (end ? 2.0 : 4.0), kk = kd * (end ? 1.0 : 2.0);
+    V diag;
+    diag.x = (T)(cmr * mk + cf.w2inv * kk);
+    diag.y = (T)(cmi * mk);
+    if ((k == 0 && rlo) || (k == pz - 1 && rhi)) {
+      diag.x += (T)cf.tG.x;
+      diag.y += (T)cf.tG.y;
+    }
+    V m = diag;
+    if (k > 0) {
+      // m = diag - off * c'
+      m.x -= off.x * cprev.x - off.y * cprev.y;
+      m.y -= off.x * cprev.y + off.y * cprev.x;
+    }
+    V rv = r[k * plane + t];
+    if (k > 0) {
+      rv.x -= off.x * dprev.x - off.y * dprev.y;
+      rv.y -= off.x * dprev.y + off.y * dprev.x;
+    }
+    const V cn = cdiv(off, m), dn = cdiv(rv, m);
+    cs[k * plane + t] = cn;
+    x[k * plane + t] = dn;
+    cprev = cn;
+    dprev = dn;
+  }
+  V nxt = x[(int64_t)(pz - 1) * plane + t];
+  const T rho = (T)cf.rho;
+  {
+    V o = nxt;
+    o.x *= rho;
+    o.y *= rho;
+    x[(int64_t)(pz - 1) * plane + t] = o;
+  }
+  for (int k = pz - 2; k >= 0; k--) {
+    const V ck = cs[k * plane + t];
+    V d = x[k * plane + t];
+    d.x -= ck.x * nxt.x - ck.y * nxt.y;
+    d.y -= ck.x * nxt.y + ck.y * nxt.x;
+    nxt = d;
+    V o = d;
+    o.x *= rho;
+    o.y *= rho;
+    x[k * plane + t] = o;
+  }
+}
+
+template <typename T, typename V>
+static void transform(int64_t outer, int L, int64_t inner, const T *Mat, const V *in, V *out, hipStream_t s) {
+  // inner counts reals
+  int tw = (int)std::min<int64_t>(inner, 64);
+  while ((int64_t)L * tw * sizeof(T) > 48 * 1024 && tw > 1) tw /= 2;
+  dim3 grid((unsigned)outer, (unsigned)((inner + tw - 1) / tw));
+  hipLaunchKernelGGL(axis_transform_kernel<T>, grid, dim3(256), (size_t)L * tw * sizeof(T), s, L, inner, tw, Mat,
+                     reinterpret_cast<const T *>(in), reinterpret_cast<T *>(out));
+  ELP_HIP(hipGetLastError());
+}
+
+template <typename T, typename V>
+static void solve_impl(const FdmBox &f, int B, const FdmCoef *coef, const V *b, V *x, V *s1, V *s2, const T *fx,
+                       const T *ix, const T *fy, const T *iy, hipStream_t s) {
+  const int px = f.dims[0], py = f.dims[1], pz = f.dims[2];
+  // x-modes: mix the fastest index (blocks of px x B complex)
+  transform<T, V>((int64_t)py * pz, px, 2LL * B, fx, b, s1, s);
+  // y-modes: per z-plane, mix j over rows of px x B complex
+  transform<T, V>((int64_t)pz, py, 2LL * B * px, fy, s1, s2, s);
+  const int64_t plane = (int64_t)px * py * B;
+  hipLaunchKernelGGL((zsolve_kernel<T, V>), dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, px, py, pz, B,
+                     f.h[2], f.robin_lo, f.robin_hi, coef, f.lamx.p, f.lamy.p, s2, s1, x);
+  ELP_HIP(hipGetLastError());
+  transform<T, V>((int64_t)pz, py, 2LL * B * px, iy, s1, s2, s);
+  transform<T, V>((int64_t)py * pz, px, 2LL * B, ix, s2, x, s);
+}
+
+void fdm_solve(const FdmBox &f, int B, const FdmCoef *coef, const float2 *b, float2 *x, float2 *s1, float2 *s2,
+               hipStream_t s) {
+  solve_impl<float, float2>(f, B, coef, b, x, s1, s2, f.fx32.p, f.ix32.p, f.fy32.p, f.iy32.p, s);
+}
+void fdm_solve(const FdmBox &f, int B, const FdmCoef *coef, const double2 *b, double2 *x, double2 *s1, double2 *s2,
+               hipStream_t s) {
+  solve_impl<double, double2>(f, B, coef, b, x, s1, s2, f.fx64.p, f.ix64.p, f.fy64.p, f.iy64.p, s);
+}
+
+}  // namespace elp

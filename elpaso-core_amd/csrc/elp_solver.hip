@@ -31,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 
+#include "elp_fdm.h"
 #include "elp_internal.h"
 #include "elp_solver.h"
 
@@ -65,25 +66,52 @@ struct OpArgs {
   const double *rK, *rM, *rG;
 };
 
-// Fused multi-frequency operator.  Each row is handled by G = max(B, 8)
-// lanes: lane l works on frequency l % B and on the nonzeros with index
-// (l / B) mod (G / B); partial sums are combined with a fixed xor tree, so the
-// result is deterministic.
+// Fused multi-frequency operator.  Each row is handled by a group of
+// G = max(B, 8) lanes; lane l works on frequency f = l % B.  The row is walked
+// in chunks of CH nonzeros: the group loads the chunk's column indices and K,
+// M values with one coalesced load per lane into LDS, then every lane gathers
+// x[col][f] for the chunk's entries it owns (entries j = sub, sub + S, ...,
+// S = G / B) with independent loads, so the gathers of a chunk are in flight
+// together.  Partial sums of the S subsets are combined with a fixed xor tree:
+// the result is deterministic.
 //   MODE 0: y = A x
 //   MODE 1: y = b - A x
 //   MODE 2: y = x + (b - A x) / D      (one l1-Jacobi sweep)
 //   MODE 3: y = b / D                  (first sweep from a zero guess)
+// Workgroups are dealt round-robin to the 8 XCDs of the MI355X, each with its
+// own L2.  Launches use a multiple of 8 workgroups and remap them so that each
+// XCD walks one contiguous range of rows: neighbouring rows share most of
+// their columns, so the gathers of x hit in that XCD's L2.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block() {
+  const int per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+static int xcd_grid(int64_t blocks) { return (int)((blocks + kXcds - 1) / kXcds * kXcds); }
+
+template <int B> struct OpShape {
+  static constexpr int G = B < 8 ? 8 : B;    // lanes per row
+  static constexpr int S = G / B;            // nonzero subsets per frequency
+  static constexpr int U = B < 8 ? 4 : 1;    // chunk = U entries per lane
+  static constexpr int CH = G * U;           // nonzeros per chunk
+  static constexpr int RPB = 256 / G;        // rows per block
+};
+
 template <typename T, int B, int MODE>
 __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__restrict__ coef,
                                                  const typename VT<T>::t *__restrict__ x,
                                                  const typename VT<T>::t *__restrict__ b,
                                                  typename VT<T>::t *__restrict__ y) {
   using V = typename VT<T>::t;
-  constexpr int G = B < 8 ? 8 : B;
-  constexpr int S = G / B;
+  using SH = OpShape<B>;
+  constexpr int G = SH::G, S = SH::S, U = SH::U, CH = SH::CH;
+  __shared__ int32_t s_col[SH::RPB][CH];
+  __shared__ T s_k[SH::RPB][CH];
+  __shared__ T s_m[SH::RPB][CH];
+  const int grp = threadIdx.x / G;
   const int lane = threadIdx.x % G;
-  const int64_t r = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
-  if (r >= a.n) return;
+  const int64_t r = (int64_t)xcd_block() * SH::RPB + grp;
+  if (r >= a.n) return;  // the whole group leaves together
   const int f = lane % B, sub = lane / B;
   const uint8_t fld = a.field[r];
   const OpCoef &cf = coef[f];
@@ -98,13 +126,31 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
   }
   V accK = {0, 0}, accM = {0, 0}, accX = {0, 0};
   if (MODE != 3) {
-    const int32_t p1 = a.rowptr[r + 1];
-    for (int32_t p = a.rowptr[r] + sub; p < p1; p += S) {
-      const int64_t c = a.colind[p];
-      const T kv = a.K[p], mv = a.M[p];
-      const V xv = x[c * B + f];
-      accK.x += kv * xv.x; accK.y += kv * xv.y;
-      accM.x += mv * xv.x; accM.y += mv * xv.y;
+    const int32_t p0 = a.rowptr[r], p1 = a.rowptr[r + 1];
+    for (int32_t base = p0; base < p1; base += CH) {
+      const int cnt = min(CH, p1 - base);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = u * G + lane;
+        if (j < cnt) {
+          s_col[grp][j] = a.colind[base + j];
+          s_k[grp][j] = a.K[base + j];
+          s_m[grp][j] = a.M[base + j];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 8
+      for (int j = sub; j < cnt; j += S) {
+        const V xv = x[(int64_t)s_col[grp][j] * B + f];
+        const T kv = s_k[grp][j], mv = s_m[grp][j];
+        accK.x += kv * xv.x; accK.y += kv * xv.y;
+        accM.x += mv * xv.x; accM.y += mv * xv.y;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 #pragma unroll
     for (int o = B; o < G; o <<= 1) {
@@ -158,24 +204,48 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
   y[o] = t;
 }
 
-// y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P
+// y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P (same
+// chunked row walk as the operator)
 template <typename T, int B, bool ADD>
 __global__ __launch_bounds__(256) void transfer_kernel(int64_t n, const int32_t *__restrict__ rp,
                                                        const int32_t *__restrict__ ci, const T *__restrict__ v,
                                                        const typename VT<T>::t *__restrict__ x,
                                                        typename VT<T>::t *__restrict__ y) {
   using V = typename VT<T>::t;
-  constexpr int G = B < 8 ? 8 : B;
-  constexpr int S = G / B;
+  using SH = OpShape<B>;
+  constexpr int G = SH::G, S = SH::S, U = SH::U, CH = SH::CH;
+  __shared__ int32_t s_col[SH::RPB][CH];
+  __shared__ T s_v[SH::RPB][CH];
+  const int grp = threadIdx.x / G;
   const int lane = threadIdx.x % G;
-  const int64_t r = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int64_t r = (int64_t)xcd_block() * SH::RPB + grp;
   if (r >= n) return;
   const int f = lane % B, sub = lane / B;
   V acc = {0, 0};
-  for (int32_t p = rp[r] + sub; p < rp[r + 1]; p += S) {
-    const V xv = x[(int64_t)ci[p] * B + f];
-    acc.x += v[p] * xv.x;
-    acc.y += v[p] * xv.y;
+  const int32_t p0 = rp[r], p1 = rp[r + 1];
+  for (int32_t base = p0; base < p1; base += CH) {
+    const int cnt = min(CH, p1 - base);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = u * G + lane;
+      if (j < cnt) {
+        s_col[grp][j] = ci[base + j];
+        s_v[grp][j] = v[base + j];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 8
+    for (int j = sub; j < cnt; j += S) {
+      const V xv = x[(int64_t)s_col[grp][j] * B + f];
+      const T w = s_v[grp][j];
+      acc.x += w * xv.x;
+      acc.y += w * xv.y;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 #pragma unroll
   for (int o = B; o < G; o <<= 1) {
@@ -412,6 +482,10 @@ struct elp::Model::SolverState {
   std::vector<std::unique_ptr<LevelData>> levels;
   bool mg = false;
   int mg_levels = 0, mg_min_nodes = 0;
+  // exact separable Hex8 solve on box meshes (used instead of the
+  // rediscretized hierarchy when available)
+  std::unique_ptr<elp::FdmBox, void (*)(elp::FdmBox *)> fdm{nullptr, nullptr};
+  int order = 1;
 };
 
 namespace elp {
@@ -425,8 +499,8 @@ void default_opts(elp_solve_opts &o) {
   o.max_iters = 4000;
   o.inner_rtol = 1e-4;
   o.restart = 50;
-  o.shift = 0.5;
-  o.smooth = 2;
+  o.shift = -1.0;  // automatic: 0 with the exact box coarse solve, 0.5 otherwise
+  o.smooth = 1;
   o.coarse_sweeps = 20;
   o.mg_levels = 0;
   o.mg_min_nodes = 2000;
@@ -598,7 +672,14 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   L0->model = &m;
   prepare_level(*L0, s);
   st->levels.push_back(std::move(L0));
-  if (want_mg) {
+  if (want_mg) st->fdm = fdm_build(m);
+  st->order = m.box.order;
+  if (st->fdm) {
+    // prolongation from the Hex8 lattice of the box to the model's rows
+    Model lattice;
+    lattice.box = m.box;
+    build_interp(*st->levels[0], lattice, m.box.nel);
+  } else if (want_mg) {
     int nel[3] = {m.box.nel[0], m.box.nel[1], m.box.nel[2]};
     bool first = true;
     const int max_levels = o.mg_levels > 0 ? o.mg_levels : 8;
@@ -651,6 +732,8 @@ struct Solver {
   DArr<double2> red_part, red_out, rho, rho_new, mu, alpha, beta;
   DArr<int> active;
   DArr<V> xi, ri, zi, pi, qi;    // inner Krylov vectors
+  DArr<V> fb, fe, fs1, fs2;      // exact box solve: rhs, solution, scratch (lattice size)
+  DArr<FdmCoef> coefF;
   double2 *h_red = nullptr;     // pinned
   // statistics
   hipEvent_t ev0, ev1;
@@ -679,6 +762,11 @@ struct Solver {
     red_out.alloc(ctx, B);
     rho.alloc(ctx, B); rho_new.alloc(ctx, B); mu.alloc(ctx, B); alpha.alloc(ctx, B); beta.alloc(ctx, B);
     active.alloc(ctx, B);
+    if (st.fdm) {
+      const int64_t nf = fdm_nodes(*st.fdm) * B;
+      fb.alloc(ctx, nf); fe.alloc(ctx, nf); fs1.alloc(ctx, nf); fs2.alloc(ctx, nf);
+      coefF.alloc(ctx, B);
+    }
     ELP_HIP(hipHostMalloc((void **)&h_red, sizeof(double2) * B, hipHostMallocDefault));
     ELP_HIP(hipEventCreate(&ev0));
     ELP_HIP(hipEventCreate(&ev1));
@@ -692,16 +780,15 @@ struct Solver {
   template <int MODE, typename TT = T>
   void op(int l, const OpCoef *coef, const typename VT<TT>::t *x, const typename VT<TT>::t *b,
           typename VT<TT>::t *y) {
-    constexpr int G = B < 8 ? 8 : B;
     const LevelData &L = *st.levels[l];
-    const int grid = nblk(L.n, 256 / G);
+    const int grid = xcd_grid(nblk(L.n, OpShape<B>::RPB));
     hipLaunchKernelGGL((op_kernel<TT, B, MODE>), dim3(grid), dim3(256), 0, s, L.template args<TT>(), coef, x, b, y);
     ELP_HIP(hipGetLastError());
   }
   template <bool ADD>
   void transfer(int64_t n, const int32_t *rp, const int32_t *ci, const T *v, const V *x, V *y) {
-    constexpr int G = B < 8 ? 8 : B;
-    hipLaunchKernelGGL((transfer_kernel<T, B, ADD>), dim3(nblk(n, 256 / G)), dim3(256), 0, s, n, rp, ci, v, x, y);
+    hipLaunchKernelGGL((transfer_kernel<T, B, ADD>), dim3(xcd_grid(nblk(n, OpShape<B>::RPB))), dim3(256), 0, s, n, rp,
+                       ci, v, x, y);
     ELP_HIP(hipGetLastError());
   }
   const T *Pv(const LevelData &L) const;
@@ -750,8 +837,31 @@ struct Solver {
     smooth(l, b, W.x.p, nu, z);
   }
 
+  // two-level cycle with the exact separable solve on the Hex8 lattice of the
+  // box: Hex20 models smooth on their own level around it, Hex8 models use it
+  // directly (it is their operator up to the Robin lumping)
+  void box_cycle(const V *b, V *z) {
+    const LevelData &L = *st.levels[0];
+    LevelWork<T> &W = lw[0];
+    const int64_t nf = fdm_nodes(*st.fdm);
+    if (st.order == 1) {
+      transfer<false>(nf, L.R_rp.p, L.R_ci.p, Rv(L), b, fb.p);
+      fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
+      transfer<false>(L.n, L.P_rp.p, L.P_ci.p, Pv(L), fe.p, z);
+      return;
+    }
+    const int nu = std::max(1, o.smooth);
+    smooth(0, b, nullptr, nu, W.x.p);
+    op<1>(0, coefP.p, W.x.p, b, W.r.p);
+    transfer<false>(nf, L.R_rp.p, L.R_ci.p, Rv(L), W.r.p, fb.p);
+    fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
+    transfer<true>(L.n, L.P_rp.p, L.P_ci.p, Pv(L), fe.p, W.x.p);
+    smooth(0, b, W.x.p, nu, z);
+  }
+
   void precond(const V *r, V *z) {
-    if (st.levels.size() == 1) smooth(0, r, nullptr, std::max(1, o.smooth), z);
+    if (st.fdm) box_cycle(r, z);
+    else if (st.levels.size() == 1) smooth(0, r, nullptr, std::max(1, o.smooth), z);
     else vcycle(0, r, z);
   }
 
@@ -853,13 +963,22 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   const elp_solve_opts &o = S.o;
   hipStream_t s = S.s;
   std::vector<OpCoef> cA(B), cP(B);
+  std::vector<FdmCoef> cF(B);
+  const double beta = o.shift >= 0 ? o.shift : (S.st.fdm ? 0.0 : 0.5);
   for (int f = 0; f < B; f++) {
     const double w = 2.0 * M_PI * freqs[std::min(f, nb - 1)];
     cA[f] = make_coef(m, w, Z, 0.0);
-    cP[f] = make_coef(m, w, Z, o.shift);
+    cP[f] = make_coef(m, w, Z, beta);
+    if (S.st.fdm) {
+      const double rho = fdm_rho(*S.st.fdm), c = fdm_c(*S.st.fdm);
+      const std::complex<double> sM = (1.0 - std::complex<double>(0, beta)) / (c * c);
+      const std::complex<double> tG = std::abs(Z) > 0 ? std::complex<double>(0, rho) / (w * Z) : 0.0;
+      cF[f] = {1.0 / (w * w), {sM.real(), sM.imag()}, {tG.real(), tG.imag()}, rho};
+    }
   }
   ELP_HIP(hipMemcpyAsync(S.coefA.p, cA.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
   ELP_HIP(hipMemcpyAsync(S.coefP.p, cP.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
+  if (S.st.fdm) ELP_HIP(hipMemcpyAsync(S.coefF.p, cF.data(), sizeof(FdmCoef) * B, hipMemcpyHostToDevice, s));
   // FP64 outer vectors
   const int64_t nt = m.n * B;
   DArr<double2> F, b64, x64, r64;

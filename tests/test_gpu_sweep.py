@@ -43,13 +43,27 @@ def test_config0_sweep(ctx, prec, pc):
     _check_solution(m, freqs, X, rel)
 
 
-def test_hex20_sweep_small(ctx):
+@pytest.mark.parametrize("pc", [_lib.PC_MG, _lib.PC_JACOBI])
+def test_hex20_sweep_small(ctx, pc):
     m = MG.acoustic_cavity(20, 16, 12, quadratic=True)
     g = GpuModel(ctx, m)
     g.assemble()
-    freqs = np.array([50.0, 400.0, 1200.0, 2000.0])
-    X, rel, it, st = g.sweep(freqs, block=4)
-    assert (rel <= 1e-8).all(), rel
+    freqs = np.array([50.0, 400.0, 1200.0, 2000.0]) if pc == _lib.PC_MG else np.array([50.0, 300.0])
+    X, rel, it, st = g.sweep(freqs, block=4, precond=pc, max_iters=20000)
+    assert (rel <= 1e-8).all(), (rel, it)
+    _check_solution(m, freqs, X, rel)
+
+
+def test_hex20_rediscretized_hierarchy_small(ctx):
+    """The multigrid path without the separable box solve (Robin face on a
+    side the box solve does not cover), at resolved frequencies."""
+    m = MG.acoustic_cavity(12, 10, 8, quadratic=True)
+    m.box["robin_sides"] |= 1 << 3  # declare an x side: disables the box solve
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = np.array([60.0, 250.0])
+    X, rel, it, st = g.sweep(freqs, block=4, shift=0.5, smooth=2, max_iters=8000)
+    assert (rel <= 1e-8).all(), (rel, it)
     _check_solution(m, freqs, X, rel)
 
 
