@@ -297,15 +297,25 @@ __global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__r
   if (rl == 0) partial[blockIdx.x * B + f] = sh[t];
 }
 
-__global__ void dot_final_kernel(int nblk, int B, const double2 *__restrict__ partial, double2 *__restrict__ out) {
-  const int f = threadIdx.x;
-  if (f >= B) return;
-  double2 s = {0, 0};
-  for (int k = 0; k < nblk; k++) {
-    s.x += partial[k * B + f].x;
-    s.y += partial[k * B + f].y;
+__global__ __launch_bounds__(256) void dot_final_kernel(int nblk, int B, const double2 *__restrict__ partial,
+                                                        double2 *__restrict__ out) {
+  __shared__ double2 sh[256];
+  const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
+  double2 acc = {0, 0};
+  for (int k = rl; k < nblk; k += R) {
+    acc.x += partial[k * B + f].x;
+    acc.y += partial[k * B + f].y;
   }
-  out[f] = s;
+  sh[t] = acc;
+  __syncthreads();
+  for (int st = R / 2; st > 0; st >>= 1) {
+    if (rl < st) {
+      sh[t].x += sh[t + st * B].x;
+      sh[t].y += sh[t + st * B].y;
+    }
+    __syncthreads();
+  }
+  if (rl == 0) out[f] = sh[t];
 }
 
 // COCG scalars on the device: alpha = rho / mu (0 if inactive), beta = rho'/rho
@@ -367,27 +377,32 @@ __global__ void cocg_dir_kernel(int64_t nt, int B, const double2 *__restrict__ b
 }
 
 // refinement: r_in = r64 * s_f (to the inner precision), and x64 += d * w_f
+// (perm: row order of the inner operator, nullptr = same order)
 template <typename V>
 __global__ void scale_to_kernel(int64_t nt, int B, const double *__restrict__ s, const double2 *__restrict__ r64,
-                                V *__restrict__ out) {
+                                V *__restrict__ out, const int32_t *__restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
-  const double sc = s[i % B];
+  const int64_t r = i / B, f = i % B;
+  const double sc = s[f];
+  const double2 v = r64[(perm ? perm[r] : r) * B + f];
   V o;
-  o.x = r64[i].x * sc;
-  o.y = r64[i].y * sc;
+  o.x = v.x * sc;
+  o.y = v.y * sc;
   out[i] = o;
 }
 
 template <typename V>
 __global__ void add_scaled_kernel(int64_t nt, int B, const double *__restrict__ w, const V *__restrict__ d,
-                                  double2 *__restrict__ x) {
+                                  double2 *__restrict__ x, const int32_t *__restrict__ iperm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
-  const double sc = w[i % B];
+  const int64_t r = i / B, f = i % B;
+  const double sc = w[f];
+  const V dv = d[(iperm ? iperm[r] : r) * B + f];
   double2 v = x[i];
-  v.x += sc * (double)d[i].x;
-  v.y += sc * (double)d[i].y;
+  v.x += sc * (double)dv.x;
+  v.y += sc * (double)dv.y;
   x[i] = v;
 }
 
@@ -460,15 +475,37 @@ struct LevelData {
   DArr<int32_t> P_rp, P_ci, R_rp, R_ci;
   DArr<float> P32, R32;
   DArr<double> P64, R64;
+  // The FP32 operator may run on a locality-improving row order (rows of a
+  // node stay together, nodes in Morton order of their coordinates):
+  // perm[new] = old, iperm[old] = new; FP64 structures keep elPaSo's order.
+  bool permuted = false;
+  DArr<int32_t> perm, iperm, rp32, ci32, xrp32, xcol32, P_rp32, P_ci32, R_ci32;
+  DArr<uint8_t> xtype32, field32;
+  DArr<double> rK32, rM32, rG32;
   template <typename T> OpArgs<T> args() const;
+  struct Csr {
+    const int32_t *rp, *ci;
+  };
+  template <typename T> Csr Pcsr() const;
+  template <typename T> Csr Rcsr() const;
 };
 
 template <>
 OpArgs<float> LevelData::args<float>() const {
   Model &m = *model;
+  if (permuted)
+    return {n, rp32.p, ci32.p, K32.p, M32.p, xrp32.p, xcol32.p, xtype32.p, X32.p, field32.p, rK32.p, rM32.p, rG32.p};
   return {n, m.d_rowptr.p, m.d_colind.p, K32.p, M32.p, m.d_x_rowptr.p, m.d_x_col.p, m.d_x_type.p, X32.p,
           m.d_row_field.p, rK.p, rM.p, rG.p};
 }
+template <> LevelData::Csr LevelData::Pcsr<float>() const {
+  return permuted ? Csr{P_rp32.p, P_ci32.p} : Csr{P_rp.p, P_ci.p};
+}
+template <> LevelData::Csr LevelData::Pcsr<double>() const { return Csr{P_rp.p, P_ci.p}; }
+template <> LevelData::Csr LevelData::Rcsr<float>() const {
+  return permuted ? Csr{R_rp.p, R_ci32.p} : Csr{R_rp.p, R_ci.p};
+}
+template <> LevelData::Csr LevelData::Rcsr<double>() const { return Csr{R_rp.p, R_ci.p}; }
 template <>
 OpArgs<double> LevelData::args<double>() const {
   Model &m = *model;
@@ -585,6 +622,145 @@ static std::unique_ptr<Model> make_box_level(const Model &fine, const int nel[3]
   return m;
 }
 
+// ---------------------------------------------------------------------------
+// locality-improving row order for the FP32 operator
+// ---------------------------------------------------------------------------
+static uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffull;
+  v = (v | v << 16) & 0x1f0000ff0000ffull;
+  v = (v | v << 8) & 0x100f00f00f00f00full;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+
+template <typename T>
+__global__ void permute_rows_kernel(int64_t n, const int32_t *__restrict__ perm, const int32_t *__restrict__ iperm,
+                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                    const T *__restrict__ a, const T *__restrict__ b, const int32_t *__restrict__ rpn,
+                                    int32_t *__restrict__ cin, T *__restrict__ an, T *__restrict__ bn) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t o = perm[r];
+  int32_t q = rpn[r];
+  for (int32_t p = rp[o]; p < rp[o + 1]; p++, q++) {
+    cin[q] = iperm[ci[p]];
+    an[q] = a[p];
+    if (b) bn[q] = b[p];
+  }
+}
+
+template <typename T>
+__global__ void gather_perm_kernel(int64_t n, const int32_t *__restrict__ perm, const T *__restrict__ in,
+                                   T *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) out[r] = in[perm[r]];
+}
+
+static void permute_level_float(LevelData &L, hipStream_t s) {
+  Model &m = *L.model;
+  Ctx *c = m.ctx;
+  // Morton order of the nodes that carry DOFs
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t r = 0; r < m.n_nodes; r++) {
+    if (m.node_row[r] < 0) continue;
+    const double *x = &m.xyz[(int64_t)m.rank_to_input[r] * 3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], x[d]);
+      hi[d] = std::max(hi[d], x[d]);
+    }
+  }
+  std::vector<std::pair<uint64_t, int32_t>> key;
+  key.reserve(m.n_nodes);
+  for (int64_t r = 0; r < m.n_nodes; r++) {
+    if (m.node_row[r] < 0) continue;
+    const double *x = &m.xyz[(int64_t)m.rank_to_input[r] * 3];
+    uint64_t k = 0;
+    for (int d = 0; d < 3; d++) {
+      const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
+      const uint64_t q = (uint64_t)std::min(2097151.0, std::floor((x[d] - lo[d]) / span * 2097151.0 + 0.5));
+      k |= spread3(q) << d;
+    }
+    key.push_back({k, (int32_t)r});
+  }
+  std::stable_sort(key.begin(), key.end());
+  std::vector<int32_t> perm, iperm(m.n);
+  perm.reserve(m.n);
+  for (auto &kv : key)
+    for (int d = 0; d < m.node_nd[kv.second]; d++) perm.push_back(m.node_row[kv.second] + d);
+  for (int64_t r = 0; r < m.n; r++) iperm[perm[r]] = (int32_t)r;
+  std::vector<int32_t> rpn(m.n + 1, 0), xrpn(m.n + 1, 0);
+  for (int64_t r = 0; r < m.n; r++) {
+    rpn[r + 1] = rpn[r] + (m.rowptr[perm[r] + 1] - m.rowptr[perm[r]]);
+    xrpn[r + 1] = xrpn[r] + (m.x_rowptr[perm[r] + 1] - m.x_rowptr[perm[r]]);
+  }
+  L.perm.upload(c, perm);
+  L.iperm.upload(c, iperm);
+  L.rp32.upload(c, rpn);
+  L.xrp32.upload(c, xrpn);
+  L.ci32.alloc(c, m.nnz);
+  DArr<float> Kn, Mn;
+  Kn.alloc(c, m.nnz);
+  Mn.alloc(c, m.nnz);
+  const int g = nblk(m.n, 256);
+  hipLaunchKernelGGL(permute_rows_kernel<float>, dim3(g), dim3(256), 0, s, m.n, L.perm.p, L.iperm.p, m.d_rowptr.p,
+                     m.d_colind.p, L.K32.p, L.M32.p, L.rp32.p, L.ci32.p, Kn.p, Mn.p);
+  const int64_t nx = (int64_t)m.x_col.size();
+  L.xcol32.alloc(c, std::max<int64_t>(nx, 1));
+  DArr<float> Xn;
+  Xn.alloc(c, std::max<int64_t>(nx, 1));
+  if (nx)
+    hipLaunchKernelGGL(permute_rows_kernel<float>, dim3(g), dim3(256), 0, s, m.n, L.perm.p, L.iperm.p,
+                       m.d_x_rowptr.p, m.d_x_col.p, L.X32.p, (const float *)nullptr, L.xrp32.p, L.xcol32.p, Xn.p,
+                       (float *)nullptr);
+  // extra-entry types follow their rows (copy through the same row map)
+  std::vector<uint8_t> xt(std::max<int64_t>(nx, 1), 0);
+  for (int64_t r = 0; r < m.n; r++) {
+    int32_t q = xrpn[r];
+    for (int32_t p = m.x_rowptr[perm[r]]; p < m.x_rowptr[perm[r] + 1]; p++) xt[q++] = m.x_type[p];
+  }
+  L.xtype32.upload(c, xt);
+  std::vector<uint8_t> fld(m.n);
+  for (int64_t r = 0; r < m.n; r++) fld[r] = m.row_field[perm[r]];
+  L.field32.upload(c, fld);
+  L.rK32.alloc(c, m.n);
+  L.rM32.alloc(c, m.n);
+  L.rG32.alloc(c, m.n);
+  hipLaunchKernelGGL(gather_perm_kernel<double>, dim3(g), dim3(256), 0, s, m.n, L.perm.p, L.rK.p, L.rK32.p);
+  hipLaunchKernelGGL(gather_perm_kernel<double>, dim3(g), dim3(256), 0, s, m.n, L.perm.p, L.rM.p, L.rM32.p);
+  hipLaunchKernelGGL(gather_perm_kernel<double>, dim3(g), dim3(256), 0, s, m.n, L.perm.p, L.rG.p, L.rG32.p);
+  ELP_HIP(hipGetLastError());
+  if (L.nc) {
+    // interpolation: permute the rows of P, map the columns of R
+    std::vector<int32_t> prp, pci, rci;
+    std::vector<float> pv;
+    L.P_rp.download(prp);
+    L.P_ci.download(pci);
+    L.P32.download(pv);
+    L.R_ci.download(rci);
+    std::vector<int32_t> prpn(m.n + 1, 0), pcin;
+    std::vector<float> pvn;
+    for (int64_t r = 0; r < m.n; r++) {
+      for (int32_t p = prp[perm[r]]; p < prp[perm[r] + 1]; p++) {
+        pcin.push_back(pci[p]);
+        pvn.push_back(pv[p]);
+      }
+      prpn[r + 1] = (int32_t)pcin.size();
+    }
+    for (auto &v : rci) v = iperm[v];
+    L.P_rp32.upload(c, prpn);
+    L.P_ci32.upload(c, pcin);
+    L.P32.upload(c, pvn);
+    L.R_ci32.upload(c, rci);
+  }
+  ELP_HIP(hipStreamSynchronize(s));
+  L.K32 = std::move(Kn);
+  L.M32 = std::move(Mn);
+  L.X32 = std::move(Xn);
+  L.permuted = true;
+}
+
 // trilinear interpolation from a coarse box lattice to the rows of `fine`
 static void build_interp(LevelData &F, const Model &coarse_like_box, const int nelc[3]) {
   const Model &fm = *F.model;
@@ -679,6 +855,9 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
     Model lattice;
     lattice.box = m.box;
     build_interp(*st->levels[0], lattice, m.box.nel);
+    permute_level_float(*st->levels[0], s);
+  } else if (!want_mg) {
+    permute_level_float(*st->levels[0], s);
   } else if (want_mg) {
     int nel[3] = {m.box.nel[0], m.box.nel[1], m.box.nel[2]};
     bool first = true;
@@ -798,7 +977,7 @@ struct Solver {
   template <typename VV>
   void reduce(int64_t n, const VV *a, const VV *b, int mode, double2 *dev_out) {
     hipLaunchKernelGGL(dot_kernel<VV>, dim3(kRedBlocks), dim3(256), 0, s, n, B, a, b, mode, red_part.p);
-    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(64), 0, s, kRedBlocks, B, red_part.p, dev_out);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, kRedBlocks, B, red_part.p, dev_out);
     ELP_HIP(hipGetLastError());
   }
   void fetch(const double2 *dev) {
@@ -831,9 +1010,9 @@ struct Solver {
     smooth(l, b, nullptr, nu, W.x.p);
     op<1>(l, coefP.p, W.x.p, b, W.r.p);
     LevelWork<T> &C = lw[l + 1];
-    transfer<false>(L.nc, L.R_rp.p, L.R_ci.p, Rv(L), W.r.p, C.b.p);
+    transfer<false>(L.nc, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, C.b.p);
     vcycle(l + 1, C.b.p, C.e.p);
-    transfer<true>(L.n, L.P_rp.p, L.P_ci.p, Pv(L), C.e.p, W.x.p);
+    transfer<true>(L.n, L.Pcsr<T>().rp, L.Pcsr<T>().ci, Pv(L), C.e.p, W.x.p);
     smooth(l, b, W.x.p, nu, z);
   }
 
@@ -845,23 +1024,23 @@ struct Solver {
     LevelWork<T> &W = lw[0];
     const int64_t nf = fdm_nodes(*st.fdm);
     if (st.order == 1) {
-      transfer<false>(nf, L.R_rp.p, L.R_ci.p, Rv(L), b, fb.p);
+      transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), b, fb.p);
       fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
-      transfer<false>(L.n, L.P_rp.p, L.P_ci.p, Pv(L), fe.p, z);
+      transfer<false>(L.n, L.Pcsr<T>().rp, L.Pcsr<T>().ci, Pv(L), fe.p, z);
       return;
     }
     const int nu = std::max(1, o.smooth);
     smooth(0, b, nullptr, nu, W.x.p);
     op<1>(0, coefP.p, W.x.p, b, W.r.p);
-    transfer<false>(nf, L.R_rp.p, L.R_ci.p, Rv(L), W.r.p, fb.p);
+    transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, fb.p);
     fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
-    transfer<true>(L.n, L.P_rp.p, L.P_ci.p, Pv(L), fe.p, W.x.p);
+    transfer<true>(L.n, L.Pcsr<T>().rp, L.Pcsr<T>().ci, Pv(L), fe.p, W.x.p);
     smooth(0, b, W.x.p, nu, z);
   }
 
   void precond(const V *r, V *z) {
     if (st.fdm) box_cycle(r, z);
-    else if (st.levels.size() == 1) smooth(0, r, nullptr, std::max(1, o.smooth), z);
+    else if (st.levels.size() == 1) smooth(0, r, nullptr, std::max(2, o.smooth), z);  // l1-Jacobi: 2 sweeps
     else vcycle(0, r, z);
   }
 
@@ -964,9 +1143,24 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   hipStream_t s = S.s;
   std::vector<OpCoef> cA(B), cP(B);
   std::vector<FdmCoef> cF(B);
-  const double beta = o.shift >= 0 ? o.shift : (S.st.fdm ? 0.0 : 0.5);
+  double hmax = 0;
+  if (S.st.fdm)
+    for (int d = 0; d < 3; d++) hmax = std::max(hmax, m.box.length[d] / m.box.nel[d]);
   for (int f = 0; f < B; f++) {
     const double w = 2.0 * M_PI * freqs[std::min(f, nb - 1)];
+    // automatic shift: none with the exact box solve while the mesh resolves
+    // the wave (k h < 0.5), growing to 0.3 on coarse meshes where the Hex8 and
+    // model discretizations disagree near resonances; 0.5 for the
+    // rediscretized hierarchy
+    double beta = o.shift;
+    if (beta < 0) {
+      if (S.st.fdm) {
+        const double kh = w / fdm_c(*S.st.fdm) * hmax;
+        beta = std::min(0.3, std::max(0.0, 0.1 * (kh - 0.5)));
+      } else {
+        beta = 0.5;
+      }
+    }
     cA[f] = make_coef(m, w, Z, 0.0);
     cP[f] = make_coef(m, w, Z, beta);
     if (S.st.fdm) {
@@ -1002,6 +1196,10 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   DArr<double> scl;
   scl.alloc(m.ctx, B);
   const int max_outer = 30;
+  const LevelData &L0 = *S.st.levels[0];
+  const bool use_perm = std::is_same<T, float>::value && L0.permuted;
+  const int32_t *perm = use_perm ? L0.perm.p : nullptr;
+  const int32_t *iperm = use_perm ? L0.iperm.p : nullptr;
   std::vector<double> prev(B, INFINITY), wprev(B, 0.0);
   std::vector<int> stalled(B, 0), lastact(B, 0);
   for (int outer = 0; outer <= max_outer; outer++) {
@@ -1026,7 +1224,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     if (any_undo) {
       ELP_HIP(hipMemcpyAsync(scl.p, undo.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL((add_scaled_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B,
-                         scl.p, S.xi.p, x64.p);
+                         scl.p, S.xi.p, x64.p, iperm);
       op64(x64.p, b64.p, r64.p);
       S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
       S.fetch(S.red_out.p);
@@ -1051,7 +1249,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     refinements++;
     ELP_HIP(hipMemcpyAsync(scl.p, inv.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((scale_to_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B, scl.p,
-                       r64.p, S.ri.p);
+                       r64.p, S.ri.p, perm);
     double tol = o.inner_rtol;
     if (std::is_same<T, double>::value) {
       // FP64 inner solve: aim directly below the target
@@ -1063,7 +1261,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     S.cocg(act, tol, o.max_iters, it);
     ELP_HIP(hipMemcpyAsync(scl.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((add_scaled_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B, scl.p,
-                       S.xi.p, x64.p);
+                       S.xi.p, x64.p, iperm);
     ELP_HIP(hipGetLastError());
     ELP_HIP(hipStreamSynchronize(s));
     for (int f = 0; f < B; f++) {

@@ -43,13 +43,12 @@ def test_config0_sweep(ctx, prec, pc):
     _check_solution(m, freqs, X, rel)
 
 
-@pytest.mark.parametrize("pc", [_lib.PC_MG, _lib.PC_JACOBI])
-def test_hex20_sweep_small(ctx, pc):
+def test_hex20_sweep_small(ctx):
     m = MG.acoustic_cavity(20, 16, 12, quadratic=True)
     g = GpuModel(ctx, m)
     g.assemble()
-    freqs = np.array([50.0, 400.0, 1200.0, 2000.0]) if pc == _lib.PC_MG else np.array([50.0, 300.0])
-    X, rel, it, st = g.sweep(freqs, block=4, precond=pc, max_iters=20000)
+    freqs = np.array([50.0, 400.0, 1200.0, 2000.0])
+    X, rel, it, st = g.sweep(freqs, block=4, max_iters=4000)
     assert (rel <= 1e-8).all(), (rel, it)
     _check_solution(m, freqs, X, rel)
 
