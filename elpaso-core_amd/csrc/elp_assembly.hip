@@ -420,186 +420,193 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
 constexpr int kMaxHex8Cols = 128;  // row length / neighbours of a Hex8 node (checked at finalize)
 template <int ND>
 struct Hex8Lds {
-  double geo[64][10];        // [k*8+g]: invJ (9), w detJ
-  double val[64][ND == 1 ? 2 : 10];
-  int32_t slot[64];
+  double geo[64][10];  // [k*8+g]: invJ (9), w detJ
+  union {
+    double xe[64][3];  // phase 0-1: coordinates of node q of element k, [k*8+q]
+    double kr[ND][kMaxHex8Cols];
+  } u;
+  double mr[kMaxHex8Cols];
   int32_t nb[kMaxHex8Cols];
   int32_t off[kMaxHex8Cols + 1];
-  double kr[ND][kMaxHex8Cols];
-  double mr[kMaxHex8Cols];
+  int16_t colslot[kMaxHex8Cols];
 };
 
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int ND>
-__global__ __launch_bounds__(64 * kWaves) void hex8_node_kernel(
+__global__ __launch_bounds__(64) void hex8_node_kernel(
     int32_t n_nodes, const int32_t *__restrict__ nodes, const int32_t *__restrict__ adj_ptr,
     const int32_t *__restrict__ adj_elem, const uint8_t *__restrict__ adj_local, const int32_t *__restrict__ conn,
     const double *__restrict__ params, const double *__restrict__ xyz, const int32_t *__restrict__ nbr_ptr,
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ nbr_off, const int32_t *__restrict__ node_row,
     const int32_t *__restrict__ rowptr, double *__restrict__ Kv, double *__restrict__ Mv, double *__restrict__ Cv,
     double alpha, double beta, int *bad) {
-  __shared__ Hex8Lds<ND> lds[kWaves];
-  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int32_t s_node = blockIdx.x * kWaves + w;
-  if (s_node >= n_nodes) return;
-  Hex8Lds<ND> &L = lds[w];
-  const int32_t i = nodes[s_node];
-  const int32_t a0 = adj_ptr[s_node], nadj = adj_ptr[s_node + 1] - a0;
-  const int32_t nb0 = nbr_ptr[i], deg = nbr_ptr[i + 1] - nb0;
-  for (int t = lane; t < deg; t += 64) {
-    L.nb[t] = nbr[nb0 + t];
-    L.off[t] = nbr_off[nb0 + t];
-  }
-  const int32_t r0 = node_row[i];
-  const int32_t len = rowptr[r0 + 1] - rowptr[r0];
-  if (lane == 0) L.off[deg] = len;
-  // 1. geometry of (element k, point g)
-  const int k = lane / 8, g = lane % 8;
-  int64_t e = -1;
-  if (k < nadj) {
-    e = adj_elem[a0 + k];
-    const double *dxi = c_tab.h8[g][1], *deta = c_tab.h8[g][2], *dzeta = c_tab.h8[g][3];
-    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int q = 0; q < 8; q++) {
-      const int64_t nd_ = conn[e * 8 + q];
-      const double x0 = xyz[nd_ * 3], x1 = xyz[nd_ * 3 + 1], x2 = xyz[nd_ * 3 + 2];
-      J[0] += dxi[q] * x0; J[1] += dxi[q] * x1; J[2] += dxi[q] * x2;
-      J[3] += deta[q] * x0; J[4] += deta[q] * x1; J[5] += deta[q] * x2;
-      J[6] += dzeta[q] * x0; J[7] += dzeta[q] * x1; J[8] += dzeta[q] * x2;
+  __shared__ Hex8Lds<ND> L;
+  const int lane = threadIdx.x;
+  const int k = lane / 8, g = lane % 8, b = lane % 8;
+  {
+    const int32_t s_node = blockIdx.x;
+    if (s_node >= n_nodes) return;
+    const int32_t i = nodes[s_node];
+    const int32_t a0 = adj_ptr[s_node], nadj = adj_ptr[s_node + 1] - a0;
+    const int32_t nb0 = nbr_ptr[i], deg = nbr_ptr[i + 1] - nb0;
+    for (int t = lane; t < deg; t += 64) {
+      L.nb[t] = nbr[nb0 + t];
+      L.off[t] = nbr_off[nb0 + t];
     }
-    const double det = J[0] * J[4] * J[8] + J[1] * J[5] * J[6] + J[2] * J[3] * J[7] -
-                       J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
-    if (!(det > 0.0)) atomicMin(bad, (int)e);
-    double *G = L.geo[lane];
-    G[0] = (J[4] * J[8] - J[5] * J[7]) / det;
-    G[3] = -(J[3] * J[8] - J[5] * J[6]) / det;
-    G[6] = (J[3] * J[7] - J[4] * J[6]) / det;
-    G[1] = -(J[1] * J[8] - J[2] * J[7]) / det;
-    G[4] = (J[0] * J[8] - J[2] * J[6]) / det;
-    G[7] = -(J[0] * J[7] - J[1] * J[6]) / det;
-    G[2] = (J[1] * J[5] - J[2] * J[4]) / det;
-    G[5] = -(J[0] * J[5] - J[2] * J[3]) / det;
-    G[8] = (J[0] * J[4] - J[1] * J[3]) / det;
-    G[9] = c_tab.w8[g] * det;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // 2. unit (k, b): element k's block of the (node, neighbour b) pair
-  const int b = lane % 8;
-  if (k < nadj) {
-    const int a = adj_local[a0 + k];
-    L.slot[lane] = find_slot(L.nb, deg, conn[e * 8 + b]);
-    double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mm = 0.0, kk = 0.0;
-    for (int gg = 0; gg < 8; gg++) {
-      const double *G = L.geo[k * 8 + gg];
-      const double ax = c_tab.h8[gg][1][a], ay = c_tab.h8[gg][2][a], az = c_tab.h8[gg][3][a];
-      const double bx = c_tab.h8[gg][1][b], by = c_tab.h8[gg][2][b], bz = c_tab.h8[gg][3][b];
-      const double wd = G[9];
-      const double p0 = G[0] * ax + G[1] * ay + G[2] * az;
-      const double p1 = G[3] * ax + G[4] * ay + G[5] * az;
-      const double p2 = G[6] * ax + G[7] * ay + G[8] * az;
-      const double q0 = G[0] * bx + G[1] * by + G[2] * bz;
-      const double q1 = G[3] * bx + G[4] * by + G[5] * bz;
-      const double q2 = G[6] * bx + G[7] * by + G[8] * bz;
-      mm += c_tab.h8[gg][0][a] * c_tab.h8[gg][0][b] * wd;
-      if constexpr (ND == 1) {
-        kk += (p0 * q0 + p1 * q1 + p2 * q2) * wd;
-      } else {
-        const double w0 = wd * p0, w1 = wd * p1, w2 = wd * p2;
-        P[0] += w0 * q0; P[1] += w0 * q1; P[2] += w0 * q2;
-        P[3] += w1 * q0; P[4] += w1 * q1; P[5] += w1 * q2;
-        P[6] += w2 * q0; P[7] += w2 * q1; P[8] += w2 * q2;
+    const int32_t r0 = node_row[i];
+    const int32_t len = rowptr[r0 + 1] - rowptr[r0];
+    if (lane == 0) L.off[deg] = len;
+    // 0. coordinates: lane (k, q) loads node q of element k
+    int32_t e = -1, a = 0;
+    if (k < nadj) {
+      e = adj_elem[a0 + k];
+      a = adj_local[a0 + k];
+      const int32_t nd_ = conn[e * 8 + g];
+      L.u.xe[lane][0] = xyz[nd_ * 3];
+      L.u.xe[lane][1] = xyz[nd_ * 3 + 1];
+      L.u.xe[lane][2] = xyz[nd_ * 3 + 2];
+    }
+    wave_sync();
+    // 1. geometry of (element k, point g): inverse Jacobian, w detJ, and
+    //    w detJ grad N_a of the node itself
+    if (k < nadj) {
+      const double *dxi = c_tab.h8[g][1], *deta = c_tab.h8[g][2], *dzeta = c_tab.h8[g][3];
+      double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const double *x = L.u.xe[k * 8 + q];
+        const double x0 = x[0], x1 = x[1], x2 = x[2];
+        J[0] += dxi[q] * x0; J[1] += dxi[q] * x1; J[2] += dxi[q] * x2;
+        J[3] += deta[q] * x0; J[4] += deta[q] * x1; J[5] += deta[q] * x2;
+        J[6] += dzeta[q] * x0; J[7] += dzeta[q] * x1; J[8] += dzeta[q] * x2;
       }
+      const double det = J[0] * J[4] * J[8] + J[1] * J[5] * J[6] + J[2] * J[3] * J[7] -
+                         J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
+      if (!(det > 0.0)) atomicMin(bad, e);
+      const double id = 1.0 / det;
+      double G[9];
+      G[0] = (J[4] * J[8] - J[5] * J[7]) * id;
+      G[3] = -(J[3] * J[8] - J[5] * J[6]) * id;
+      G[6] = (J[3] * J[7] - J[4] * J[6]) * id;
+      G[1] = -(J[1] * J[8] - J[2] * J[7]) * id;
+      G[4] = (J[0] * J[8] - J[2] * J[6]) * id;
+      G[7] = -(J[0] * J[7] - J[1] * J[6]) * id;
+      G[2] = (J[1] * J[5] - J[2] * J[4]) * id;
+      G[5] = -(J[0] * J[5] - J[2] * J[3]) * id;
+      G[8] = (J[0] * J[4] - J[1] * J[3]) * id;
+      double *O = L.geo[lane];
+#pragma unroll
+      for (int t = 0; t < 9; t++) O[t] = G[t];
+      O[9] = c_tab.w8[g] * det;
     }
-    // material law of this element (elements of one node may differ in material)
-    const double p0m = params[e * 4], p1m = params[e * 4 + 1], p2m = params[e * 4 + 2];
-    double *V = L.val[lane];
-    if constexpr (ND == 1) {
-      V[0] = kk / p0m;
-      V[1] = p1m * mm / p0m;
-    } else {
-      const double tr = P[0] + P[4] + P[8];
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int dd = 0; dd < 3; dd++)
-          V[d * 3 + dd] = p0m * P[d * 3 + dd] + p1m * P[dd * 3 + d] + (d == dd ? p1m * tr : 0.0);
-      V[9] = p2m * mm;
-    }
-  } else {
-    L.slot[lane] = -1;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // 3. per neighbour slot: ordered sum over contributing units (u ascending =
-  //    element order), finished with the material law
-  const int nu = nadj * 8;
-  for (int s = lane; s < deg; s += 64) {
-    double acc[ND == 1 ? 2 : 10];
-#pragma unroll
-    for (int t = 0; t < (ND == 1 ? 2 : 10); t++) acc[t] = 0.0;
-    int last_u = -1;
-    for (int u = 0; u < nu; u++) {
-      if (L.slot[u] != s) continue;
-      last_u = u;
-#pragma unroll
-      for (int t = 0; t < (ND == 1 ? 2 : 10); t++) acc[t] += L.val[u][t];
-    }
-    const int off = L.off[s];
-    if (last_u < 0) {  // neighbour coupled only through an interface element
-      for (int q = off; q < L.off[s + 1]; q++)
-        for (int d = 0; d < ND; d++) L.kr[d][q] = 0.0;
-      L.mr[s] = 0.0;
-      continue;
-    }
-    if constexpr (ND == 1) {
-      L.kr[0][off] = acc[0];
-      L.mr[s] = acc[1];
-    } else {
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int dd = 0; dd < 3; dd++) L.kr[d][off + dd] = acc[d * 3 + dd];
-      L.mr[s] = acc[9];
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // 4. coalesced row writes
-  for (int d = 0; d < ND; d++) {
-    const int64_t base = rowptr[r0 + d];
+    wave_sync();
+    // zero the row accumulators (they reuse the coordinate staging) and map
+    // every column of the rows to its neighbour slot
     for (int q = lane; q < len; q += 64) {
-      int lo = 0, hi = deg - 1;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (L.off[mid] <= q) lo = mid;
-        else hi = mid - 1;
+#pragma unroll
+      for (int d = 0; d < ND; d++) L.u.kr[d][q] = 0.0;
+    }
+    for (int sl = lane; sl < deg; sl += 64) {
+      L.mr[sl] = 0.0;
+      for (int q = L.off[sl]; q < L.off[sl + 1]; q++) L.colslot[q] = (int16_t)sl;
+    }
+    // 2. unit (k, b): element k's block of the (node, neighbour b) pair, with
+    //    the element's own material law
+    double V[ND == 1 ? 2 : 10];
+    int slot = -1;
+    if (k < nadj) {
+      slot = find_slot(L.nb, deg, conn[e * 8 + b]);
+      double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mm = 0.0, kk = 0.0;
+#pragma unroll
+      for (int gg = 0; gg < 8; gg++) {
+        const double *G = L.geo[k * 8 + gg];
+        const double ax = c_tab.h8[gg][1][a], ay = c_tab.h8[gg][2][a], az = c_tab.h8[gg][3][a];
+        const double bx = c_tab.h8[gg][1][b], by = c_tab.h8[gg][2][b], bz = c_tab.h8[gg][3][b];
+        const double wd = G[9];
+        const double p0 = G[0] * ax + G[1] * ay + G[2] * az;
+        const double p1 = G[3] * ax + G[4] * ay + G[5] * az;
+        const double p2 = G[6] * ax + G[7] * ay + G[8] * az;
+        const double q0 = G[0] * bx + G[1] * by + G[2] * bz;
+        const double q1 = G[3] * bx + G[4] * by + G[5] * bz;
+        const double q2 = G[6] * bx + G[7] * by + G[8] * bz;
+        mm += c_tab.h8[gg][0][a] * c_tab.h8[gg][0][b] * wd;
+        const double w0 = wd * p0, w1 = wd * p1, w2 = wd * p2;
+        if constexpr (ND == 1) {
+          kk += w0 * q0 + w1 * q1 + w2 * q2;
+        } else {
+          P[0] += w0 * q0; P[1] += w0 * q1; P[2] += w0 * q2;
+          P[3] += w1 * q0; P[4] += w1 * q1; P[5] += w1 * q2;
+          P[6] += w2 * q0; P[7] += w2 * q1; P[8] += w2 * q2;
+        }
       }
-      const int dd = q - L.off[lo];
-      const double kv = L.kr[d][q];
-      const double mv = (dd == d) ? L.mr[lo] : 0.0;
-      Kv[base + q] = kv;
-      Mv[base + q] = mv;
-      if (Cv) Cv[base + q] = alpha * kv + beta * mv;
+      const double p0m = params[e * 4], p1m = params[e * 4 + 1], p2m = params[e * 4 + 2];
+      if constexpr (ND == 1) {
+        V[0] = kk / p0m;
+        V[1] = p1m * mm / p0m;
+      } else {
+        const double tr = P[0] + P[4] + P[8];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++)
+            V[d * 3 + dd] = p0m * P[d * 3 + dd] + p1m * P[dd * 3 + d] + (d == dd ? p1m * tr : 0.0);
+        V[9] = p2m * mm;
+      }
+    }
+    wave_sync();
+    // 3. ordered accumulation: element k's 8 units (distinct slots) add in
+    //    round k, so every entry sums in ascending element id
+    for (int kk = 0; kk < nadj; kk++) {
+      if (k == kk) {
+        const int off = L.off[slot];
+        if constexpr (ND == 1) {
+          L.u.kr[0][off] += V[0];
+          L.mr[slot] += V[1];
+        } else {
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int dd = 0; dd < 3; dd++) L.u.kr[d][off + dd] += V[d * 3 + dd];
+          L.mr[slot] += V[9];
+        }
+      }
+      wave_sync();
+    }
+    // 4. coalesced row writes
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const int64_t base = rowptr[r0 + d];
+      for (int q = lane; q < len; q += 64) {
+        const int sl = L.colslot[q];
+        const int dd = q - L.off[sl];
+        const double kv = L.u.kr[d][q];
+        const double mv = (dd == d) ? L.mr[sl] : 0.0;
+        Kv[base + q] = kv;
+        Mv[base + q] = mv;
+        if (Cv) Cv[base + q] = alpha * kv + beta * mv;
+      }
     }
   }
 }
 
 static void launch_hex8_node(Model &m, KindBlock &kb, int *bad, hipStream_t s) {
   const int32_t nn = (int32_t)kb.nodes.size();
-  const int grid = blocks_for(nn, kWaves);
-  if (grid == 0) return;
+  if (nn == 0) return;
+  const int grid = nn;  // one wavefront per node
   double *Cv = m.has_rayleigh ? m.d_C.p : nullptr;
 #define ELP_HEX8_ARGS                                                                                   \
   nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, kb.d_params.p,   \
       m.d_xyz.p, m.d_nbr_ptr.p, m.d_nbr.p, m.d_nbr_off.p, m.d_node_row.p, m.d_rowptr.p, m.d_K.p,     \
       m.d_M.p, Cv, m.ray_a, m.ray_b, bad
   if (kb.kind == ELP_FLUID8)
-    hipLaunchKernelGGL(hex8_node_kernel<1>, dim3(grid), dim3(64 * kWaves), 0, s, ELP_HEX8_ARGS);
+    hipLaunchKernelGGL(hex8_node_kernel<1>, dim3(grid), dim3(64), 0, s, ELP_HEX8_ARGS);
   else
-    hipLaunchKernelGGL(hex8_node_kernel<3>, dim3(grid), dim3(64 * kWaves), 0, s, ELP_HEX8_ARGS);
+    hipLaunchKernelGGL(hex8_node_kernel<3>, dim3(grid), dim3(64), 0, s, ELP_HEX8_ARGS);
 #undef ELP_HEX8_ARGS
   ELP_HIP(hipGetLastError());
 }
