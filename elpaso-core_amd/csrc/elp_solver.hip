@@ -318,7 +318,8 @@ __global__ __launch_bounds__(256) void dot_final_kernel(int nblk, int B, const d
   if (rl == 0) out[f] = sh[t];
 }
 
-// COCG scalars on the device: alpha = rho / mu (0 if inactive), beta = rho'/rho
+// COCG scalars on the device: alpha = rho / mu (0 if inactive), beta = rho'/rho.
+// active[f]: 0 = frozen column, 1 = iterating, 2 = (re)starting (beta = 0, p = z)
 __global__ void cocg_alpha_kernel(int B, const double2 *rho, const double2 *mu, const int *active, double2 *alpha) {
   const int f = threadIdx.x;
   if (f >= B) return;
@@ -336,7 +337,7 @@ __global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, co
   if (f >= B) return;
   double2 b = {0, 0};
   const double den = rho[f].x * rho[f].x + rho[f].y * rho[f].y;
-  if (active[f] && den > 0 && isfinite(den)) {
+  if (active[f] == 1 && den > 0 && isfinite(den)) {
     b.x = (rho_new[f].x * rho[f].x + rho_new[f].y * rho[f].y) / den;
     b.y = (rho_new[f].y * rho[f].x - rho_new[f].x * rho[f].y) / den;
   }
@@ -351,6 +352,7 @@ __global__ void cocg_update_kernel(int64_t nt, int B, const double2 *__restrict_
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
   const double2 a = alpha[i % B];
+  if (a.x == 0 && a.y == 0) return;  // frozen column: x, r untouched
   const V pv = p[i], qv = q[i];
   V xv = x[i], rv = r[i];
   xv.x += a.x * pv.x - a.y * pv.y;
@@ -369,6 +371,10 @@ __global__ void cocg_dir_kernel(int64_t nt, int B, const double2 *__restrict__ b
   if (i >= nt) return;
   const double2 bt = beta[i % B];
   const V zv = z[i];
+  if (bt.x == 0 && bt.y == 0) {  // (re)start: p = z, whatever p held
+    p[i] = zv;
+    return;
+  }
   V pv = p[i];
   V o;
   o.x = zv.x + bt.x * pv.x - bt.y * pv.y;
@@ -399,6 +405,7 @@ __global__ void add_scaled_kernel(int64_t nt, int B, const double *__restrict__ 
   if (i >= nt) return;
   const int64_t r = i / B, f = i % B;
   const double sc = w[f];
+  if (sc == 0.0) return;
   const V dv = d[(iperm ? iperm[r] : r) * B + f];
   double2 v = x[i];
   v.x += sc * (double)dv.x;
@@ -417,6 +424,42 @@ __global__ void unpack_kernel(int64_t n, int B, int nb, const double2 *__restric
   if (i >= n * nb) return;
   const int64_t f = i / n, r = i % n;
   out[i] = x[r * B + f];
+}
+
+// slot scheduler: column c of the [row][B] layout to a contiguous vector
+__global__ void unpack_col_kernel(int64_t n, int B, int c, const double2 *__restrict__ x, double2 *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  out[r] = x[r * B + c];
+}
+
+// columns taking a new frequency: x64 = 0, r64 = b
+__global__ void slot_fresh_kernel(int64_t n, int B, const int *__restrict__ fresh, const double2 *__restrict__ F,
+                                  double2 *__restrict__ x64, double2 *__restrict__ r64) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * B) return;
+  const int f = (int)(i % B);
+  if (!fresh[f]) return;
+  x64[i] = {0, 0};
+  r64[i] = F[i / B];
+}
+
+// columns (re)starting the inner solve: x_in = 0, r_in = r64 * s_f
+template <typename V>
+__global__ void slot_restart_kernel(int64_t nt, int B, const int *__restrict__ active, const double *__restrict__ s,
+                                    const double2 *__restrict__ r64, V *__restrict__ r, V *__restrict__ x,
+                                    const int32_t *__restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const int64_t row = i / B, f = i % B;
+  if (active[f] != 2) return;
+  const double sc = s[f];
+  const double2 v = r64[(perm ? perm[row] : row) * B + f];
+  V o;
+  o.x = v.x * sc;
+  o.y = v.y * sc;
+  r[i] = o;
+  x[i] = {0, 0};
 }
 
 // solver shadows: FP32 copies, Dirichlet columns zeroed; row l1 sums
@@ -1284,16 +1327,281 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   ELP_HIP(hipStreamSynchronize(s));
 }
 
+// Continuous batching of a sweep on B slots (mixed precision).  Each slot
+// holds one frequency; all slots advance together through one fused COCG
+// iteration (one multi-frequency SpMV and one preconditioner application
+// for the whole block).  When a slot's inner solve reaches inner_rtol it is
+// frozen; once a quarter of the slots are frozen (or none is iterating) the
+// frozen slots take an FP64 refinement step together: x64 += w d, r64 = b -
+// A x64 in FP64.  A slot whose FP64 residual reaches rtol (or stops
+// improving, or runs out of iterations) is written out and takes the next
+// frequency of the sweep; the others restart their inner solve on the new
+// residual.  Every column is computed independently of the others, so a
+// frequency's result does not depend on which frequencies share the block.
+template <typename T, int B>
+static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::complex<double> Z, double *X,
+                        double *relres, int32_t *iters, int &refinements) {
+  Model &m = S.m;
+  const elp_solve_opts &o = S.o;
+  hipStream_t s = S.s;
+  using V = typename VT<T>::t;
+  const int64_t nt = m.n * B;
+  const int tpb = 256;
+  // right-hand side (the same for every frequency)
+  DArr<double2> F, b64, x64, r64, col;
+  std::vector<double2> hF(m.n);
+  for (int64_t r = 0; r < m.n; r++) hF[r] = {m.F[r].real(), m.F[r].imag()};
+  for (size_t k = 0; k < m.dir_rows.size(); k++) hF[m.dir_rows[k]] = {m.dir_vals[k].real(), m.dir_vals[k].imag()};
+  double nb2 = 0;
+  for (const auto &v : hF) nb2 += v.x * v.x + v.y * v.y;
+  const double nbn = std::sqrt(nb2);
+  if (!(nbn > 0)) {  // b = 0: x = 0 for every frequency
+    if (X) std::fill(X, X + 2 * nf * m.n, 0.0);
+    for (int64_t k = 0; k < nf; k++) {
+      if (relres) relres[k] = 0.0;
+      if (iters) iters[k] = 0;
+    }
+    return;
+  }
+  F.upload(m.ctx, hF);
+  b64.alloc(m.ctx, nt);
+  x64.alloc(m.ctx, nt);
+  r64.alloc(m.ctx, nt);
+  if (X) col.alloc(m.ctx, m.n);
+  hipLaunchKernelGGL(bcast_rhs_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, F.p, b64.p);
+  ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
+  ELP_HIP(hipMemsetAsync(r64.p, 0, nt * sizeof(double2), s));
+  ELP_HIP(hipMemsetAsync(S.ri.p, 0, nt * sizeof(V), s));
+  ELP_HIP(hipMemsetAsync(S.xi.p, 0, nt * sizeof(V), s));
+  ELP_HIP(hipMemsetAsync(S.pi.p, 0, nt * sizeof(V), s));
+  const LevelData &L0 = *S.st.levels[0];
+  const bool use_perm = std::is_same<T, float>::value && L0.permuted;
+  const int32_t *perm = use_perm ? L0.perm.p : nullptr;
+  const int32_t *iperm = use_perm ? L0.iperm.p : nullptr;
+  double hmax = 0;
+  if (S.st.fdm)
+    for (int d = 0; d < 3; d++) hmax = std::max(hmax, m.box.length[d] / m.box.nel[d]);
+
+  // host mirrors (modified only while the stream is idle)
+  std::vector<OpCoef> cA(B), cP(B);
+  std::vector<FdmCoef> cF(B);
+  // coefficients of frequency fi in slot k
+  auto set_coefs = [&](int k, int64_t fi) {
+    const double w = 2.0 * M_PI * freqs[fi];
+    double beta = o.shift;
+    if (beta < 0) {
+      if (S.st.fdm) {
+        const double kh = w / fdm_c(*S.st.fdm) * hmax;
+        beta = std::min(0.3, std::max(0.0, 0.1 * (kh - 0.5)));
+      } else {
+        beta = 0.5;
+      }
+    }
+    cA[k] = make_coef(m, w, Z, 0.0);
+    cP[k] = make_coef(m, w, Z, beta);
+    if (S.st.fdm) {
+      const double rho = fdm_rho(*S.st.fdm), c = fdm_c(*S.st.fdm);
+      const std::complex<double> sM = (1.0 - std::complex<double>(0, beta)) / (c * c);
+      const std::complex<double> tG = std::abs(Z) > 0 ? std::complex<double>(0, rho) / (w * Z) : 0.0;
+      cF[k] = {1.0 / (w * w), {sM.real(), sM.imag()}, {tG.real(), tG.imag()}, rho};
+    }
+  };
+  // slots left empty keep valid (unused) coefficients
+  for (int k = 0; k < B; k++) set_coefs(k, 0);
+  std::vector<int> act(B, 0), fresh(B, 0);
+  std::vector<double> scale(B, 0.0), wgt(B, 0.0);
+  DArr<int> d_fresh;
+  DArr<double> d_scale, d_wgt;
+  d_fresh.alloc(m.ctx, B);
+  d_scale.alloc(m.ctx, B);
+  d_wgt.alloc(m.ctx, B);
+  struct Slot {
+    int64_t f = -1;   // frequency index, -1 = empty
+    int state = 0;    // 0 empty, 1 iterating, 2 frozen (inner solve done)
+    bool broken = false;
+    double prev = 1;  // FP64 relative residual before the current inner solve
+    double w = 0;     // ||r64|| the inner right-hand side was scaled by
+  };
+  std::vector<Slot> sl(B);
+  int64_t next = 0;
+  std::vector<int> it_f(nf, 0);
+
+  // assign the next frequency to slot k (stream idle)
+  auto assign = [&](int k) {
+    if (next >= nf) {
+      sl[k] = Slot();
+      act[k] = 0;
+      return false;
+    }
+    const int64_t fi = next++;
+    set_coefs(k, fi);
+    sl[k] = Slot();
+    sl[k].f = fi;
+    sl[k].state = 1;
+    sl[k].prev = 1.0;
+    sl[k].w = nbn;
+    fresh[k] = 1;
+    act[k] = 2;
+    scale[k] = 1.0 / nbn;
+    return true;
+  };
+  auto upload_coefs = [&]() {
+    ELP_HIP(hipMemcpyAsync(S.coefA.p, cA.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
+    ELP_HIP(hipMemcpyAsync(S.coefP.p, cP.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
+    if (S.st.fdm) ELP_HIP(hipMemcpyAsync(S.coefF.p, cF.data(), sizeof(FdmCoef) * B, hipMemcpyHostToDevice, s));
+  };
+  auto finish = [&](int k, double rel) {
+    const int64_t fi = sl[k].f;
+    if (relres) relres[fi] = rel;
+    if (iters) iters[fi] = it_f[fi];
+    if (X) {
+      hipLaunchKernelGGL(unpack_col_kernel, dim3(nblk(m.n, tpb)), dim3(tpb), 0, s, m.n, B, k, x64.p, col.p);
+      ELP_HIP(hipGetLastError());
+      ELP_HIP(hipMemcpyAsync(X + 2 * fi * m.n, col.p, sizeof(double2) * m.n, hipMemcpyDeviceToHost, s));
+    }
+  };
+  // start: fill the slots, x64 = 0, r64 = b, inner rhs = b / ||b||
+  auto start_columns = [&]() {
+    ELP_HIP(hipMemcpyAsync(d_fresh.p, fresh.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(slot_fresh_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, d_fresh.p, F.p, x64.p,
+                       r64.p);
+    ELP_HIP(hipMemcpyAsync(d_scale.p, scale.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+    ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((slot_restart_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.active.p,
+                       d_scale.p, r64.p, S.ri.p, S.xi.p, perm);
+    ELP_HIP(hipGetLastError());
+    std::fill(fresh.begin(), fresh.end(), 0);
+  };
+  for (int k = 0; k < B; k++) assign(k);
+  upload_coefs();
+  start_columns();
+
+  const int freeze_batch = std::max(1, B / 4);
+  const double inner_tol = o.inner_rtol;
+  while (true) {
+    int n_iter = 0;
+    for (int k = 0; k < B; k++) n_iter += sl[k].state == 1;
+    if (n_iter == 0) break;
+    // z = M r, rho' = r.z, beta (0 for restarting columns), p = z + beta p
+    S.precond(S.ri.p, S.zi.p);
+    S.reduce(m.n, S.ri.p, S.zi.p, 0, S.rho_new.p);
+    hipLaunchKernelGGL(cocg_beta_kernel, dim3(1), dim3(64), 0, s, B, S.rho_new.p, S.rho.p, S.active.p, S.beta.p);
+    hipLaunchKernelGGL((cocg_dir_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.beta.p, S.zi.p,
+                       S.pi.p);
+    ELP_HIP(hipGetLastError());
+    bool restarted = false;
+    for (int k = 0; k < B; k++)
+      if (act[k] == 2) {
+        act[k] = 1;
+        restarted = true;
+      }
+    if (restarted) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    // q = A p, alpha = rho / p.q, x += alpha p, r -= alpha q
+    if (S.timing) ELP_HIP(hipEventRecord(S.ev0, s));
+    S.template op<0>(0, S.coefA.p, S.pi.p, nullptr, S.qi.p);
+    if (S.timing) ELP_HIP(hipEventRecord(S.ev1, s));
+    S.spmv_calls++;
+    S.reduce(m.n, S.pi.p, S.qi.p, 0, S.mu.p);
+    hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
+    hipLaunchKernelGGL((cocg_update_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.alpha.p, S.pi.p,
+                       S.qi.p, S.xi.p, S.ri.p);
+    S.reduce(m.n, S.ri.p, S.ri.p, 1, S.red_out.p);
+    S.fetch(S.red_out.p);
+    if (S.timing) {
+      float ms = 0;
+      ELP_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+      S.spmv_ms += ms;
+    }
+    int n_frozen = 0;
+    n_iter = 0;
+    bool changed = false;
+    for (int k = 0; k < B; k++) {
+      Slot &q = sl[k];
+      if (q.state == 1) {
+        it_f[q.f]++;
+        const double nr = std::sqrt(S.h_red[k].x);
+        const bool bad = !std::isfinite(nr) || nr > 1e6;
+        if (bad || !(nr > inner_tol) || it_f[q.f] >= o.max_iters) {
+          q.state = 2;
+          q.broken = bad;
+          act[k] = 0;
+          changed = true;
+        }
+      }
+      n_frozen += q.state == 2;
+      n_iter += q.state == 1;
+    }
+    if (changed) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    if (n_frozen == 0 || (n_frozen < freeze_batch && n_iter > 0)) continue;
+
+    // FP64 refinement of the frozen columns
+    refinements++;
+    for (int k = 0; k < B; k++) wgt[k] = (sl[k].state == 2 && !sl[k].broken) ? sl[k].w : 0.0;
+    ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
+                       iperm);
+    S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+    S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+    S.fetch(S.red_out.p);
+    std::vector<double> rel(B);
+    for (int k = 0; k < B; k++) rel[k] = std::sqrt(S.h_red[k].x) / nbn;
+    // a step that did not reduce the residual is taken back
+    bool any_undo = false;
+    std::vector<int> undo(B, 0);
+    for (int k = 0; k < B; k++) {
+      wgt[k] = 0.0;
+      if (sl[k].state != 2 || sl[k].broken) continue;
+      if (!(rel[k] < sl[k].prev)) {
+        undo[k] = 1;
+        wgt[k] = -sl[k].w;
+        any_undo = true;
+      }
+    }
+    if (any_undo) {
+      ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p,
+                         x64.p, iperm);
+      ELP_HIP(hipGetLastError());
+    }
+    bool any_new = false;
+    for (int k = 0; k < B; k++) {
+      Slot &q = sl[k];
+      if (q.state != 2) continue;
+      const bool kept_back = q.broken || undo[k];
+      const bool done = kept_back || rel[k] <= o.rtol || rel[k] > 0.5 * q.prev || it_f[q.f] >= o.max_iters;
+      if (done) {
+        finish(k, kept_back ? q.prev : rel[k]);
+        any_new |= assign(k);
+      } else {
+        q.prev = rel[k];
+        q.w = rel[k] * nbn;
+        q.state = 1;
+        act[k] = 2;
+        scale[k] = 1.0 / q.w;
+      }
+    }
+    // the stream is idle here only after a sync: the host mirrors are reused
+    ELP_HIP(hipStreamSynchronize(s));
+    if (any_new) upload_coefs();
+    start_columns();
+  }
+  ELP_HIP(hipStreamSynchronize(s));
+}
+
 template <typename T, int B>
 static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double *freqs, std::complex<double> Z,
                       const elp_solve_opts &o, double *X, double *relres, int32_t *iters, elp_sweep_stats *stats,
                       hipStream_t s) {
   Solver<T, B> S(m, st, o, s);
   int refinements = 0;
-  for (int64_t f0 = 0; f0 < nf; f0 += B) {
-    const int nb = (int)std::min<int64_t>(B, nf - f0);
-    solve_block<T, B>(S, freqs + f0, nb, Z, X ? X + 2 * f0 * m.n : nullptr, relres ? relres + f0 : nullptr,
-                      iters ? iters + f0 : nullptr, refinements);
+  if (std::is_same<T, float>::value) {
+    solve_slots<T, B>(S, nf, freqs, Z, X, relres, iters, refinements);
+  } else {
+    for (int64_t f0 = 0; f0 < nf; f0 += B) {
+      const int nb = (int)std::min<int64_t>(B, nf - f0);
+      solve_block<T, B>(S, freqs + f0, nb, Z, X ? X + 2 * f0 * m.n : nullptr, relres ? relres + f0 : nullptr,
+                        iters ? iters + f0 : nullptr, refinements);
+    }
   }
   if (stats) {
     stats->spmv_calls = S.spmv_calls;

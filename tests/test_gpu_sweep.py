@@ -111,3 +111,23 @@ def test_coupled_sweep_jacobi_small(ctx):
     # the thin clamped plate makes A(w) ill-conditioned (cond ~ 1e5): a 1e-8
     # residual bounds the error only to ~1e-3
     _check_solution(m, freqs, X, rel, err_tol=1e-3)
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_frequency_alone_equals_mixed(ctx, quadratic):
+    """Continuous batching: a frequency solved alone in the block gives the
+    same bits as the same frequency solved among others (slots are refilled as
+    frequencies converge, columns never mix), and more frequencies than slots
+    all reach the bar."""
+    m = MG.acoustic_cavity(10, 8, 6, quadratic=quadratic)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = MG.linear_frequencies(15, 1500, 11)
+    print(f"[sweep] alone-vs-mixed quadratic={quadratic}", flush=True)
+    Xmix, rmix, imix, _ = g.sweep(freqs, block=4)
+    assert (rmix <= 1e-8).all(), rmix
+    for k in (0, 5, 10):
+        Xa, ra, ia, _ = g.sweep(freqs[k:k + 1], block=4)
+        assert np.array_equal(Xa[0].view(np.uint64), Xmix[k].view(np.uint64)), k
+        assert ra[0] == rmix[k] and ia[0] == imix[k]
+    _check_solution(m, freqs[::5], Xmix[::5], rmix[::5])
