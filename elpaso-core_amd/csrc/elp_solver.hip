@@ -97,6 +97,77 @@ template <int B> struct OpShape {
   static constexpr int RPB = 256 / G;        // rows per block
 };
 
+// Row epilogue shared by the operator kernels: combine the K, M and extra
+// sums with the frequency's coefficients and apply MODE.
+template <typename T, int B, int MODE>
+__device__ inline void op_finish(const OpArgs<T> &a, const OpCoef &cf, int64_t r, int f, uint8_t fld,
+                                 typename VT<T>::t accK, typename VT<T>::t accM, typename VT<T>::t accX,
+                                 const typename VT<T>::t *__restrict__ x, const typename VT<T>::t *__restrict__ b,
+                                 typename VT<T>::t *__restrict__ y) {
+  using V = typename VT<T>::t;
+  const int64_t o = r * B + f;
+  const bool fl = fld == 1;
+  const V ck = fl ? V{(T)cf.kf.x, (T)cf.kf.y} : V{(T)cf.ks.x, (T)cf.ks.y};
+  const V cm = fl ? V{(T)cf.mf.x, (T)cf.mf.y} : V{(T)cf.ms.x, (T)cf.ms.y};
+  V ax = cmul(ck, accK), bx = cmul(cm, accM);
+  V res;
+  res.x = ax.x + bx.x + accX.x;
+  res.y = ax.y + bx.y + accX.y;
+  if (MODE == 0) {
+    y[o] = res;
+    return;
+  }
+  const V bv = b[o];
+  if (MODE == 1) {
+    V t; t.x = bv.x - res.x; t.y = bv.y - res.y;
+    y[o] = t;
+    return;
+  }
+  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+  const T inv = (T)(1.0 / d);
+  if (MODE == 3) {
+    V t; t.x = bv.x * inv; t.y = bv.y * inv;
+    y[o] = t;
+    return;
+  }
+  const V xv = x[o];
+  V t;
+  t.x = xv.x + (bv.x - res.x) * inv;
+  t.y = xv.y + (bv.y - res.y) * inv;
+  y[o] = t;
+}
+
+// Dirichlet row: the operator is the identity
+template <typename T, int B, int MODE>
+__device__ inline void op_identity_row(int64_t r, int f, const typename VT<T>::t *__restrict__ x,
+                                       const typename VT<T>::t *__restrict__ b, typename VT<T>::t *__restrict__ y) {
+  using V = typename VT<T>::t;
+  const int64_t o = r * B + f;
+  if (MODE == 0) y[o] = x[o];
+  else if (MODE == 1) { V t; t.x = b[o].x - x[o].x; t.y = b[o].y - x[o].y; y[o] = t; }
+  else y[o] = b[o];
+}
+
+// Extra entries of row r (Robin face C_G: scaled by i/(w Z); S: as is)
+template <typename T, int B>
+__device__ inline typename VT<T>::t op_extras(const OpArgs<T> &a, const OpCoef &cf, int64_t r, int f,
+                                              const typename VT<T>::t *__restrict__ x) {
+  using V = typename VT<T>::t;
+  V accX = {0, 0};
+  const V cg = {(T)cf.g.x, (T)cf.g.y};
+  for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
+    const V xv = x[(int64_t)a.xcol[k] * B + f];
+    const T v = a.xval[k];
+    if (a.xtype[k] == 0) {
+      V t = cmul(cg, xv);
+      accX.x += v * t.x; accX.y += v * t.y;
+    } else {
+      accX.x += v * xv.x; accX.y += v * xv.y;
+    }
+  }
+  return accX;
+}
+
 template <typename T, int B, int MODE>
 __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__restrict__ coef,
                                                  const typename VT<T>::t *__restrict__ x,
@@ -116,12 +187,7 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
   const uint8_t fld = a.field[r];
   const OpCoef &cf = coef[f];
   if (fld == 2) {  // Dirichlet row: identity
-    if (sub == 0) {
-      const int64_t o = r * B + f;
-      if (MODE == 0) y[o] = x[o];
-      else if (MODE == 1) { V t; t.x = b[o].x - x[o].x; t.y = b[o].y - x[o].y; y[o] = t; }
-      else y[o] = b[o];
-    }
+    if (sub == 0) op_identity_row<T, B, MODE>(r, f, x, b, y);
     return;
   }
   V accK = {0, 0}, accM = {0, 0}, accX = {0, 0};
@@ -157,51 +223,117 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
       accK.x += __shfl_xor(accK.x, o, G); accK.y += __shfl_xor(accK.y, o, G);
       accM.x += __shfl_xor(accM.x, o, G); accM.y += __shfl_xor(accM.y, o, G);
     }
-    if (sub == 0) {
-      const V cg = {(T)cf.g.x, (T)cf.g.y};
-      for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
-        const V xv = x[(int64_t)a.xcol[k] * B + f];
-        const T v = a.xval[k];
-        if (a.xtype[k] == 0) {
-          V t = cmul(cg, xv);
-          accX.x += v * t.x; accX.y += v * t.y;
-        } else {
-          accX.x += v * xv.x; accX.y += v * xv.y;
-        }
-      }
-    }
+    if (sub == 0) accX = op_extras<T, B>(a, cf, r, f, x);
   }
   if (sub != 0) return;
-  const int64_t o = r * B + f;
+  op_finish<T, B, MODE>(a, cf, r, f, fld, accK, accM, accX, x, b, y);
+}
+
+constexpr int kRowPad = 8;  // padded rows hold a multiple of this many entries
+constexpr int64_t kRowOpMaxRows = (1LL << 31) / 256;  // x of B = 32 within 31-bit byte offsets
+
+// Row-padded copy of the FP32 operator for row_op_kernel: each row holds a
+// multiple of kRowPad entries; entries store the byte offset of their column's
+// x line (column * 256) and padding entries an offset past the end of x
+// (which reads 0) with K = M = 0.
+struct PadCsr {
+  const int32_t *rp;
+  const uint32_t *off;
+  const float *K, *M;
+};
+
+// CH entries of a padded row: all loads of x are issued before the FMAs
+template <int CH>
+__device__ inline void row_chunk(const PadCsr &pc, int32_t base, __amdgpu_buffer_rsrc_t xr, uint32_t vo, float &aK,
+                                 float &aM) {
+  uint32_t so[CH];
+  float kv[CH], mv[CH], xv[CH];
+#pragma unroll
+  for (int u = 0; u < CH; u++) {
+    so[u] = pc.off[base + u];
+    kv[u] = pc.K[base + u];
+    mv[u] = pc.M[base + u];
+  }
+#pragma unroll
+  for (int u = 0; u < CH; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, so[u], 0));
+#pragma unroll
+  for (int u = 0; u < CH; u++) {
+    aK = fmaf(kv[u], xv[u], aK);
+    aM = fmaf(mv[u], xv[u], aM);
+  }
+}
+
+// Row-per-wavefront operator for B = 32 (FP32): lane l holds float l of the
+// row's 32 complex frequencies (frequency l / 2, real part for even l,
+// imaginary part for odd l).  K and M are real, so both parts accumulate
+// independently: the row's x offsets and K, M values are the same for every
+// lane and are read with scalar loads into scalar registers, and each nonzero
+// costs one coalesced 256-byte buffer load of x and two FMAs, with no
+// per-entry address arithmetic.  The complex coefficients are applied once per
+// row with the partner lane's part.  Entries are summed in row order.
+constexpr int kRowWg = 256;  // threads per workgroup: kRowWg / 64 consecutive rows
+
+template <int MODE>
+__global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr pc, const OpCoef *__restrict__ coef,
+                                                     const float *__restrict__ x, const float *__restrict__ b,
+                                                     float *__restrict__ y) {
+  constexpr int W = 64;  // floats per row: 32 complex frequencies
+  const int lane = threadIdx.x & 63;
+  const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
+  if (r >= a.n) return;
+  const int64_t o = r * W + lane;
+  const uint8_t fld = a.field[r];
+  if (fld == 2) {  // Dirichlet row: identity
+    if (MODE == 0) y[o] = x[o];
+    else if (MODE == 1) y[o] = b[o] - x[o];
+    else y[o] = b[o];
+    return;
+  }
+  float aK = 0.f, aM = 0.f, aG = 0.f, aS = 0.f;
+  if (MODE != 3) {
+    const int xbytes = (int)(a.n * W * (int64_t)sizeof(float));
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, xbytes, 0x00020000);
+    const uint32_t vo = (uint32_t)lane * sizeof(float);
+    int32_t base = pc.rp[r];
+    const int32_t p1 = pc.rp[r + 1];
+    for (; base + 2 * kRowPad <= p1; base += 2 * kRowPad) row_chunk<2 * kRowPad>(pc, base, xr, vo, aK, aM);
+    if (base < p1) row_chunk<kRowPad>(pc, base, xr, vo, aK, aM);
+    const float *xl = x + lane;
+    for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
+      const float xv = xl[(uint32_t)a.xcol[k] * (uint32_t)W];
+      if (a.xtype[k] == 0) aG = fmaf(a.xval[k], xv, aG);
+      else aS = fmaf(a.xval[k], xv, aS);
+    }
+  }
+  // partner part (real <-> imaginary) of each sum
+  const float pK = __shfl_xor(aK, 1), pM = __shfl_xor(aM, 1), pG = __shfl_xor(aG, 1);
+  const bool im = lane & 1;
+  const float2 K = im ? float2{pK, aK} : float2{aK, pK};
+  const float2 M = im ? float2{pM, aM} : float2{aM, pM};
+  const float2 Gs = im ? float2{pG, aG} : float2{aG, pG};
+  const OpCoef &cf = coef[lane >> 1];
   const bool fl = fld == 1;
-  const V ck = fl ? V{(T)cf.kf.x, (T)cf.kf.y} : V{(T)cf.ks.x, (T)cf.ks.y};
-  const V cm = fl ? V{(T)cf.mf.x, (T)cf.mf.y} : V{(T)cf.ms.x, (T)cf.ms.y};
-  V ax = cmul(ck, accK), bx = cmul(cm, accM);
-  V res;
-  res.x = ax.x + bx.x + accX.x;
-  res.y = ax.y + bx.y + accX.y;
+  const float2 ck = fl ? float2{(float)cf.kf.x, (float)cf.kf.y} : float2{(float)cf.ks.x, (float)cf.ks.y};
+  const float2 cm = fl ? float2{(float)cf.mf.x, (float)cf.mf.y} : float2{(float)cf.ms.x, (float)cf.ms.y};
+  const float2 cg = {(float)cf.g.x, (float)cf.g.y};
+  const float2 t1 = cmul(ck, K), t2 = cmul(cm, M), t3 = cmul(cg, Gs);
+  const float res = im ? (t1.y + t2.y + t3.y + aS) : (t1.x + t2.x + t3.x + aS);
   if (MODE == 0) {
     y[o] = res;
     return;
   }
-  const V bv = b[o];
+  const float bv = b[o];
   if (MODE == 1) {
-    V t; t.x = bv.x - res.x; t.y = bv.y - res.y;
-    y[o] = t;
+    y[o] = bv - res;
     return;
   }
   const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
-  const T inv = (T)(1.0 / d);
+  const float inv = (float)(1.0 / d);
   if (MODE == 3) {
-    V t; t.x = bv.x * inv; t.y = bv.y * inv;
-    y[o] = t;
+    y[o] = bv * inv;
     return;
   }
-  const V xv = x[o];
-  V t;
-  t.x = xv.x + (bv.x - res.x) * inv;
-  t.y = xv.y + (bv.y - res.y) * inv;
-  y[o] = t;
+  y[o] = x[o] + (bv - res) * inv;
 }
 
 // y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P (same
@@ -525,6 +657,11 @@ struct LevelData {
   DArr<int32_t> perm, iperm, rp32, ci32, xrp32, xcol32, P_rp32, P_ci32, R_ci32;
   DArr<uint8_t> xtype32, field32;
   DArr<double> rK32, rM32, rG32;
+  // row-padded FP32 operator (row_op_kernel, 32 frequencies)
+  DArr<int32_t> pad_rp;
+  DArr<uint32_t> pad_off;
+  DArr<float> pad_K, pad_M;
+  PadCsr padded() const { return {pad_rp.p, pad_off.p, pad_K.p, pad_M.p}; }
   template <typename T> OpArgs<T> args() const;
   struct Csr {
     const int32_t *rp, *ci;
@@ -694,6 +831,26 @@ __global__ void permute_rows_kernel(int64_t n, const int32_t *__restrict__ perm,
   }
 }
 
+// row-padded copy for row_op_kernel (one thread per row)
+__global__ void pad_rows_kernel(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                const float *__restrict__ K, const float *__restrict__ M,
+                                const int32_t *__restrict__ rp8, uint32_t *__restrict__ off8, float *__restrict__ K8,
+                                float *__restrict__ M8, uint32_t line_bytes, uint32_t oob) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int32_t q = rp8[r];
+  for (int32_t p = rp[r]; p < rp[r + 1]; p++, q++) {
+    off8[q] = (uint32_t)ci[p] * line_bytes;
+    K8[q] = K[p];
+    M8[q] = M[p];
+  }
+  for (; q < rp8[r + 1]; q++) {
+    off8[q] = oob;
+    K8[q] = 0.f;
+    M8[q] = 0.f;
+  }
+}
+
 template <typename T>
 __global__ void gather_perm_kernel(int64_t n, const int32_t *__restrict__ perm, const T *__restrict__ in,
                                    T *__restrict__ out) {
@@ -802,6 +959,22 @@ static void permute_level_float(LevelData &L, hipStream_t s) {
   L.M32 = std::move(Mn);
   L.X32 = std::move(Xn);
   L.permuted = true;
+  if (m.n <= kRowOpMaxRows) {
+    std::vector<int32_t> rp8(m.n + 1, 0);
+    for (int64_t r = 0; r < m.n; r++) {
+      const int32_t len = rpn[r + 1] - rpn[r];
+      rp8[r + 1] = rp8[r] + (len + kRowPad - 1) / kRowPad * kRowPad;
+    }
+    L.pad_rp.upload(c, rp8);
+    L.pad_off.alloc(c, rp8[m.n]);
+    L.pad_K.alloc(c, rp8[m.n]);
+    L.pad_M.alloc(c, rp8[m.n]);
+    const uint32_t line = 32 * sizeof(float2);
+    hipLaunchKernelGGL(pad_rows_kernel, dim3(g), dim3(256), 0, s, m.n, L.rp32.p, L.ci32.p, L.K32.p, L.M32.p,
+                       L.pad_rp.p, L.pad_off.p, L.pad_K.p, L.pad_M.p, line, (uint32_t)(m.n * line));
+    ELP_HIP(hipGetLastError());
+    ELP_HIP(hipStreamSynchronize(s));
+  }
 }
 
 // trilinear interpolation from a coarse box lattice to the rows of `fine`
@@ -1003,6 +1176,15 @@ struct Solver {
   void op(int l, const OpCoef *coef, const typename VT<TT>::t *x, const typename VT<TT>::t *b,
           typename VT<TT>::t *y) {
     const LevelData &L = *st.levels[l];
+    if constexpr (std::is_same<TT, float>::value && B == 32) {
+      if (L.pad_rp.p) {
+        hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
+                           L.template args<float>(), L.padded(), coef, reinterpret_cast<const float *>(x),
+                           reinterpret_cast<const float *>(b), reinterpret_cast<float *>(y));
+        ELP_HIP(hipGetLastError());
+        return;
+      }
+    }
     const int grid = xcd_grid(nblk(L.n, OpShape<B>::RPB));
     hipLaunchKernelGGL((op_kernel<TT, B, MODE>), dim3(grid), dim3(256), 0, s, L.template args<TT>(), coef, x, b, y);
     ELP_HIP(hipGetLastError());
@@ -1070,6 +1252,14 @@ struct Solver {
       transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), b, fb.p);
       fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
       transfer<false>(L.n, L.Pcsr<T>().rp, L.Pcsr<T>().ci, Pv(L), fe.p, z);
+      return;
+    }
+    if (o.smooth == 0) {
+      // additive two-level: z = D^-1 b + P A_box^-1 R b (no fine-level operator)
+      op<3>(0, coefP.p, nullptr, b, z);
+      transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), b, fb.p);
+      fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
+      transfer<true>(L.n, L.Pcsr<T>().rp, L.Pcsr<T>().ci, Pv(L), fe.p, z);
       return;
     }
     const int nu = std::max(1, o.smooth);

@@ -5,5 +5,5 @@ its C-ABI (``include/elpaso_amd.h``); this package only marshals arrays.  It
 raises if the library is missing: there is no CPU fallback.
 """
 from .model import (BRICK8, BRICK20, FLUID8, FLUID20, ElementGroup, Material, Model)  # noqa: F401
-from ._lib import (ElpError, Context, GpuModel, SolveOpts, library_path, load_library,  # noqa: F401
+from ._lib import (ElpError, Context, GpuModel, Server, SolveOpts, library_path, load_library,  # noqa: F401
                    EXPORTED_SYMBOLS)

@@ -282,3 +282,57 @@ class GpuModel:
         _check(load_library().elp_sweep(self._h, nf, _ptr(freqs), Z.real, Z.imag, C.byref(o),
                                         _ptr(X) if X is not None else None, _ptr(rel), _ptr(it), C.byref(st)))
         return X, rel, it, st
+
+
+class Server:
+    """Continuous-batching sweep server (wraps elp_server*).  Requests are
+    admitted in arrival order as slots and the HBM budget allow; each runs
+    on its own stream, overlapping with the others on the device."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_slots: int = 4):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        self._n = {}
+        _check(load_library().elp_server_create(ctx.handle, int(hbm_budget_bytes), int(max_slots),
+                                                C.byref(self._h)))
+
+    def submit(self, gmodel: "GpuModel", freqs, Z=0j, opts=None, **kw):
+        """Queue a sweep of ``gmodel`` (the server takes the model over)."""
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        o = opts if opts is not None else default_opts(**kw)
+        jid = C.c_int64()
+        Z = complex(Z)
+        _check(load_library().elp_server_submit(self._h, gmodel.handle, len(freqs), _ptr(freqs), Z.real, Z.imag,
+                                                C.byref(o), C.byref(jid)))
+        gmodel.release()
+        self._n[jid.value] = (gmodel.n, len(freqs))
+        return jid.value
+
+    def poll(self, job):
+        return load_library().elp_server_poll(self._h, int(job)) == 1
+
+    def wait(self, job, want_x=True):
+        n, nf = self._n.pop(job)
+        X = np.zeros((nf, n), dtype=np.complex128) if want_x else None
+        rel = np.zeros(nf)
+        it = np.zeros(nf, dtype=np.int32)
+        lat = C.c_double()
+        _check(load_library().elp_server_wait(self._h, int(job), _ptr(X) if X is not None else None, _ptr(rel),
+                                              _ptr(it), C.byref(lat)))
+        return X, rel, it, lat.value
+
+    def stats(self):
+        a, q, b = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(load_library().elp_server_stats(self._h, C.byref(a), C.byref(q), C.byref(b)))
+        return {"admitted": a.value, "queued": q.value, "bytes_in_use": b.value}
+
+    def close(self):
+        if self._h:
+            load_library().elp_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -131,3 +131,17 @@ def test_frequency_alone_equals_mixed(ctx, quadratic):
         assert np.array_equal(Xa[0].view(np.uint64), Xmix[k].view(np.uint64)), k
         assert ra[0] == rmix[k] and ia[0] == imix[k]
     _check_solution(m, freqs[::5], Xmix[::5], rmix[::5])
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_sweep_32_slots(ctx, quadratic):
+    """32 slots (the row-per-wavefront FP32 operator) with more frequencies
+    than slots: every frequency reaches the bar and matches SuperLU."""
+    m = MG.acoustic_cavity(12, 10, 8, quadratic=quadratic)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = MG.linear_frequencies(20, 900, 40)  # resolved on this coarse mesh
+    print(f"[sweep] 32 slots quadratic={quadratic}", flush=True)
+    X, rel, it, st = g.sweep(freqs, block=32)
+    assert (rel <= 1e-8).all(), rel
+    _check_solution(m, freqs[::7], X[::7], rel[::7])
