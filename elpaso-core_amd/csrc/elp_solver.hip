@@ -398,43 +398,108 @@ __global__ __launch_bounds__(256) void transfer_kernel(int64_t n, const int32_t 
 
 // Deterministic per-frequency reductions over [n][B] vectors.
 //   mode 0: sum a*b (unconjugated), mode 1: sum |a|^2
-constexpr int kRedBlocks = 512;
+// Block k of kRedBlocks sums rows [k*chunk, (k+1)*chunk) with lane (f, rl)
+// taking rows rl, rl + R, ... (R = 256 / B) in order, four loads in flight;
+// the per-block partials are combined by dot_final_kernel in a fixed tree.
+constexpr int kRedBlocks = 2048;
+constexpr int kRedFinalThreads = 1024;
+
+template <typename V>
+__device__ inline void red_term(double2 &acc, V av, V bv, int mode) {
+  if (mode == 0) {
+    acc.x += (double)av.x * bv.x - (double)av.y * bv.y;
+    acc.y += (double)av.x * bv.y + (double)av.y * bv.x;
+  } else {
+    acc.x += (double)av.x * av.x + (double)av.y * av.y;
+  }
+}
+
+// block tree over the R row-lanes of each frequency; lane rl == 0 writes
+__device__ inline void red_block_store(double2 acc, int B, double2 *__restrict__ partial) {
+  __shared__ double2 sh[256];
+  const int t = threadIdx.x, rl = t / B, R = 256 / B;
+  sh[t] = acc;
+  __syncthreads();
+  for (int st = R / 2; st > 0; st >>= 1) {
+    if (rl < st) {
+      sh[t].x += sh[t + st * B].x;
+      sh[t].y += sh[t + st * B].y;
+    }
+    __syncthreads();
+  }
+  if (rl == 0) partial[blockIdx.x * B + t] = sh[t];
+}
+
 template <typename V>
 __global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__restrict__ a, const V *__restrict__ b,
                                                   int mode, double2 *__restrict__ partial) {
-  __shared__ double2 sh[256];
   const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
   double2 acc = {0, 0};
-  for (int64_t r = r0 + rl; r < r1; r += R) {
-    const V av = a[r * B + f];
-    if (mode == 0) {
-      const V bv = b[r * B + f];
-      acc.x += (double)av.x * bv.x - (double)av.y * bv.y;
-      acc.y += (double)av.x * bv.y + (double)av.y * bv.x;
-    } else {
-      acc.x += (double)av.x * av.x + (double)av.y * av.y;
+  int64_t r = r0 + rl;
+  for (; r + 3 * R < r1; r += 4 * R) {
+    V av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      av[u] = a[(r + u * R) * B + f];
+      bv[u] = mode == 0 ? b[(r + u * R) * B + f] : av[u];
     }
+#pragma unroll
+    for (int u = 0; u < 4; u++) red_term(acc, av[u], bv[u], mode);
   }
-  sh[t] = acc;
-  __syncthreads();
-  for (int s = R / 2; s > 0; s >>= 1) {
-    if (rl < s) {
-      sh[t].x += sh[t + s * B].x;
-      sh[t].y += sh[t + s * B].y;
-    }
-    __syncthreads();
-  }
-  if (rl == 0) partial[blockIdx.x * B + f] = sh[t];
+  for (; r < r1; r += R) red_term(acc, a[r * B + f], mode == 0 ? b[r * B + f] : a[r * B + f], mode);
+  red_block_store(acc, B, partial);
 }
 
-__global__ __launch_bounds__(256) void dot_final_kernel(int nblk, int B, const double2 *__restrict__ partial,
-                                                        double2 *__restrict__ out) {
-  __shared__ double2 sh[256];
+// x += alpha p ; r -= alpha q ; partial sums of |r|^2 (dot_kernel's layout)
+template <typename V>
+__global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B, const double2 *__restrict__ alpha,
+                                                               const V *__restrict__ p, const V *__restrict__ q,
+                                                               V *__restrict__ x, V *__restrict__ r,
+                                                               double2 *__restrict__ partial) {
   const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+  const double2 a = alpha[f];
+  const bool upd = !(a.x == 0 && a.y == 0);  // frozen column: x, r untouched
   double2 acc = {0, 0};
-  for (int k = rl; k < nblk; k += R) {
+  for (int64_t row = r0 + rl; row < r1; row += R) {
+    const int64_t i = row * B + f;
+    V rv = r[i];
+    if (upd) {
+      const V pv = p[i], qv = q[i];
+      V xv = x[i];
+      xv.x += a.x * pv.x - a.y * pv.y;
+      xv.y += a.x * pv.y + a.y * pv.x;
+      rv.x -= a.x * qv.x - a.y * qv.y;
+      rv.y -= a.x * qv.y + a.y * qv.x;
+      x[i] = xv;
+      r[i] = rv;
+    }
+    acc.x += (double)rv.x * rv.x + (double)rv.y * rv.y;
+  }
+  red_block_store(acc, B, partial);
+}
+
+__global__ __launch_bounds__(kRedFinalThreads) void dot_final_kernel(int nblk, int B,
+                                                                     const double2 *__restrict__ partial,
+                                                                     double2 *__restrict__ out) {
+  __shared__ double2 sh[kRedFinalThreads];
+  const int t = threadIdx.x, f = t % B, rl = t / B, R = kRedFinalThreads / B;
+  double2 acc = {0, 0};
+  int k = rl;
+  for (; k + 7 * R < nblk; k += 8 * R) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = partial[(k + u * R) * B + f];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      acc.x += v[u].x;
+      acc.y += v[u].y;
+    }
+  }
+  for (; k < nblk; k += R) {
     acc.x += partial[k * B + f].x;
     acc.y += partial[k * B + f].y;
   }
@@ -475,24 +540,6 @@ __global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, co
   }
   beta[f] = b;
   rho[f] = rho_new[f];
-}
-
-// x += alpha p ; r -= alpha q
-template <typename V>
-__global__ void cocg_update_kernel(int64_t nt, int B, const double2 *__restrict__ alpha, const V *__restrict__ p,
-                                   const V *__restrict__ q, V *__restrict__ x, V *__restrict__ r) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
-  const double2 a = alpha[i % B];
-  if (a.x == 0 && a.y == 0) return;  // frozen column: x, r untouched
-  const V pv = p[i], qv = q[i];
-  V xv = x[i], rv = r[i];
-  xv.x += a.x * pv.x - a.y * pv.y;
-  xv.y += a.x * pv.y + a.y * pv.x;
-  rv.x -= a.x * qv.x - a.y * qv.y;
-  rv.y -= a.x * qv.y + a.y * qv.x;
-  x[i] = xv;
-  r[i] = rv;
 }
 
 // p = z + beta p
@@ -1202,8 +1249,17 @@ struct Solver {
   template <typename VV>
   void reduce(int64_t n, const VV *a, const VV *b, int mode, double2 *dev_out) {
     hipLaunchKernelGGL(dot_kernel<VV>, dim3(kRedBlocks), dim3(256), 0, s, n, B, a, b, mode, red_part.p);
-    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, kRedBlocks, B, red_part.p, dev_out);
+    final_reduce(dev_out);
+  }
+  void final_reduce(double2 *dev_out) {
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kRedFinalThreads), 0, s, kRedBlocks, B, red_part.p, dev_out);
     ELP_HIP(hipGetLastError());
+  }
+  // x += alpha p, r -= alpha q, red_out = |r|^2 per frequency (level 0)
+  void update_norm() {
+    hipLaunchKernelGGL((cocg_update_norm_kernel<V>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, alpha.p, pi.p, qi.p,
+                       xi.p, ri.p, red_part.p);
+    final_reduce(red_out.p);
   }
   void fetch(const double2 *dev) {
     ELP_HIP(hipMemcpyAsync(h_red, dev, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
@@ -1300,9 +1356,7 @@ struct Solver {
       spmv_calls++;
       reduce(m.n, pi.p, qi.p, 0, mu.p);
       hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, rho.p, mu.p, active.p, alpha.p);
-      hipLaunchKernelGGL((cocg_update_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, alpha.p, pi.p, qi.p,
-                         xi.p, ri.p);
-      reduce(m.n, ri.p, ri.p, 1, red_out.p);
+      update_norm();
       fetch(red_out.p);
       if (timing) {
         float ms = 0;
@@ -1693,9 +1747,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     S.spmv_calls++;
     S.reduce(m.n, S.pi.p, S.qi.p, 0, S.mu.p);
     hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
-    hipLaunchKernelGGL((cocg_update_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.alpha.p, S.pi.p,
-                       S.qi.p, S.xi.p, S.ri.p);
-    S.reduce(m.n, S.ri.p, S.ri.p, 1, S.red_out.p);
+    S.update_norm();
     S.fetch(S.red_out.p);
     if (S.timing) {
       float ms = 0;
