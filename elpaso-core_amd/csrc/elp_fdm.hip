@@ -12,6 +12,8 @@
 
 namespace elp {
 
+constexpr int kFdmMaxAxis = 256;  // nodes along x and y of the box lattice
+
 struct FdmBox {
   int dims[3];  // px, py, pz (nodes per direction)
   double h[3];
@@ -54,6 +56,7 @@ std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m) {
   for (auto v : m.row_field)
     if (v != 1) return none;
   if (m.box.robin_sides & ~((1 << 2) | (1 << 4))) return none;  // Robin only on z sides
+  if (m.box.nel[0] + 1 > kFdmMaxAxis || m.box.nel[1] + 1 > kFdmMaxAxis) return none;
   // the model's corner nodes must form the uniform lattice of the box
   auto f = std::unique_ptr<FdmBox, void (*)(FdmBox *)>(new FdmBox(), fdm_free);
   for (int d = 0; d < 3; d++) {
@@ -85,28 +88,80 @@ std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m) {
   return f;
 }
 
-// out[o][p][c] = sum_l Mat[p][l] in[o][l][c], the tile in[o][:][c0:c0+TW] staged in LDS
-template <typename T>
-__global__ __launch_bounds__(256) void axis_transform_kernel(int L, int64_t inner, int tw, const T *__restrict__ Mat,
-                                                             const T *__restrict__ in, T *__restrict__ out) {
+// out[o][p][c] = sum_l Mat[p][l] in[o][l][c].  A workgroup owns CW "columns"
+// (o, c): a tile c0 .. c0+CW of one line o when inner >= CW, or
+// floor(CW / inner) whole lines when inner < CW.  The columns' values for
+// all l are staged in LDS once; Mat is staged by chunks of KC columns,
+// transposed.  Thread (pg, cg) accumulates the 4 columns 4cg .. 4cg+3 for the
+// NP modes p = pg, pg + PG, ... in registers, so each LDS read of the tile
+// feeds 4 NP FMAs.  Every output sums over l in increasing order.
+constexpr int kAxisKC = 16;
+template <typename T, int CW, int NP>
+__global__ __launch_bounds__(256) void axis_transform_kernel(int64_t outer, int L, int64_t inner,
+                                                             const T *__restrict__ Mat, const T *__restrict__ in,
+                                                             T *__restrict__ out) {
+  constexpr int CG = CW / 4, PG = 256 / CG;
   extern __shared__ unsigned char smem[];
-  T *tile = reinterpret_cast<T *>(smem);
-  const int64_t o = blockIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.y * tw;
-  const int w = (int)min((int64_t)tw, inner - c0);
-  const T *src = in + o * L * inner + c0;
-  for (int idx = threadIdx.x; idx < L * w; idx += blockDim.x) {
-    const int l = idx / w, cc = idx % w;
-    tile[l * tw + cc] = src[(int64_t)l * inner + cc];
+  T *tile = reinterpret_cast<T *>(smem);  // [L][CW]
+  T *mt = tile + (size_t)L * CW;          // [KC][L]: mt[k][p] = Mat[p][l0 + k]
+  // column j of this workgroup -> offset of (o, l = 0, c), or -1 past the end
+  auto colbase = [&](int j) -> int64_t {
+    int64_t o, c;
+    if (inner >= CW) {
+      o = blockIdx.x;
+      c = (int64_t)blockIdx.y * CW + j;
+      if (c >= inner) return -1;
+    } else {
+      const int lines = CW / (int)inner;
+      if (j >= lines * inner) return -1;
+      o = (int64_t)blockIdx.x * lines + j / inner;
+      c = j % inner;
+      if (o >= outer) return -1;
+    }
+    return o * L * inner + c;
+  };
+  for (int idx = threadIdx.x; idx < L * CW; idx += 256) {
+    const int l = idx / CW, cc = idx % CW;
+    const int64_t bse = colbase(cc);
+    tile[idx] = bse >= 0 ? in[bse + (int64_t)l * inner] : T(0);
   }
-  __syncthreads();
-  T *dst = out + o * L * inner + c0;
-  for (int idx = threadIdx.x; idx < L * w; idx += blockDim.x) {
-    const int p = idx / w, cc = idx % w;
-    const T *mrow = Mat + (int64_t)p * L;
-    T acc = 0;
-    for (int l = 0; l < L; l++) acc += mrow[l] * tile[l * tw + cc];
-    dst[(int64_t)p * inner + cc] = acc;
+  const int cg = threadIdx.x % CG, pg = threadIdx.x / CG;
+  T acc[NP][4];
+#pragma unroll
+  for (int i = 0; i < NP; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0;
+  for (int l0 = 0; l0 < L; l0 += kAxisKC) {
+    const int kc = min(kAxisKC, L - l0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kc * L; idx += 256) {
+      const int p = idx / kc, k = idx % kc;
+      mt[k * L + p] = Mat[(int64_t)p * L + l0 + k];
+    }
+    __syncthreads();
+    for (int k = 0; k < kc; k++) {
+      const T *tr = tile + (l0 + k) * CW + 4 * cg;
+      const T t0 = tr[0], t1 = tr[1], t2 = tr[2], t3 = tr[3];
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const int p = pg + i * PG;
+        const T mv = p < L ? mt[k * L + p] : T(0);
+        acc[i][0] += mv * t0;
+        acc[i][1] += mv * t1;
+        acc[i][2] += mv * t2;
+        acc[i][3] += mv * t3;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int64_t bse = colbase(4 * cg + j);
+    if (bse < 0) continue;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const int p = pg + i * PG;
+      if (p < L) out[bse + (int64_t)p * inner] = acc[i][j];
+    }
   }
 }
 
@@ -191,15 +246,30 @@ __global__ void zsolve_kernel(int px, int py, int pz, int B, double hz, int rlo,
   }
 }
 
+template <typename T, int CW, int NP>
+static void launch_axis(int64_t outer, int L, int64_t inner, const T *Mat, const T *in, T *out, hipStream_t s) {
+  const size_t lds = ((size_t)L * CW + (size_t)kAxisKC * L) * sizeof(T);
+  const auto kern = axis_transform_kernel<T, CW, NP>;
+  ELP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const dim3 grid = inner >= CW ? dim3((unsigned)outer, (unsigned)((inner + CW - 1) / CW))
+                                 : dim3((unsigned)((outer + CW / inner - 1) / (CW / inner)), 1);  // CW/inner lines
+  hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, outer, L, inner, Mat, in, out);
+  ELP_HIP(hipGetLastError());
+}
+
+// transform along an axis of L <= kFdmMaxAxis points; inner counts reals
 template <typename T, typename V>
 static void transform(int64_t outer, int L, int64_t inner, const T *Mat, const V *in, V *out, hipStream_t s) {
-  // inner counts reals
-  int tw = (int)std::min<int64_t>(inner, 64);
-  while ((int64_t)L * tw * sizeof(T) > 48 * 1024 && tw > 1) tw /= 2;
-  dim3 grid((unsigned)outer, (unsigned)((inner + tw - 1) / tw));
-  hipLaunchKernelGGL(axis_transform_kernel<T>, grid, dim3(256), (size_t)L * tw * sizeof(T), s, L, inner, tw, Mat,
-                     reinterpret_cast<const T *>(in), reinterpret_cast<T *>(out));
-  ELP_HIP(hipGetLastError());
+  constexpr int CW = sizeof(T) == 4 ? 64 : 32;
+  ELP_REQUIRE(L <= kFdmMaxAxis && inner > 0, ELP_EINVAL, "box transform shape");
+  constexpr int PG = 256 / (CW / 4);
+  const T *i = reinterpret_cast<const T *>(in);
+  T *o = reinterpret_cast<T *>(out);
+  const int np = (L + PG - 1) / PG;
+  if (np <= 2) launch_axis<T, CW, 2>(outer, L, inner, Mat, i, o, s);
+  else if (np <= 4) launch_axis<T, CW, 4>(outer, L, inner, Mat, i, o, s);
+  else if (np <= 8) launch_axis<T, CW, 8>(outer, L, inner, Mat, i, o, s);
+  else launch_axis<T, CW, 16>(outer, L, inner, Mat, i, o, s);
 }
 
 template <typename T, typename V>
