@@ -263,6 +263,33 @@ __device__ inline void row_chunk(const PadCsr &pc, int32_t base, __amdgpu_buffer
   }
 }
 
+// First l1-Jacobi sweep from a zero guess, y = b / D, one thread per
+// (row, frequency): the same values as op_kernel's MODE 3.
+template <typename T>
+__global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const OpCoef *__restrict__ coef,
+                                                      const typename VT<T>::t *__restrict__ b,
+                                                      typename VT<T>::t *__restrict__ y) {
+  using V = typename VT<T>::t;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n * B) return;
+  const int64_t r = i / B;
+  const int f = (int)(i % B);
+  const uint8_t fld = a.field[r];
+  const V bv = b[i];
+  if (fld == 2) {
+    y[i] = bv;
+    return;
+  }
+  const OpCoef &cf = coef[f];
+  const bool fl = fld == 1;
+  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+  const T inv = (T)(1.0 / d);
+  V t;
+  t.x = bv.x * inv;
+  t.y = bv.y * inv;
+  y[i] = t;
+}
+
 // Row-per-wavefront operator for B = 32 (FP32): lane l holds float l of the
 // row's 32 complex frequencies (frequency l / 2, real part for even l,
 // imaginary part for odd l).  K and M are real, so both parts accumulate
@@ -289,6 +316,9 @@ __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr 
     else y[o] = b[o];
     return;
   }
+  // the row's own values are loaded first so their latency overlaps the gathers
+  const float bv = MODE == 0 ? 0.f : b[o];
+  const float xo = MODE == 2 ? x[o] : 0.f;
   float aK = 0.f, aM = 0.f, aG = 0.f, aS = 0.f;
   if (MODE != 3) {
     const int xbytes = (int)(a.n * W * (int64_t)sizeof(float));
@@ -322,7 +352,6 @@ __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr 
     y[o] = res;
     return;
   }
-  const float bv = b[o];
   if (MODE == 1) {
     y[o] = bv - res;
     return;
@@ -333,7 +362,7 @@ __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr 
     y[o] = bv * inv;
     return;
   }
-  y[o] = x[o] + (bv - res) * inv;
+  y[o] = xo + (bv - res) * inv;
 }
 
 // y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P (same
@@ -1223,6 +1252,12 @@ struct Solver {
   void op(int l, const OpCoef *coef, const typename VT<TT>::t *x, const typename VT<TT>::t *b,
           typename VT<TT>::t *y) {
     const LevelData &L = *st.levels[l];
+    if constexpr (MODE == 3) {
+      hipLaunchKernelGGL((jacobi0_kernel<TT>), dim3(nblk(L.n * B, 256)), dim3(256), 0, s, L.template args<TT>(), B,
+                         coef, b, y);
+      ELP_HIP(hipGetLastError());
+      return;
+    }
     if constexpr (std::is_same<TT, float>::value && B == 32) {
       if (L.pad_rp.p) {
         hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
