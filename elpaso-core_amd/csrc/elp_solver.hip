@@ -301,13 +301,10 @@ __global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const 
 constexpr int kRowWg = 256;  // threads per workgroup: kRowWg / 64 consecutive rows
 
 template <int MODE>
-__global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr pc, const OpCoef *__restrict__ coef,
-                                                     const float *__restrict__ x, const float *__restrict__ b,
-                                                     float *__restrict__ y) {
+__device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const OpCoef *__restrict__ coef,
+                                 const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
+                                 int64_t r, int lane) {
   constexpr int W = 64;  // floats per row: 32 complex frequencies
-  const int lane = threadIdx.x & 63;
-  const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
-  if (r >= a.n) return;
   const int64_t o = r * W + lane;
   const uint8_t fld = a.field[r];
   if (fld == 2) {  // Dirichlet row: identity
@@ -363,6 +360,15 @@ __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr 
     return;
   }
   y[o] = xo + (bv - res) * inv;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr pc, const OpCoef *__restrict__ coef,
+                                                     const float *__restrict__ x, const float *__restrict__ b,
+                                                     float *__restrict__ y) {
+  const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
+  if (r >= a.n) return;
+  row_apply<MODE>(a, pc, coef, x, b, y, r, threadIdx.x & 63);
 }
 
 // y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P (same
@@ -1704,6 +1710,11 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   std::vector<Slot> sl(B);
   int64_t next = 0;
   std::vector<int> it_f(nf, 0);
+  // slots take frequencies from the highest down: high frequencies need the
+  // most iterations, so the batch drains on the cheap ones at the end
+  std::vector<int64_t> order(nf);
+  for (int64_t k = 0; k < nf; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return freqs[a] > freqs[b]; });
 
   // assign the next frequency to slot k (stream idle)
   auto assign = [&](int k) {
@@ -1712,7 +1723,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       act[k] = 0;
       return false;
     }
-    const int64_t fi = next++;
+    const int64_t fi = order[next++];
     set_coefs(k, fi);
     sl[k] = Slot();
     sl[k].f = fi;
