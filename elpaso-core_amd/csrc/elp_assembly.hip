@@ -407,6 +407,29 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
   ELP_HIP(hipGetLastError());
 }
 
+// Hex8 shape values at a 2x2x2 Gauss point computed in registers, the same
+// arithmetic as the host table (lin1 factors 1 +- 0.577350269189626 chosen
+// by the signs of node and point, products ((a*b)*c)*0.125): bit-identical to
+// c_tab.h8 without table loads.  Node k's natural coordinate signs are bits
+// of 0x66 (x), 0xCC (y), 0xF0 (z); point g has x index g % 2, y (g / 2) % 2,
+// z g / 4, index 0 = +0.577350269189626.
+struct Hex8Shape {
+  double n, dx, dy, dz;
+};
+__device__ inline Hex8Shape hex8_shape(int k, int g) {
+  constexpr double gp = 0.577350269189626, hi = 1. + gp, lo = 1. - gp;
+  const bool sx = (0x66 >> k) & 1, sy = (0xCC >> k) & 1, sz = (0xF0 >> k) & 1;
+  const double a = (sx == !(g & 1)) ? hi : lo;
+  const double b = (sy == !((g >> 1) & 1)) ? hi : lo;
+  const double c = (sz == !((g >> 2) & 1)) ? hi : lo;
+  Hex8Shape r;
+  r.n = a * b * c * 0.125;
+  r.dx = (sx ? 1.0 : -1.0) * (b * c * 0.125);
+  r.dy = (sy ? 1.0 : -1.0) * (a * c * 0.125);
+  r.dz = (sz ? 1.0 : -1.0) * (a * b * 0.125);
+  return r;
+}
+
 // Fused assembly of trilinear (Hex8) elements directly into the CSR rows of
 // one node per wavefront, without element matrices in HBM:
 //  1. lane (k, g): geometry of adjacent element k at quadrature point g
@@ -420,7 +443,10 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
 constexpr int kMaxHex8Cols = 128;  // row length / neighbours of a Hex8 node (checked at finalize)
 template <int ND>
 struct Hex8Lds {
-  double geo[64][10];  // [k*8+g]: invJ (9), w detJ
+  // [(k*8+g)*10 + k + t]: invJ (t < 9), w detJ (t = 9); the pad of one
+  // double per element puts the 8 elements' rows read together in phase 2
+  // in distinct LDS banks
+  double geo[64 * 10 + 8];
   union {
     double xe[64][3];  // phase 0-1: coordinates of node q of element k, [k*8+q]
     double kr[ND][kMaxHex8Cols];
@@ -475,15 +501,15 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
     // 1. geometry of (element k, point g): inverse Jacobian, w detJ, and
     //    w detJ grad N_a of the node itself
     if (k < nadj) {
-      const double *dxi = c_tab.h8[g][1], *deta = c_tab.h8[g][2], *dzeta = c_tab.h8[g][3];
       double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const double *x = L.u.xe[k * 8 + q];
         const double x0 = x[0], x1 = x[1], x2 = x[2];
-        J[0] += dxi[q] * x0; J[1] += dxi[q] * x1; J[2] += dxi[q] * x2;
-        J[3] += deta[q] * x0; J[4] += deta[q] * x1; J[5] += deta[q] * x2;
-        J[6] += dzeta[q] * x0; J[7] += dzeta[q] * x1; J[8] += dzeta[q] * x2;
+        const Hex8Shape h = hex8_shape(q, g);
+        J[0] += h.dx * x0; J[1] += h.dx * x1; J[2] += h.dx * x2;
+        J[3] += h.dy * x0; J[4] += h.dy * x1; J[5] += h.dy * x2;
+        J[6] += h.dz * x0; J[7] += h.dz * x1; J[8] += h.dz * x2;
       }
       const double det = J[0] * J[4] * J[8] + J[1] * J[5] * J[6] + J[2] * J[3] * J[7] -
                          J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
@@ -499,7 +525,7 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
       G[2] = (J[1] * J[5] - J[2] * J[4]) * id;
       G[5] = -(J[0] * J[5] - J[2] * J[3]) * id;
       G[8] = (J[0] * J[4] - J[1] * J[3]) * id;
-      double *O = L.geo[lane];
+      double *O = L.geo + lane * 10 + k;
 #pragma unroll
       for (int t = 0; t < 9; t++) O[t] = G[t];
       O[9] = c_tab.w8[g] * det;
@@ -524,9 +550,10 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
       double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mm = 0.0, kk = 0.0;
 #pragma unroll
       for (int gg = 0; gg < 8; gg++) {
-        const double *G = L.geo[k * 8 + gg];
-        const double ax = c_tab.h8[gg][1][a], ay = c_tab.h8[gg][2][a], az = c_tab.h8[gg][3][a];
-        const double bx = c_tab.h8[gg][1][b], by = c_tab.h8[gg][2][b], bz = c_tab.h8[gg][3][b];
+        const double *G = L.geo + (k * 8 + gg) * 10 + k;
+        const Hex8Shape ha = hex8_shape(a, gg), hb = hex8_shape(b, gg);
+        const double ax = ha.dx, ay = ha.dy, az = ha.dz;
+        const double bx = hb.dx, by = hb.dy, bz = hb.dz;
         const double wd = G[9];
         const double p0 = G[0] * ax + G[1] * ay + G[2] * az;
         const double p1 = G[3] * ax + G[4] * ay + G[5] * az;
@@ -534,7 +561,7 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
         const double q0 = G[0] * bx + G[1] * by + G[2] * bz;
         const double q1 = G[3] * bx + G[4] * by + G[5] * bz;
         const double q2 = G[6] * bx + G[7] * by + G[8] * bz;
-        mm += c_tab.h8[gg][0][a] * c_tab.h8[gg][0][b] * wd;
+        mm += ha.n * hb.n * wd;
         const double w0 = wd * p0, w1 = wd * p1, w2 = wd * p2;
         if constexpr (ND == 1) {
           kk += w0 * q0 + w1 * q1 + w2 * q2;
