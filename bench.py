@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--assembly-nel", type=str, default="200,200,4")
+    ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving stream (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (frequency blocks would shard "
@@ -99,7 +100,10 @@ def main():
     converged = int((rels <= opts.rtol).sum())
     value = n_solved / elapsed
 
-    # plain per-frequency FP64 GPU solve on a sample of the same frequencies
+    # plain per-frequency FP64 GPU solve on a sample of the same frequencies,
+    # and the error of the mixed solutions against it (the mixed run of the
+    # sample on the same 32 slots gives the sweep's own bits: a frequency's
+    # result does not depend on the others in the batch)
     fp64 = None
     if args.fp64_sample > 0:
         idx = np.linspace(0, len(freqs) - 1, args.fp64_sample).round().astype(int)
@@ -107,13 +111,14 @@ def main():
         t = time.perf_counter()
         _, rel64, it64, _ = g.sweep(freqs[idx], opts=o64, want_x=False)
         dt64 = time.perf_counter() - t
-        # the same frequencies in the mixed path, one block
-        t = time.perf_counter()
-        _, relm, itm, _ = g.sweep(freqs[idx], opts=_lib.default_opts(block=_pick_block(len(idx))), want_x=False)
-        dtm = time.perf_counter() - t
+        X64, _, _, _ = g.sweep(freqs[idx], opts=o64, want_x=True)
+        Xm, relm, itm, _ = g.sweep(freqs[idx], opts=opts, want_x=True)
+        err = np.linalg.norm(Xm - X64, axis=1) / np.linalg.norm(X64, axis=1)
+        del Xm, X64
         fp64 = {"sample_freqs_hz": [round(float(f), 3) for f in freqs[idx]],
                 "fp64_solves_per_s": len(idx) / dt64, "fp64_relres_max": float(rel64.max()),
-                "fp64_iters": it64.tolist(), "mixed_same_sample_solves_per_s": len(idx) / dtm,
+                "fp64_iters": it64.tolist(), "mixed_iters": itm.tolist(),
+                "err_mixed_vs_fp64": [float(e) for e in err],
                 "speedup_mixed_batched_vs_fp64": value / (len(idx) / dt64)}
         log(f"[bench] fp64 sample: {fp64}")
 
@@ -121,7 +126,8 @@ def main():
     achieved = spmv_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"op_kernel<float,{B},0> (fused multi-frequency SpMV, level 0)",
+            "kernel": ("row_op_kernel<0> (fused 32-frequency FP32 operator, level 0)" if B == 32
+                       else f"op_kernel<float,{B},0> (fused multi-frequency FP32 operator, level 0)"),
             "bytes_per_launch": int(spmv_bytes), "avg_launch_ms": round(per_launch_ms, 4),
             "launches": int(spmv_calls)}
 
@@ -141,6 +147,11 @@ def main():
         log(f"[bench] assembly: {assembly}")
         sg.close()
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_stream(ctx, args.serve_requests)
+        log(f"[bench] serving: {serving}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(model, freqs)
@@ -158,16 +169,58 @@ def main():
         "converged": f"{converged}/{len(rels)}", "relres_max": float(rels.max()),
         "iters_mean": float(iters.mean()), "iters_max": int(iters.max()),
         "sweep_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "fp64_baseline": fp64, "roofline": roof, "assembly": assembly, "cpu_baseline": cpu,
+        "fp64_baseline": fp64, "roofline": roof, "assembly": assembly, "serving": serving,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
-def _pick_block(n):
-    for b in (1, 4, 8, 16, 32):
-        if b >= n:
-            return b
-    return 32
+def serve_stream(ctx, n_req, rate=20.0, seed=0):
+    """Seeded Poisson stream (mean rate 20/s) of sweep requests: acoustic box
+    cavities with element counts drawn log-uniform for about 5k-300k DOF (Hex8
+    and Hex20), 16-64 frequencies uniform in 10-1000 Hz, Z = rho c (a + bi)
+    with a, b uniform in [0.2, 2].  Models are built before the stream starts;
+    each request is submitted at its arrival time to a server with 4 slots and
+    a 64 GB HBM budget.  Latency = submit to result."""
+    from elpaso_amd import GpuModel, Server
+    from elpaso_amd import _lib
+    from elpaso_amd import meshgen as MG
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_req):
+        quad = bool(rng.integers(0, 2))
+        ndof = np.exp(rng.uniform(np.log(5e3), np.log(3e5)))
+        # elements per direction in the 10:8:6 aspect of the cavity
+        scale = (ndof / (480.0 * (8 if quad else 1))) ** (1.0 / 3.0)
+        nel = tuple(max(2, int(round(scale * r))) for r in (10, 8, 6))
+        a, b = rng.uniform(0.2, 2.0, size=2)
+        nf = int(rng.integers(16, 65))
+        freqs = np.sort(rng.uniform(10.0, 1000.0, size=nf))
+        reqs.append((nel, quad, 1.21 * 343.0 * (a + 1j * b), freqs))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    models = [GpuModel(ctx, MG.acoustic_cavity(*nel, quadratic=quad, Z=Z)) for nel, quad, Z, _ in reqs]
+    ndofs = [gm.n for gm in models]
+    sv = Server(ctx, 64 << 30, max_slots=4)
+    opts = _lib.default_opts(block=32)
+    t0 = time.perf_counter()
+    jobs = []
+    for k, (gm, (_, _, _, freqs)) in enumerate(zip(models, reqs)):
+        delay = arrivals[k] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        jobs.append(sv.submit(gm, freqs, opts=opts))
+    lat, ok, solves = [], 0, 0
+    for j, (_, _, _, freqs) in zip(jobs, reqs):
+        _, rel, it, l = sv.wait(j, want_x=False)
+        lat.append(l)
+        ok += int((rel <= 1e-8).all())
+        solves += len(freqs)
+    makespan = time.perf_counter() - t0
+    sv.close()
+    return {"requests": n_req, "arrival_rate_per_s": rate, "dof_range": [int(min(ndofs)), int(max(ndofs))],
+            "all_converged": f"{ok}/{n_req}", "latency_s": {"p50": float(np.percentile(lat, 50)),
+                                                           "p99": float(np.percentile(lat, 99))},
+            "solves_per_s": solves / makespan, "makespan_s": makespan, "slots": 4}
 
 
 def cpu_baseline(model, freqs):

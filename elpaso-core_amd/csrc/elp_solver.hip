@@ -796,7 +796,7 @@ void default_opts(elp_solve_opts &o) {
   o.block = 32;
   o.rtol = 1e-8;
   o.max_iters = 4000;
-  o.inner_rtol = 1e-4;
+  o.inner_rtol = 3e-3;
   o.restart = 50;
   o.shift = -1.0;  // automatic: 0 with the exact box coarse solve, 0.5 otherwise
   o.smooth = 1;
@@ -1615,7 +1615,8 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
 // Continuous batching of a sweep on B slots (mixed precision).  Each slot
 // holds one frequency; all slots advance together through one fused COCG
 // iteration (one multi-frequency SpMV and one preconditioner application
-// for the whole block).  When a slot's inner solve reaches inner_rtol it is
+// for the whole block).  When a slot's inner solve reaches its tolerance
+// (inner_rtol, or just past rtol when one FP32 solve can reach it) it is
 // frozen; once a quarter of the slots are frozen (or none is iterating) the
 // frozen slots take an FP64 refinement step together: x64 += w d, r64 = b -
 // A x64 in FP64.  A slot whose FP64 residual reaches rtol (or stops
@@ -1706,6 +1707,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     bool broken = false;
     double prev = 1;  // FP64 relative residual before the current inner solve
     double w = 0;     // ||r64|| the inner right-hand side was scaled by
+    double tol = 0;   // relative tolerance of the current inner solve
   };
   std::vector<Slot> sl(B);
   int64_t next = 0;
@@ -1716,6 +1718,12 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   for (int64_t k = 0; k < nf; k++) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return freqs[a] > freqs[b]; });
 
+  // inner tolerance for a slot at FP64 relative residual rel: inner_rtol, or
+  // just past the target when that is within reach of one FP32 solve
+  auto inner_tol = [&](double rel) {
+    const double finish = 0.5 * o.rtol / rel;
+    return finish >= 1e-4 ? std::min(o.inner_rtol, finish) : o.inner_rtol;
+  };
   // assign the next frequency to slot k (stream idle)
   auto assign = [&](int k) {
     if (next >= nf) {
@@ -1730,6 +1738,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     sl[k].state = 1;
     sl[k].prev = 1.0;
     sl[k].w = nbn;
+    sl[k].tol = inner_tol(1.0);
     fresh[k] = 1;
     act[k] = 2;
     scale[k] = 1.0 / nbn;
@@ -1767,7 +1776,6 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   start_columns();
 
   const int freeze_batch = std::max(1, B / 4);
-  const double inner_tol = o.inner_rtol;
   while (true) {
     int n_iter = 0;
     for (int k = 0; k < B; k++) n_iter += sl[k].state == 1;
@@ -1809,7 +1817,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
         it_f[q.f]++;
         const double nr = std::sqrt(S.h_red[k].x);
         const bool bad = !std::isfinite(nr) || nr > 1e6;
-        if (bad || !(nr > inner_tol) || it_f[q.f] >= o.max_iters) {
+        if (bad || !(nr > q.tol) || it_f[q.f] >= o.max_iters) {
           q.state = 2;
           q.broken = bad;
           act[k] = 0;
@@ -1862,6 +1870,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
         any_new |= assign(k);
       } else {
         q.prev = rel[k];
+        q.tol = inner_tol(rel[k]);
         q.w = rel[k] * nbn;
         q.state = 1;
         act[k] = 2;
