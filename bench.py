@@ -29,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "elpaso-core_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# on the default workloads, scaled by the factors a 1 GiB copy gives (x2, x1);
+# the commands and counter values are in the profile below.
+PMC_PROFILE = "profiles/pmc_traffic_row_op_kernel_and_hex8_node_kernel.md"
+PMC_TRAFFIC = {("row_op_kernel<0>", "100,80,60"): 4.275e9, ("hex8_node_kernel<3>", "200,200,4"): 1.461e9}
 
 
 def log(*a):
@@ -48,6 +53,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--assembly-nel", type=str, default="200,200,4")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 device-to-device copies of 1 GiB (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (frequency blocks would shard "
@@ -124,8 +131,10 @@ def main():
 
     per_launch_ms = spmv_ms / max(1, spmv_calls)
     achieved = spmv_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    traffic = PMC_TRAFFIC.get(("row_op_kernel<0>", args.nel)) if B == 32 else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": PMC_PROFILE if traffic else None,
             "kernel": ("row_op_kernel<0> (fused 32-frequency FP32 operator, level 0)" if B == 32
                        else f"op_kernel<float,{B},0> (fused multi-frequency FP32 operator, level 0)"),
             "bytes_per_launch": int(spmv_bytes), "avg_launch_ms": round(per_launch_ms, 4),
@@ -143,9 +152,14 @@ def main():
         assembly = {"workload": f"steel Brick8 {ax}x{ay}x{az} jittered, K+M+C FP64 CSR",
                     "n": sg.n, "nnz": sg.nnz, "ms": round(ms, 4), "algorithmic_bytes": alg,
                     "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
-                    "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                    "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "kernel": "hex8_node_kernel<3>",
+                    "traffic": PMC_TRAFFIC.get(("hex8_node_kernel<3>", args.assembly_nel))}
         log(f"[bench] assembly: {assembly}")
         sg.close()
+
+    if args.copy_calibration:
+        copy_calibration()
 
     serving = None
     if args.serve_requests > 0:
@@ -173,6 +187,25 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration(nbytes=1 << 30, repeats=3):
+    """Plain device-to-device copies of known size: hardware byte counters
+    (FETCH_SIZE, WRITE_SIZE) read on them give the factor between counter
+    and true bytes on this GPU."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(repeats):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: {repeats} x {nbytes} B device-to-device")
 
 
 def serve_stream(ctx, n_req, rate=20.0, seed=0):
