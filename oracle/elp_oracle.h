@@ -2,9 +2,9 @@
  * elp_oracle.h -- CPU restatement of elPaSo-Core's FE assembly and harmonic
  * system set-up, in plain C (FP64, complex where elPaSo is complex).
  *
- * TEST INFRASTRUCTURE ONLY. This code is the checker: only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
- * product library never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY. This is the oracle the GPU results are compared
+ * with: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  The product library never links, loads or calls it.
  *
  * Every function cites the elPaSo source it restates (paths relative to
  * elpasoCore/source).  Loop orders and the order of floating-point operations

@@ -1,7 +1,7 @@
 """Extract the golden vectors of elPaSo-Core's own unit tests into JSON fixtures.
 
 Run once in a container that holds the elPaSo-Core sources at
-/root/reference (the GPU box does not need it):
+/root/reference (GPU runs read only the committed JSON):
 
     python tests/golden/make_golden.py
 
@@ -27,13 +27,6 @@ def _array(src, name, after=0):
     j = src.index("{", i)
     k = src.index("}", j)
     return _floats(src[j + 1:k]), k
-
-
-def _vec(src, func, name):
-    i = src.index(func)
-    j = src.index(name, i)
-    vals, _ = _array(src, "{", j - 1) if False else _array(src, name, i)
-    return vals
 
 
 def main():
