@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
 
 #include "elp_fdm.h"
 #include "elp_internal.h"
@@ -242,27 +243,6 @@ struct PadCsr {
   const float *K, *M;
 };
 
-// CH entries of a padded row: all loads of x are issued before the FMAs
-template <int CH>
-__device__ inline void row_chunk(const PadCsr &pc, int32_t base, __amdgpu_buffer_rsrc_t xr, uint32_t vo, float &aK,
-                                 float &aM) {
-  uint32_t so[CH];
-  float kv[CH], mv[CH], xv[CH];
-#pragma unroll
-  for (int u = 0; u < CH; u++) {
-    so[u] = pc.off[base + u];
-    kv[u] = pc.K[base + u];
-    mv[u] = pc.M[base + u];
-  }
-#pragma unroll
-  for (int u = 0; u < CH; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, so[u], 0));
-#pragma unroll
-  for (int u = 0; u < CH; u++) {
-    aK = fmaf(kv[u], xv[u], aK);
-    aM = fmaf(mv[u], xv[u], aM);
-  }
-}
-
 // First l1-Jacobi sweep from a zero guess, y = b / D, one thread per
 // (row, frequency): the same values as op_kernel's MODE 3.
 template <typename T>
@@ -290,14 +270,39 @@ __global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const 
   y[i] = t;
 }
 
+// A padded row in batches of NB entries: lane u loads entry u's x offset,
+// K and M with three coalesced vector loads; every entry's offset is
+// broadcast with v_readlane and all NB loads of x are issued before the
+// first FMA.  Entries past the row read an offset past the end of x (0).
+template <int NB>
+__device__ inline void row_batches(const PadCsr &pc, int32_t base, int32_t p1, uint32_t oob,
+                                   __amdgpu_buffer_rsrc_t xr, uint32_t vo, int lane, float &aK, float &aM) {
+  for (; base < p1; base += NB) {
+    const bool in = lane < NB && base + lane < p1;
+    const uint32_t myoff = in ? pc.off[base + lane] : oob;
+    const float myk = in ? pc.K[base + lane] : 0.f, mym = in ? pc.M[base + lane] : 0.f;
+    float xv[NB];
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      const uint32_t so = __builtin_amdgcn_readlane(myoff, u);
+      xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, so, 0));
+    }
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      aK = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, myk), u)), xv[u], aK);
+      aM = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mym), u)), xv[u], aM);
+    }
+  }
+}
+
 // Row-per-wavefront operator for B = 32 (FP32): lane l holds float l of the
 // row's 32 complex frequencies (frequency l / 2, real part for even l,
 // imaginary part for odd l).  K and M are real, so both parts accumulate
 // independently: the row's x offsets and K, M values are the same for every
-// lane and are read with scalar loads into scalar registers, and each nonzero
-// costs one coalesced 256-byte buffer load of x and two FMAs, with no
-// per-entry address arithmetic.  The complex coefficients are applied once per
-// row with the partner lane's part.  Entries are summed in row order.
+// lane (broadcast from the lane that loaded them), and each nonzero costs one
+// coalesced 256-byte buffer load of x and two FMAs, with no per-entry address
+// arithmetic.  The complex coefficients are applied once per row with the
+// partner lane's part.  Entries are summed in row order.
 constexpr int kRowWg = 256;  // threads per workgroup: kRowWg / 64 consecutive rows
 
 template <int MODE>
@@ -321,10 +326,8 @@ __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const
     const int xbytes = (int)(a.n * W * (int64_t)sizeof(float));
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, xbytes, 0x00020000);
     const uint32_t vo = (uint32_t)lane * sizeof(float);
-    int32_t base = pc.rp[r];
-    const int32_t p1 = pc.rp[r + 1];
-    for (; base + 2 * kRowPad <= p1; base += 2 * kRowPad) row_chunk<2 * kRowPad>(pc, base, xr, vo, aK, aM);
-    if (base < p1) row_chunk<kRowPad>(pc, base, xr, vo, aK, aM);
+    const int32_t base = pc.rp[r], p1 = pc.rp[r + 1];
+    row_batches<32>(pc, base, p1, (uint32_t)xbytes, xr, vo, lane, aK, aM);
     const float *xl = x + lane;
     for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
       const float xv = xl[(uint32_t)a.xcol[k] * (uint32_t)W];
@@ -369,6 +372,92 @@ __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr 
   const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
   if (r >= a.n) return;
   row_apply<MODE>(a, pc, coef, x, b, y, r, threadIdx.x & 63);
+}
+
+// Post-smoothing of the two-level box cycle fused with the prolongation,
+// for 32 frequencies.  With u = D^-1 r (first sweep), v = r - A u and e the
+// box-lattice correction, the cycle's last step is
+//   z = x + D^-1 (r - A x),  x = u + P e,
+// and A x = (r - v) + (A P) e, so
+//   z = u + P e + D^-1 (v - (cK KP + cM MP + cG GP) e)
+// with the frequency-independent products KP, MP, GP (C_Gamma P) formed once.
+// A Hex20 row of A P has at most 27 lattice columns (57 for A), and e is the
+// lattice vector, a quarter of the fine one.  Row layout as row_op_kernel:
+// one wavefront per row, lane = frequency x real/imaginary part, rows padded
+// to multiples of kRowPad with offsets past the end of e; P rows padded to 8.
+struct PadCsr3 {
+  const int32_t *rp;
+  const uint32_t *off;
+  const float *K, *M, *G;
+};
+
+template <int CH>
+__device__ inline void ap_chunk(const PadCsr3 &pc, int32_t base, __amdgpu_buffer_rsrc_t er, uint32_t vo, float &aK,
+                                float &aM, float &aG) {
+  uint32_t so[CH];
+  float xv[CH];
+#pragma unroll
+  for (int u = 0; u < CH; u++) so[u] = pc.off[base + u];
+#pragma unroll
+  for (int u = 0; u < CH; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(er, vo, so[u], 0));
+#pragma unroll
+  for (int u = 0; u < CH; u++) {
+    aK = fmaf(pc.K[base + u], xv[u], aK);
+    aM = fmaf(pc.M[base + u], xv[u], aM);
+    aG = fmaf(pc.G[base + u], xv[u], aG);
+  }
+}
+
+__global__ __launch_bounds__(kRowWg) void box_post_kernel(OpArgs<float> a, PadCsr3 ap, const uint32_t *__restrict__ poff,
+                                                       const float *__restrict__ pval, const OpCoef *__restrict__ coef,
+                                                       const float *__restrict__ e, int64_t ne,
+                                                       const float *__restrict__ u, const float *__restrict__ v,
+                                                       float *__restrict__ z) {
+  constexpr int W = 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
+  if (r >= a.n) return;
+  const int64_t o = r * W + lane;
+  const float uu = u[o], vv = v[o];
+  const int ebytes = (int)(ne * W * (int64_t)sizeof(float));
+  const __amdgpu_buffer_rsrc_t er = __builtin_amdgcn_make_buffer_rsrc((void *)e, (short)0, ebytes, 0x00020000);
+  const uint32_t vo = (uint32_t)lane * sizeof(float);
+  // P e
+  float pe = 0.f;
+  {
+    uint32_t so[8];
+    float pv[8], xv[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      so[t] = poff[r * 8 + t];
+      pv[t] = pval[r * 8 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) xv[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(er, vo, so[t], 0));
+#pragma unroll
+    for (int t = 0; t < 8; t++) pe = fmaf(pv[t], xv[t], pe);
+  }
+  // (A P) e
+  float aK = 0.f, aM = 0.f, aG = 0.f;
+  int32_t base = ap.rp[r];
+  const int32_t p1 = ap.rp[r + 1];
+  for (; base + 2 * kRowPad <= p1; base += 2 * kRowPad) ap_chunk<2 * kRowPad>(ap, base, er, vo, aK, aM, aG);
+  if (base < p1) ap_chunk<kRowPad>(ap, base, er, vo, aK, aM, aG);
+  const float pK = __shfl_xor(aK, 1), pM = __shfl_xor(aM, 1), pG = __shfl_xor(aG, 1);
+  const bool im = lane & 1;
+  const float2 K = im ? float2{pK, aK} : float2{aK, pK};
+  const float2 M = im ? float2{pM, aM} : float2{aM, pM};
+  const float2 G = im ? float2{pG, aG} : float2{aG, pG};
+  const OpCoef &cf = coef[lane >> 1];
+  const bool fl = a.field[r] == 1;
+  const float2 ck = fl ? float2{(float)cf.kf.x, (float)cf.kf.y} : float2{(float)cf.ks.x, (float)cf.ks.y};
+  const float2 cm = fl ? float2{(float)cf.mf.x, (float)cf.mf.y} : float2{(float)cf.ms.x, (float)cf.ms.y};
+  const float2 cg = {(float)cf.g.x, (float)cf.g.y};
+  const float2 t1 = cmul(ck, K), t2 = cmul(cm, M), t3 = cmul(cg, G);
+  const float ax = im ? (t1.y + t2.y + t3.y) : (t1.x + t2.x + t3.x);
+  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+  const float inv = (float)(1.0 / d);
+  z[o] = uu + pe + (vv - ax) * inv;
 }
 
 // y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P (same
@@ -744,6 +833,12 @@ struct LevelData {
   DArr<uint32_t> pad_off;
   DArr<float> pad_K, pad_M;
   PadCsr padded() const { return {pad_rp.p, pad_off.p, pad_K.p, pad_M.p}; }
+  // box cycle: K P, M P, C_Gamma P (row-padded, permuted rows, lattice
+  // columns as byte offsets) and P rows padded to 8 (box_post_kernel)
+  DArr<int32_t> ap_rp;
+  DArr<uint32_t> ap_off, pp_off;
+  DArr<float> ap_K, ap_M, ap_G, pp_v;
+  PadCsr3 ap() const { return {ap_rp.p, ap_off.p, ap_K.p, ap_M.p, ap_G.p}; }
   template <typename T> OpArgs<T> args() const;
   struct Csr {
     const int32_t *rp, *ci;
@@ -938,6 +1033,116 @@ __global__ void gather_perm_kernel(int64_t n, const int32_t *__restrict__ perm, 
                                    T *__restrict__ out) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n) out[r] = in[perm[r]];
+}
+
+// K P, M P and C_Gamma P for box_post_kernel, in the FP32 operator's row
+// order, from the FP32 values the operator itself applies (so that both
+// smoothing steps of the cycle see the same operator): the products of FP32
+// entries are summed in FP64 over the row's entries in row order and rounded
+// once.  Built on the host in parallel over row ranges; skipped (the cycle
+// then uses the plain operator) when a model has FSI entries or exceeds the
+// 32-bit offsets.
+static void build_box_products(LevelData &L, hipStream_t s) {
+  Model &m = *L.model;
+  const int64_t n = m.n, nc = L.nc;
+  if (n > kRowOpMaxRows || nc <= 0 || nc > kRowOpMaxRows || !L.permuted) return;
+  for (auto t : m.x_type)
+    if (t != 0) return;
+  constexpr uint32_t line = 32 * sizeof(float2);
+  ELP_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> rp, ci, xrp, xci, prp, pci;
+  std::vector<float> K, M, X, pv;
+  L.rp32.download(rp);
+  L.ci32.download(ci);
+  L.K32.download(K);
+  L.M32.download(M);
+  L.xrp32.download(xrp);
+  L.xcol32.download(xci);
+  L.X32.download(X);
+  L.P_rp32.download(prp);
+  L.P_ci32.download(pci);
+  L.P32.download(pv);
+  for (int64_t r = 0; r < n; r++)
+    if (prp[r + 1] - prp[r] > 8) return;
+  struct Part {
+    std::vector<int32_t> len;
+    std::vector<uint32_t> off;
+    std::vector<float> K, M, G;
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int64_t chunk = (n + nt - 1) / nt;
+  std::vector<Part> part(nt);
+  auto work = [&](unsigned t) {
+    const int64_t a = (int64_t)t * chunk, b = std::min<int64_t>(n, a + chunk);
+    if (a >= b) return;
+    std::vector<int32_t> stamp(nc, -1), list;
+    std::vector<double> aK(nc), aM(nc), aG(nc);
+    Part &P = part[t];
+    for (int64_t i = a; i < b; i++) {
+      list.clear();
+      auto add = [&](int32_t j, double kv, double mv, double gv) {
+        for (int32_t q = prp[j]; q < prp[j + 1]; q++) {
+          const int32_t c = pci[q];
+          const double w = pv[q];
+          if (stamp[c] != (int32_t)i) {
+            stamp[c] = (int32_t)i;
+            aK[c] = aM[c] = aG[c] = 0.0;
+            list.push_back(c);
+          }
+          aK[c] += kv * w;
+          aM[c] += mv * w;
+          aG[c] += gv * w;
+        }
+      };
+      for (int32_t p = rp[i]; p < rp[i + 1]; p++) add(ci[p], K[p], M[p], 0.0);
+      for (int32_t p = xrp[i]; p < xrp[i + 1]; p++) add(xci[p], 0.0, 0.0, X[p]);
+      std::sort(list.begin(), list.end());
+      const int len = (int)list.size(), padded = (len + kRowPad - 1) / kRowPad * kRowPad;
+      P.len.push_back(padded);
+      for (int32_t c : list) {
+        P.off.push_back((uint32_t)c * line);
+        P.K.push_back((float)aK[c]);
+        P.M.push_back((float)aM[c]);
+        P.G.push_back((float)aG[c]);
+      }
+      for (int k = len; k < padded; k++) {
+        P.off.push_back((uint32_t)(nc * line));
+        P.K.push_back(0.f);
+        P.M.push_back(0.f);
+        P.G.push_back(0.f);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++) th.emplace_back(work, t);
+  for (auto &x : th) x.join();
+  std::vector<int32_t> arp(1, 0);
+  std::vector<uint32_t> off;
+  std::vector<float> vK, vM, vG;
+  for (auto &P : part) {
+    for (int32_t l : P.len) arp.push_back(arp.back() + l);
+    off.insert(off.end(), P.off.begin(), P.off.end());
+    vK.insert(vK.end(), P.K.begin(), P.K.end());
+    vM.insert(vM.end(), P.M.begin(), P.M.end());
+    vG.insert(vG.end(), P.G.begin(), P.G.end());
+    P = Part();
+  }
+  // prolongation rows, 8 entries each
+  std::vector<uint32_t> poff((size_t)n * 8, (uint32_t)(nc * line));
+  std::vector<float> pval((size_t)n * 8, 0.f);
+  for (int64_t i = 0; i < n; i++)
+    for (int32_t q = prp[i]; q < prp[i + 1]; q++) {
+      poff[i * 8 + (q - prp[i])] = (uint32_t)pci[q] * line;
+      pval[i * 8 + (q - prp[i])] = pv[q];
+    }
+  Ctx *c = m.ctx;
+  L.ap_rp.upload(c, arp);
+  L.ap_off.upload(c, off);
+  L.ap_K.upload(c, vK);
+  L.ap_M.upload(c, vM);
+  L.ap_G.upload(c, vG);
+  L.pp_off.upload(c, poff);
+  L.pp_v.upload(c, pval);
 }
 
 static void permute_level_float(LevelData &L, hipStream_t s) {
@@ -1154,6 +1359,7 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
     lattice.box = m.box;
     build_interp(*st->levels[0], lattice, m.box.nel);
     permute_level_float(*st->levels[0], s);
+    if (m.box.order == 2) build_box_products(*st->levels[0], s);
   } else if (!want_mg) {
     permute_level_float(*st->levels[0], s);
   } else if (want_mg) {
@@ -1360,6 +1566,22 @@ struct Solver {
       return;
     }
     const int nu = std::max(1, o.smooth);
+    if constexpr (std::is_same<T, float>::value && B == 32) {
+      if (nu == 1 && L.ap_rp.p) {
+        // u = D^-1 r, v = r - A u, e = box solve of R v, then the fused
+        // z = u + P e + D^-1 (v - (A P) e)
+        smooth(0, b, nullptr, 1, W.x.p);
+        op<1>(0, coefP.p, W.x.p, b, W.r.p);
+        transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, fb.p);
+        fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
+        hipLaunchKernelGGL(box_post_kernel, dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
+                           L.template args<float>(), L.ap(), L.pp_off.p, L.pp_v.p, coefP.p,
+                           reinterpret_cast<const float *>(fe.p), nf, reinterpret_cast<const float *>(W.x.p),
+                           reinterpret_cast<const float *>(W.r.p), reinterpret_cast<float *>(z));
+        ELP_HIP(hipGetLastError());
+        return;
+      }
+    }
     smooth(0, b, nullptr, nu, W.x.p);
     op<1>(0, coefP.p, W.x.p, b, W.r.p);
     transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, fb.p);
