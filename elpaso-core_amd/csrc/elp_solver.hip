@@ -391,20 +391,28 @@ struct PadCsr3 {
   const float *K, *M, *G;
 };
 
-template <int CH>
-__device__ inline void ap_chunk(const PadCsr3 &pc, int32_t base, __amdgpu_buffer_rsrc_t er, uint32_t vo, float &aK,
-                                float &aM, float &aG) {
-  uint32_t so[CH];
-  float xv[CH];
+// (A P) e over a padded row in batches of NB entries (row_batches' scheme:
+// per-entry values loaded by the lanes, broadcast with v_readlane)
+template <int NB>
+__device__ inline void ap_batches(const PadCsr3 &pc, int32_t base, int32_t p1, uint32_t oob,
+                                  __amdgpu_buffer_rsrc_t er, uint32_t vo, int lane, float &aK, float &aM, float &aG) {
+  for (; base < p1; base += NB) {
+    const bool in = lane < NB && base + lane < p1;
+    const uint32_t myoff = in ? pc.off[base + lane] : oob;
+    const float myk = in ? pc.K[base + lane] : 0.f, mym = in ? pc.M[base + lane] : 0.f;
+    const float myg = in ? pc.G[base + lane] : 0.f;
+    float xv[NB];
 #pragma unroll
-  for (int u = 0; u < CH; u++) so[u] = pc.off[base + u];
+    for (int u = 0; u < NB; u++) {
+      const uint32_t so = __builtin_amdgcn_readlane(myoff, u);
+      xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(er, vo, so, 0));
+    }
 #pragma unroll
-  for (int u = 0; u < CH; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(er, vo, so[u], 0));
-#pragma unroll
-  for (int u = 0; u < CH; u++) {
-    aK = fmaf(pc.K[base + u], xv[u], aK);
-    aM = fmaf(pc.M[base + u], xv[u], aM);
-    aG = fmaf(pc.G[base + u], xv[u], aG);
+    for (int u = 0; u < NB; u++) {
+      aK = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, myk), u)), xv[u], aK);
+      aM = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mym), u)), xv[u], aM);
+      aG = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, myg), u)), xv[u], aG);
+    }
   }
 }
 
@@ -439,10 +447,7 @@ __global__ __launch_bounds__(kRowWg) void box_post_kernel(OpArgs<float> a, PadCs
   }
   // (A P) e
   float aK = 0.f, aM = 0.f, aG = 0.f;
-  int32_t base = ap.rp[r];
-  const int32_t p1 = ap.rp[r + 1];
-  for (; base + 2 * kRowPad <= p1; base += 2 * kRowPad) ap_chunk<2 * kRowPad>(ap, base, er, vo, aK, aM, aG);
-  if (base < p1) ap_chunk<kRowPad>(ap, base, er, vo, aK, aM, aG);
+  ap_batches<32>(ap, ap.rp[r], ap.rp[r + 1], (uint32_t)ebytes, er, vo, lane, aK, aM, aG);
   const float pK = __shfl_xor(aK, 1), pM = __shfl_xor(aM, 1), pG = __shfl_xor(aG, 1);
   const bool im = lane & 1;
   const float2 K = im ? float2{pK, aK} : float2{aK, pK};
