@@ -1844,7 +1844,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
 // iteration (one multi-frequency SpMV and one preconditioner application
 // for the whole block).  When a slot's inner solve reaches its tolerance
 // (inner_rtol, or just past rtol when one FP32 solve can reach it) it is
-// frozen; once a quarter of the slots are frozen (or none is iterating) the
+// frozen; once a third of the slots are frozen (or none is iterating) the
 // frozen slots take an FP64 refinement step together: x64 += w d, r64 = b -
 // A x64 in FP64.  A slot whose FP64 residual reaches rtol (or stops
 // improving, or runs out of iterations) is written out and takes the next
@@ -2002,7 +2002,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   upload_coefs();
   start_columns();
 
-  const int freeze_batch = std::max(1, B / 4);
+  const int freeze_batch = std::max(1, B / 3);
   while (true) {
     int n_iter = 0;
     for (int k = 0; k < B; k++) n_iter += sl[k].state == 1;
