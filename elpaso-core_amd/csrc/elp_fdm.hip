@@ -165,6 +165,84 @@ __global__ __launch_bounds__(256) void axis_transform_kernel(int64_t outer, int 
   }
 }
 
+// FP32 transforms on the matrix cores: out[o][p][c] = sum_l Mat[p][l] in[o][l][c]
+// as GEMMs Mat (L x L) x tile (L x 64) with v_mfma_f32_16x16x4f32.  A
+// workgroup stages Mat once in LDS (row stride KP + 1 words: conflict-free
+// reads of the A fragments) and walks the 64-column tiles t = blockIdx.x,
+// blockIdx.x + gridDim.x, ...: each tile (columns as in axis_transform_kernel)
+// is staged in LDS with a row stride of 80 words (conflict-free B fragments),
+// wave w computes output columns 16w .. 16w+15 for all RT row tiles.
+// Fragment layouts (checked on the device): A element (i, k) from lane
+// i + 16 k, B element (k, j) from lane j + 16 k, result (4 (l / 16) + r, l % 16)
+// in element r of lane l.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaTileStride = 80;
+// L (L + 5) + 80 (L + 3) words of LDS must fit the 160 KB of a CU
+constexpr int kMfmaMaxAxis = 160;
+template <int RT>
+__global__ __launch_bounds__(256) void axis_mfma_kernel(int64_t outer, int L, int64_t inner, int64_t ntiles,
+                                                        const float *__restrict__ Mat, const float *__restrict__ in,
+                                                        float *__restrict__ out) {
+  constexpr int CW = 64;
+  constexpr int NPRE = ((kMfmaMaxAxis + 3) / 4 * 4 * (CW / 4) + 255) / 256;  // float4 per thread per tile
+  extern __shared__ float sm[];
+  const int KP = (L + 3) / 4 * 4, KS = KP + 1;
+  float *ma = sm;                   // [L][KS]: ma[p * KS + l] = Mat[p][l], zero for l >= L
+  float *tb = sm + (size_t)L * KS;  // [KP][kMfmaTileStride]
+  for (int idx = threadIdx.x; idx < L * KP; idx += 256) {
+    const int p = idx / KP, l = idx % KP;
+    ma[p * KS + l] = l < L ? Mat[(int64_t)p * L + l] : 0.f;
+  }
+  // inner is a multiple of CW here: tile t = columns (t % tpl) CW .. + CW of
+  // line t / tpl, loaded as float4 rows
+  const int64_t tpl = inner / CW;
+  const int nv = KP * (CW / 4);
+  auto load = [&](int64_t t, float4 *v) {
+    const float *src = in + (t / tpl) * L * inner + (t % tpl) * CW;
+#pragma unroll
+    for (int u = 0; u < NPRE; u++) {
+      const int idx = threadIdx.x + u * 256;
+      const int l = idx / (CW / 4), q = idx % (CW / 4);
+      v[u] = (idx < nv && l < L) ? *reinterpret_cast<const float4 *>(src + (int64_t)l * inner + 4 * q)
+                                 : float4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  float4 pre[NPRE];
+  if ((int64_t)blockIdx.x < ntiles) load(blockIdx.x, pre);
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's reads of tb are done (and ma is staged)
+#pragma unroll
+    for (int u = 0; u < NPRE; u++) {
+      const int idx = threadIdx.x + u * 256;
+      if (idx < nv) *reinterpret_cast<float4 *>(tb + (idx / (CW / 4)) * kMfmaTileStride + 4 * (idx % (CW / 4))) = pre[u];
+    }
+    __syncthreads();
+    if (t + gridDim.x < ntiles) load(t + gridDim.x, pre);  // in flight during the products
+    mfma_f4 acc[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) acc[rt] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < KP; k0 += 4) {
+      const float bv = tb[(k0 + lk) * kMfmaTileStride + 16 * w + li];
+#pragma unroll
+      for (int rt = 0; rt < RT; rt++) {
+        const int row = 16 * rt + li;
+        const float av = row < L ? ma[row * KS + k0 + lk] : 0.f;
+        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[rt], 0, 0, 0);
+      }
+    }
+    float *dst = out + (t / tpl) * L * inner + (t % tpl) * CW + 16 * w + li;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int p = 16 * rt + 4 * lk + r;
+        if (p < L) dst[(int64_t)p * inner] = acc[rt][r];
+      }
+  }
+}
+
 template <typename V>
 __device__ inline V cdiv(V a, V b) {
   const auto d = b.x * b.x + b.y * b.y;
@@ -257,9 +335,36 @@ static void launch_axis(int64_t outer, int L, int64_t inner, const T *Mat, const
   ELP_HIP(hipGetLastError());
 }
 
+template <int RT>
+static void launch_axis_mfma(int64_t outer, int L, int64_t inner, const float *Mat, const float *in, float *out,
+                             hipStream_t s) {
+  constexpr int CW = 64;
+  const int KP = (L + 3) / 4 * 4;
+  const size_t lds = ((size_t)L * (KP + 1) + (size_t)KP * kMfmaTileStride) * sizeof(float);
+  const int64_t ntiles = outer * (inner / CW);
+  const auto kern = axis_mfma_kernel<RT>;
+  ELP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 1024);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, outer, L, inner, ntiles, Mat, in, out);
+  ELP_HIP(hipGetLastError());
+}
+
 // transform along an axis of L <= kFdmMaxAxis points; inner counts reals
 template <typename T, typename V>
 static void transform(int64_t outer, int L, int64_t inner, const T *Mat, const V *in, V *out, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L <= kMfmaMaxAxis && inner % 64 == 0) {  // whole 64-column tiles (B = 32 frequencies)
+      const float *i = reinterpret_cast<const float *>(in);
+      float *o = reinterpret_cast<float *>(out);
+      const int rt = (L + 15) / 16;
+      if (rt <= 2) launch_axis_mfma<2>(outer, L, inner, Mat, i, o, s);
+      else if (rt <= 4) launch_axis_mfma<4>(outer, L, inner, Mat, i, o, s);
+      else if (rt <= 6) launch_axis_mfma<6>(outer, L, inner, Mat, i, o, s);
+      else if (rt <= 8) launch_axis_mfma<8>(outer, L, inner, Mat, i, o, s);
+      else launch_axis_mfma<12>(outer, L, inner, Mat, i, o, s);
+      return;
+    }
+  }
   constexpr int CW = sizeof(T) == 4 ? 64 : 32;
   ELP_REQUIRE(L <= kFdmMaxAxis && inner > 0, ELP_EINVAL, "box transform shape");
   constexpr int PG = 256 / (CW / 4);
