@@ -246,20 +246,11 @@ struct PadCsr {
 // First l1-Jacobi sweep from a zero guess, y = b / D, one thread per
 // (row, frequency): the same values as op_kernel's MODE 3.
 template <typename T>
-__global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const OpCoef *__restrict__ coef,
-                                                      const typename VT<T>::t *__restrict__ b,
-                                                      typename VT<T>::t *__restrict__ y) {
+__device__ inline typename VT<T>::t jacobi0_value(const OpArgs<T> &a, const OpCoef *__restrict__ coef, int64_t r,
+                                                  int f, typename VT<T>::t bv) {
   using V = typename VT<T>::t;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n * B) return;
-  const int64_t r = i / B;
-  const int f = (int)(i % B);
   const uint8_t fld = a.field[r];
-  const V bv = b[i];
-  if (fld == 2) {
-    y[i] = bv;
-    return;
-  }
+  if (fld == 2) return bv;
   const OpCoef &cf = coef[f];
   const bool fl = fld == 1;
   const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
@@ -267,7 +258,16 @@ __global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const 
   V t;
   t.x = bv.x * inv;
   t.y = bv.y * inv;
-  y[i] = t;
+  return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const OpCoef *__restrict__ coef,
+                                                      const typename VT<T>::t *__restrict__ b,
+                                                      typename VT<T>::t *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n * B) return;
+  y[i] = jacobi0_value<T>(a, coef, i / B, (int)(i % B), b[i]);
 }
 
 // A padded row in batches of NB entries: lane u loads entry u's x offset,
@@ -581,12 +581,19 @@ __global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__r
   red_block_store(acc, B, partial);
 }
 
-// x += alpha p ; r -= alpha q ; partial sums of |r|^2 (dot_kernel's layout)
-template <typename V>
+// x += alpha p ; r -= alpha q ; partial sums of |r|^2 (dot_kernel's layout);
+// with u != nullptr also u = D^-1 r, the first l1-Jacobi sweep of the
+// preconditioner (jacobi0_kernel's values)
+template <typename T>
 __global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B, const double2 *__restrict__ alpha,
-                                                               const V *__restrict__ p, const V *__restrict__ q,
-                                                               V *__restrict__ x, V *__restrict__ r,
-                                                               double2 *__restrict__ partial) {
+                                                               const typename VT<T>::t *__restrict__ p,
+                                                               const typename VT<T>::t *__restrict__ q,
+                                                               typename VT<T>::t *__restrict__ x,
+                                                               typename VT<T>::t *__restrict__ r,
+                                                               double2 *__restrict__ partial, OpArgs<T> la,
+                                                               const OpCoef *__restrict__ coefp,
+                                                               typename VT<T>::t *__restrict__ u) {
+  using V = typename VT<T>::t;
   const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
@@ -606,6 +613,7 @@ __global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B,
       x[i] = xv;
       r[i] = rv;
     }
+    if (u) u[i] = jacobi0_value<T>(la, coefp, row, f, rv);
     acc.x += (double)rv.x * rv.x + (double)rv.y * rv.y;
   }
   red_block_store(acc, B, partial);
@@ -753,10 +761,12 @@ __global__ void slot_fresh_kernel(int64_t n, int B, const int *__restrict__ fres
 }
 
 // columns (re)starting the inner solve: x_in = 0, r_in = r64 * s_f
-template <typename V>
+template <typename T>
 __global__ void slot_restart_kernel(int64_t nt, int B, const int *__restrict__ active, const double *__restrict__ s,
-                                    const double2 *__restrict__ r64, V *__restrict__ r, V *__restrict__ x,
-                                    const int32_t *__restrict__ perm) {
+                                    const double2 *__restrict__ r64, typename VT<T>::t *__restrict__ r,
+                                    typename VT<T>::t *__restrict__ x, const int32_t *__restrict__ perm,
+                                    OpArgs<T> la, const OpCoef *__restrict__ coefp, typename VT<T>::t *__restrict__ u) {
+  using V = typename VT<T>::t;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
   const int64_t row = i / B, f = i % B;
@@ -768,6 +778,7 @@ __global__ void slot_restart_kernel(int64_t nt, int B, const int *__restrict__ a
   o.y = v.y * sc;
   r[i] = o;
   x[i] = {0, 0};
+  if (u) u[i] = jacobi0_value<T>(la, coefp, row, (int)f, o);
 }
 
 // solver shadows: FP32 copies, Dirichlet columns zeroed; row l1 sums
@@ -1428,6 +1439,7 @@ struct Solver {
   double spmv_ms = 0;
   int64_t spmv_calls = 0;
   bool timing = true;
+  const V *u_of = nullptr;  // level-0 x holds D^-1 u_of (first smoothing sweep)
 
   Solver(Model &mm, Model::SolverState &ss, const elp_solve_opts &oo, hipStream_t ss2)
       : m(mm), st(ss), o(oo), s(ss2), ctx(mm.ctx) {
@@ -1507,11 +1519,20 @@ struct Solver {
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kRedFinalThreads), 0, s, kRedBlocks, B, red_part.p, dev_out);
     ELP_HIP(hipGetLastError());
   }
-  // x += alpha p, r -= alpha q, red_out = |r|^2 per frequency (level 0)
-  void update_norm() {
-    hipLaunchKernelGGL((cocg_update_norm_kernel<V>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, alpha.p, pi.p, qi.p,
-                       xi.p, ri.p, red_part.p);
+  // x += alpha p, r -= alpha q, red_out = |r|^2 per frequency (level 0);
+  // with track_u, also the preconditioner's first sweep u = D^-1 r into the
+  // level-0 work vector x (box_cycle then skips it)
+  void update_norm(bool track_u = false) {
+    hipLaunchKernelGGL((cocg_update_norm_kernel<T>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, alpha.p, pi.p, qi.p,
+                       xi.p, ri.p, red_part.p, st.levels[0]->template args<T>(), coefP.p,
+                       track_u ? lw[0].x.p : nullptr);
     final_reduce(red_out.p);
+    u_of = track_u ? ri.p : nullptr;
+  }
+  // the fused box cycle (32 frequencies, Hex20 box with products A P) is used
+  bool fused_box() const {
+    return std::is_same<T, float>::value && B == 32 && st.fdm && st.order == 2 && std::max(1, o.smooth) == 1 &&
+           st.levels[0]->ap_rp.p;
   }
   void fetch(const double2 *dev) {
     ELP_HIP(hipMemcpyAsync(h_red, dev, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
@@ -1574,8 +1595,8 @@ struct Solver {
     if constexpr (std::is_same<T, float>::value && B == 32) {
       if (nu == 1 && L.ap_rp.p) {
         // u = D^-1 r, v = r - A u, e = box solve of R v, then the fused
-        // z = u + P e + D^-1 (v - (A P) e)
-        smooth(0, b, nullptr, 1, W.x.p);
+        // z = u + P e + D^-1 (v - (A P) e); u may come with r (update_norm)
+        if (u_of != b) smooth(0, b, nullptr, 1, W.x.p);
         op<1>(0, coefP.p, W.x.p, b, W.r.p);
         transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, fb.p);
         fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
@@ -1993,8 +2014,11 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
                        r64.p);
     ELP_HIP(hipMemcpyAsync(d_scale.p, scale.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL((slot_restart_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.active.p,
-                       d_scale.p, r64.p, S.ri.p, S.xi.p, perm);
+    // restarted columns also get the preconditioner's first sweep (u) when
+    // the update keeps it for the others
+    hipLaunchKernelGGL((slot_restart_kernel<T>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.active.p,
+                       d_scale.p, r64.p, S.ri.p, S.xi.p, perm, S.st.levels[0]->template args<T>(), S.coefP.p,
+                       S.fused_box() ? S.lw[0].x.p : nullptr);
     ELP_HIP(hipGetLastError());
     std::fill(fresh.begin(), fresh.end(), 0);
   };
@@ -2028,7 +2052,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     S.spmv_calls++;
     S.reduce(m.n, S.pi.p, S.qi.p, 0, S.mu.p);
     hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
-    S.update_norm();
+    S.update_norm(S.fused_box());
     S.fetch(S.red_out.p);
     if (S.timing) {
       float ms = 0;
