@@ -85,10 +85,13 @@ typedef struct elp_solve_opts {
   int precision;     /* ELP_PREC_*                                        */
   int method;        /* ELP_METHOD_*                                      */
   int precond;       /* ELP_PC_*                                          */
-  int block;         /* frequencies per block (mixed), e.g. 16 or 32      */
+  int block;         /* mixed: frequency slots solved together (1, 4, 8,  */
+                     /* 16 or 32; a slot takes the next frequency as soon */
+                     /* as its current one is done)                       */
   double rtol;       /* FP64 relative residual target, e.g. 1e-8          */
-  int max_iters;     /* Krylov iterations per frequency block             */
+  int max_iters;     /* Krylov iterations per frequency                   */
   double inner_rtol; /* mixed: FP32 inner reduction per refinement step   */
+                     /* (the last step aims just past rtol when it can)   */
   int restart;       /* GMRES restart length                              */
   double shift;      /* shifted-Laplacian damping beta (MG)               */
   int smooth;        /* MG pre/post smoothing sweeps                      */
