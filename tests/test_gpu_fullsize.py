@@ -93,3 +93,22 @@ def test_config3_coupled_full(ctx):
     assert _null_err(Kff, np.ones(int(fl.sum()))) <= 1e-10
     Kss = K[~fl][:, ~fl]
     assert _sym_err(Kss) <= 1e-13 and _sym_err(Kff) <= 1e-13
+
+
+def test_config2_sweep_full_independent_residual(ctx):
+    """Three frequencies of the full Hex20 cavity sweep on 32 slots; the FP64
+    residual of each returned solution is recomputed on the host from
+    elPaSo's system A(w), F (elp_dynamic_matrix, the arrays PETSc would get)."""
+    m = MG.acoustic_cavity(100, 80, 60, quadratic=True)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = np.array([50.0, 1000.0, 1990.0])
+    X, rel, it, _ = g.sweep(freqs, block=32)
+    print(f"[fullsize] sweep relres {rel} iters {it}", flush=True)
+    assert (rel <= 1e-8).all()
+    rp, ci = g.pattern()
+    for k, f in enumerate(freqs):
+        a, F = g.dynamic_matrix(2 * np.pi * f)
+        A = sp.csr_matrix((a, ci, rp), shape=(g.n, g.n))
+        r = np.linalg.norm(F - A @ X[k]) / np.linalg.norm(F)
+        assert r <= 1.5e-8, (f, r, rel[k])
