@@ -56,6 +56,8 @@ template <int NPE, int NG>
 __device__ inline bool point_geometry(ElemLds<NPE, NG> &L, int g) {
   const double *dxi = tab_n<NPE>(g, 1), *deta = tab_n<NPE>(g, 2), *dzeta = tab_n<NPE>(g, 3);
   double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // partial unrolling keeps the 20-node loops of Hex20 within the register budget
+#pragma unroll 4
   for (int k = 0; k < NPE; k++) {
     double x0 = L.x[3 * k], x1 = L.x[3 * k + 1], x2 = L.x[3 * k + 2];
     J[0] += dxi[k] * x0; J[1] += dxi[k] * x1; J[2] += dxi[k] * x2;
@@ -75,6 +77,7 @@ __device__ inline bool point_geometry(ElemLds<NPE, NG> &L, int g) {
   iJ[5] = -(J[0] * J[5] - J[2] * J[3]) / det;
   iJ[8] = (J[0] * J[4] - J[1] * J[3]) / det;
   L.wdj[g] = tab_w<NPE>(g) * det;
+#pragma unroll 4
   for (int k = 0; k < NPE; k++) {
     double a = dxi[k], b = deta[k], c = dzeta[k];
     L.grad[g][0][k] = iJ[0] * a + iJ[1] * b + iJ[2] * c;
@@ -123,6 +126,7 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
     for (int t = lane; t < NPE * NPE; t += 64) {
       const int a = t / NPE, b = t % NPE;
       double k = 0.0, m = 0.0;
+#pragma unroll 3
       for (int g = 0; g < NG; g++) {
         const double wd = L.wdj[g];
         double r = L.grad[g][0][a] * L.grad[g][0][b] + L.grad[g][1][a] * L.grad[g][1][b] +
@@ -140,6 +144,7 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
       const int row = t / NDF, col = t % NDF;
       const int a = row / 3, d = row % 3, b = col / 3, dd = col % 3;
       double k = 0.0;
+#pragma unroll 3
       for (int g = 0; g < NG; g++) {
         const double pd = L.grad[g][d][a], qe = L.grad[g][dd][b];
         const double pe = L.grad[g][dd][a], qd = L.grad[g][d][b];
@@ -154,6 +159,7 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
     for (int t = lane; t < NPE * NPE; t += 64) {
       const int a = t / NPE, b = t % NPE;
       double m = 0.0;
+#pragma unroll 3
       for (int g = 0; g < NG; g++) {
         const double *N = tab_n<NPE>(g, 0);
         m += rho * N[a] * N[b] * L.wdj[g];
