@@ -481,7 +481,10 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
   const int lane = threadIdx.x;
   const int k = lane / 8, g = lane % 8, b = lane % 8;
   {
-    const int32_t s_node = blockIdx.x;
+    // consecutive nodes (sharing elements and coordinates) on one XCD: the
+    // hardware deals workgroups round-robin to the 8 XCDs
+    const int32_t per = gridDim.x / 8;
+    const int32_t s_node = (int32_t)(blockIdx.x % 8) * per + (int32_t)(blockIdx.x / 8);
     if (s_node >= n_nodes) return;
     const int32_t i = nodes[s_node];
     const int32_t a0 = adj_ptr[s_node], nadj = adj_ptr[s_node + 1] - a0;
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
 static void launch_hex8_node(Model &m, KindBlock &kb, int *bad, hipStream_t s) {
   const int32_t nn = (int32_t)kb.nodes.size();
   if (nn == 0) return;
-  const int grid = nn;  // one wavefront per node
+  const int grid = (nn + 7) / 8 * 8;  // one wavefront per node, a multiple of the 8 XCDs
   double *Cv = m.has_rayleigh ? m.d_C.p : nullptr;
 #define ELP_HEX8_ARGS                                                                                   \
   nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, kb.d_params.p,   \
