@@ -1,5 +1,6 @@
-// Harmonic sweep solver: fused multi-frequency operator, shifted-Laplacian
-// multigrid / l1-Jacobi preconditioners, batched COCG, FP64 refinement.
+// Harmonic sweep solver: fused multi-frequency operator, box / multigrid /
+// l1-Jacobi preconditioners, COCG on continuously refilled frequency slots,
+// FP64 refinement.
 #pragma once
 
 #include <complex>
@@ -14,7 +15,7 @@ void default_opts(elp_solve_opts &o);
 void sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z, const elp_solve_opts &o,
            double *X, double *relres, int32_t *iters, elp_sweep_stats *stats, hipStream_t s);
 
-// device bytes the solver needs for a block of `block` frequencies
+// device bytes the solver needs for `block` frequency slots
 int64_t solver_bytes_estimate(const Model &m, int block, int precision);
 
 }  // namespace elp

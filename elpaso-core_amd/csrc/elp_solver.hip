@@ -10,21 +10,24 @@
 // (conjugate orthogonal CG: CG with the unconjugated bilinear form), whose
 // short recurrence keeps the memory per frequency at a handful of vectors.
 //
-// Vectors of a block of B frequencies are stored [row][B] (frequency
-// fastest): one gather of a column index fetches the B values of that column
-// for all frequencies in one contiguous segment, and K, M and the column
-// indices are read once per block of frequencies (the fused multi-frequency
-// SpMV).  Mixed precision: the Krylov iteration and the preconditioner run in
-// complex FP32 on FP32 shadows of K and M; an outer FP64 iterative refinement
-// recomputes the true residual with the FP64 matrices until ||r||/||b|| <= rtol.
+// Vectors of B frequency slots are stored [row][B] (frequency fastest): one
+// gather of a column fetches the B values of that column in one contiguous
+// line, and K, M and the column indices are read once for all slots (the
+// fused multi-frequency operator; for B = 32 row_op_kernel, one wavefront per
+// row).  Mixed precision: the Krylov iteration and the preconditioner run in
+// complex FP32 on FP32 copies of K and M; an outer FP64 iterative refinement
+// recomputes the true residual with the FP64 matrices until ||r||/||b|| <=
+// rtol.  A sweep runs on B slots that are refilled as frequencies finish
+// (solve_slots).
 //
-// Preconditioners: l1-Jacobi (any mesh), or a complex-shifted-Laplacian
-// multigrid V-cycle for fluid box models: P(w) = K/w^2 - (1 - i beta) M + ...,
-// levels = the model, then (for Hex20) the Hex8 mesh on the same grid, then
-// Hex8 meshes coarsened by 2, each rediscretized with the same assembly kernels
-// and connected by trilinear interpolation; l1-Jacobi smoothing.  The V-cycle
-// is a fixed linear operator and complex symmetric (R = P^T, symmetric
-// smoothing), as COCG requires.
+// Preconditioners: for fluid box models the exact fast-diagonalization solve
+// of the Hex8 box operator (elp_fdm), used directly for Hex8 models and as the
+// coarse level of a symmetric two-level cycle (l1-Jacobi pre- and post-sweep)
+// for Hex20, with the shifted operator P(w) = K/w^2 - (1 - i beta) M + ...;
+// otherwise a rediscretized Hex8 multigrid hierarchy (trilinear transfer,
+// l1-Jacobi smoothing) or l1-Jacobi alone.  Every cycle is a fixed linear
+// operator and complex symmetric (R = P^T, symmetric smoothing), as COCG
+// requires.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2165,6 +2168,8 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
 int64_t solver_bytes_estimate(const Model &m, int block, int precision) {
   const int64_t vb = precision == ELP_PREC_FP64 ? 16 : 8;
   int64_t b = m.nnz * 8 + m.n * 24;                    // FP32 shadows, l1 sums
+  b += m.nnz * 18;                                     // permuted columns, row-padded operator copy
+  b += m.n * 512;                                      // box products A P, padded P rows, lattice
   b += (int64_t)m.n * block * (5 * vb * 2 + 4 * 16);   // level-0 + Krylov + FP64 outer vectors
   return b + b / 4;                                    // coarse levels (about 1/8 each) and slack
 }
