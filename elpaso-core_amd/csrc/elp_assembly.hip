@@ -17,6 +17,8 @@
 // a segmented reduction over a precomputed, ordered contribution list.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "elp_fem.h"
 #include "elp_internal.h"
 
@@ -413,54 +415,78 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
   ELP_HIP(hipGetLastError());
 }
 
-// Hex8 shape values at a 2x2x2 Gauss point computed in registers, the same
-// arithmetic as the host table (lin1 factors 1 +- 0.577350269189626 chosen
-// by the signs of node and point, products ((a*b)*c)*0.125): bit-identical to
-// c_tab.h8 without table loads.  Node k's natural coordinate signs are bits
-// of 0x66 (x), 0xCC (y), 0xF0 (z); point g has x index g % 2, y (g / 2) % 2,
-// z g / 4, index 0 = +0.577350269189626.
-struct Hex8Shape {
-  double n, dx, dy, dz;
-};
-__device__ inline Hex8Shape hex8_shape(int k, int g) {
-  constexpr double gp = 0.577350269189626, hi = 1. + gp, lo = 1. - gp;
-  const bool sx = (0x66 >> k) & 1, sy = (0xCC >> k) & 1, sz = (0xF0 >> k) & 1;
-  const double a = (sx == !(g & 1)) ? hi : lo;
-  const double b = (sy == !((g >> 1) & 1)) ? hi : lo;
-  const double c = (sz == !((g >> 2) & 1)) ? hi : lo;
-  Hex8Shape r;
-  r.n = a * b * c * 0.125;
-  r.dx = (sx ? 1.0 : -1.0) * (b * c * 0.125);
-  r.dy = (sy ? 1.0 : -1.0) * (a * c * 0.125);
-  r.dz = (sz ? 1.0 : -1.0) * (a * b * 0.125);
-  return r;
-}
+// Hex8 trilinear factors at the 2x2x2 Gauss points.  Node q's natural
+// coordinate signs are bits of 0x66 (x), 0xCC (y), 0xF0 (z); point g has the
+// indices g % 2, (g / 2) % 2, g / 4 (index 0 = +0.577350269189626).  The 1-D
+// factor of a node at a point is 1 + 0.577350269189626 where the node's sign
+// bit differs from the point's index and 1 - 0.577350269189626 where it is
+// equal.  Derivatives +-(f_y f_z) * 0.125 (and cyclic) are the arithmetic of
+// the host table c_tab.h8, so the Jacobian is bit-identical to a table-driven
+// one.
+constexpr double kH8Hi = 1. + 0.577350269189626, kH8Lo = 1. - 0.577350269189626;
+__host__ __device__ constexpr int h8_sx(int q) { return (0x66 >> q) & 1; }
+__host__ __device__ constexpr int h8_sy(int q) { return (0xCC >> q) & 1; }
+__host__ __device__ constexpr int h8_sz(int q) { return (0xF0 >> q) & 1; }
+__device__ inline double h8_f(int s, int ix) { return s != ix ? kH8Hi : kH8Lo; }
 
 // Fused assembly of trilinear (Hex8) elements directly into the CSR rows of
-// one node per wavefront, without element matrices in HBM:
-//  1. lane (k, g): geometry of adjacent element k at quadrature point g
-//     (inverse Jacobian and w detJ) into LDS;
-//  2. lane (k, b): the contribution of element k to the (node, neighbour b)
-//     block, contracted over the 8 points from LDS;
-//  3. lane per neighbour slot: sum of its contributions in ascending element
-//     order (elPaSo's MatSetValues order), finished into K, M (and C) rows in
-//     LDS;
-//  4. the node's rows are written with coalesced stores.
+// the nodes, without element matrices in HBM.  A persistent wavefront takes
+// its nodes one after another; the node's adjacent elements are taken in
+// batches of 8 in ascending element id, and per batch:
+//  0. lane (k, q) stages the coordinates of node q of element k in LDS;
+//  1. lane (k, g) computes the geometry of element k at quadrature point g
+//     (Jacobian, determinant, inverse) and the physical gradients of the 8
+//     shape functions there, staged in LDS with w detJ;
+//  2. lane (k, b) contracts element k's block of the (node a, neighbour b)
+//     pair over the 8 points: P = sum_g w detJ grad N_a (x) grad N_b gives
+//     the block lambda P + mu P^T + mu tr(P) I (Brick8) or tr(P) / rho
+//     (Fluid8), plus the mass term sum_g w detJ N_a N_b;
+//  3. the blocks go to LDS and the lanes owning a neighbour slot (two lanes
+//     per slot for 3 DOF, one per half of the block) add them into registers
+//     in ascending element id, the order of elPaSo's MatSetValues.
+// After the last batch the slot lanes write their entries of K, M (and C)
+// straight into the node's CSR rows.  No atomics on values: repeated runs are
+// bit-identical.
+//
+// The inputs of a node sit four dependent loads deep (descriptor ->
+// adjacency -> connectivity -> coordinates).  For the nodes in at most 8
+// elements with at most one pass of slots (32 for 3 DOF, 64 for 1: all nodes
+// of a structured mesh away from interfaces) the wavefront issues those of
+// its next node between the phases of the current one, so their latency
+// hides behind its arithmetic, and runs one batch and one pass.  The others
+// (GENERAL) load their inputs in place and run the batches once per group of
+// slots.
 constexpr int kMaxHex8Cols = 128;  // row length / neighbours of a Hex8 node (checked at finalize)
 template <int ND>
 struct Hex8Lds {
-  // [(k*8+g)*10 + k + t]: invJ (t < 9), w detJ (t = 9); the pad of one
-  // double per element puts the 8 elements' rows read together in phase 2
-  // in distinct LDS banks
-  double geo[64 * 10 + 8];
+  // lane (k, b)'s block at an odd stride of doubles (no LDS bank conflicts):
+  // 3 DOF: half 0 = K00 K01 K02 K10 K11 M, half 1 = K12 K20 K21 K22 0 M;
+  // 1 DOF: K, M
+  static constexpr int kRec = ND == 1 ? 3 : 13, kHalf = ND == 1 ? 2 : 6;
   union {
-    double xe[64][3];  // phase 0-1: coordinates of node q of element k, [k*8+q]
-    double kr[ND][kMaxHex8Cols];
+    double xe[3][64];      // phase 0-1: coordinate c of node q of element k, [c][k*8+q]
+    double gr[3][8 * 72];  // phase 1-2: derivative c of N_q at point g of element k,
+                           // [c][k*72 + g*9 + q] (odd point stride: no bank conflicts);
+                           // the ninth double of each point row holds w detJ (c = 0) and
+                           // the slot table (c = 2)
+    double v[64][kRec];    // phase 3: the block of lane (k, b) (below gr[2])
   } u;
-  double mr[kMaxHex8Cols];
-  int32_t nb[kMaxHex8Cols];
-  int32_t off[kMaxHex8Cols + 1];
-  int16_t colslot[kMaxHex8Cols];
+  __device__ double &wd(int k, int g) { return u.gr[0][k * 72 + g * 9 + 8]; }
+  // slot sp of the pass: byte k = b of batch element k touching it, 0xFF if none
+  __device__ uint64_t &cb(int sp) {
+    return *reinterpret_cast<uint64_t *>(&u.gr[2][(sp >> 3) * 72 + (sp & 7) * 9 + 8]);
+  }
+};
+static_assert(sizeof(double) * 64 * 13 <= sizeof(double) * 2 * 8 * 72, "blocks overlap the slot table");
+
+// The chain of one node's inputs that is loaded ahead (lane (k, j) holds
+// element k's share); what the node needs only from its phase 2 on (local
+// index, slots, material, row offsets) is loaded when its work starts.
+struct Hex8Fetch {
+  int32_t raw;                                // descriptor field lane & 7
+  int32_t a0, nadj, nb0, deg, rowbase, len;  // descriptor (wave-uniform)
+  int32_t e, nq;                              // element k, rank of its node j
+  double x0, x1, x2;                          // coordinates of node j
 };
 
 __device__ inline void wave_sync() {
@@ -469,182 +495,376 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int ND>
+template <int ND, bool GENERAL, bool WITH_C>
 __global__ __launch_bounds__(64) void hex8_node_kernel(
-    int32_t n_nodes, const int32_t *__restrict__ nodes, const int32_t *__restrict__ adj_ptr,
-    const int32_t *__restrict__ adj_elem, const uint8_t *__restrict__ adj_local, const int32_t *__restrict__ conn,
-    const double *__restrict__ params, const double *__restrict__ xyz, const int32_t *__restrict__ nbr_ptr,
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ nbr_off, const int32_t *__restrict__ node_row,
-    const int32_t *__restrict__ rowptr, double *__restrict__ Kv, double *__restrict__ Mv, double *__restrict__ Cv,
-    double alpha, double beta, int *bad) {
+    int32_t n_nodes, const int32_t *__restrict__ desc, const int32_t *__restrict__ adj_elem,
+    const uint8_t *__restrict__ adj_local, const uint8_t *__restrict__ adj_slot, const int32_t *__restrict__ conn,
+    const double *__restrict__ params, const double *__restrict__ xyz, const int32_t *__restrict__ nbr_off,
+    double *__restrict__ Kv, double *__restrict__ Mv, double *__restrict__ Cv, double alpha, double beta,
+    int *bad) {
+  constexpr int H = ND == 1 ? 1 : 2;  // lanes per neighbour slot
+  constexpr int SPP = 64 / H;         // slots per pass
+  constexpr int RH = Hex8Lds<ND>::kHalf;
   __shared__ Hex8Lds<ND> L;
   const int lane = threadIdx.x;
-  const int k = lane / 8, g = lane % 8, b = lane % 8;
-  {
-    // consecutive nodes (sharing elements and coordinates) on one XCD: the
-    // hardware deals workgroups round-robin to the 8 XCDs
-    const int32_t per = gridDim.x / 8;
-    const int32_t s_node = (int32_t)(blockIdx.x % 8) * per + (int32_t)(blockIdx.x / 8);
-    if (s_node >= n_nodes) return;
-    const int32_t i = nodes[s_node];
-    const int32_t a0 = adj_ptr[s_node], nadj = adj_ptr[s_node + 1] - a0;
-    const int32_t nb0 = nbr_ptr[i], deg = nbr_ptr[i + 1] - nb0;
-    for (int t = lane; t < deg; t += 64) {
-      L.nb[t] = nbr[nb0 + t];
-      L.off[t] = nbr_off[nb0 + t];
-    }
-    const int32_t r0 = node_row[i];
-    const int32_t len = rowptr[r0 + 1] - rowptr[r0];
-    if (lane == 0) L.off[deg] = len;
-    // 0. coordinates: lane (k, q) loads node q of element k
-    int32_t e = -1, a = 0;
-    if (k < nadj) {
-      e = adj_elem[a0 + k];
-      a = adj_local[a0 + k];
-      const int32_t nd_ = conn[e * 8 + g];
-      L.u.xe[lane][0] = xyz[nd_ * 3];
-      L.u.xe[lane][1] = xyz[nd_ * 3 + 1];
-      L.u.xe[lane][2] = xyz[nd_ * 3 + 2];
-    }
-    wave_sync();
-    // 1. geometry of (element k, point g): inverse Jacobian, w detJ, and
-    //    w detJ grad N_a of the node itself
-    if (k < nadj) {
-      double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int k = lane >> 3, j = lane & 7;  // j: node q (phase 0), point g (1), neighbour b (2)
+  const int gxi = j & 1, gyi = (j >> 1) & 1, gzi = j >> 2;
+  int32_t bad_elem = INT32_MAX;  // lowest element with detJ <= 0 seen by this lane
+
+  // the nodes are cut into 8 ranges, one per XCD (the hardware deals
+  // workgroups round-robin to the XCDs): neighbouring nodes, which share
+  // elements and coordinates, run on one XCD at the same time
+  const int32_t wx = (int32_t)(gridDim.x >> 3);
+  const int32_t per = (n_nodes + 7) >> 3;
+  const int32_t beg = (int32_t)(blockIdx.x & 7) * per, end = min(n_nodes, beg + per);
+  int32_t t = beg + (int32_t)(blockIdx.x >> 3);
+  if (t >= end) return;
+
+  // the four load levels of node tt
+  auto fetch1 = [&](Hex8Fetch &f, int32_t tt) { f.raw = desc[(int64_t)tt * 8 + j]; };
+  auto fetch2 = [&](Hex8Fetch &f) {
+    f.a0 = __builtin_amdgcn_readlane(f.raw, 0);
+    f.nadj = __builtin_amdgcn_readlane(f.raw, 1);
+    f.nb0 = __builtin_amdgcn_readlane(f.raw, 2);
+    f.deg = __builtin_amdgcn_readlane(f.raw, 3);
+    f.rowbase = __builtin_amdgcn_readlane(f.raw, 4);
+    f.len = __builtin_amdgcn_readlane(f.raw, 5);
+    // Every lane loads, lanes past the node's elements repeating the last
+    // one: loads under a branch would make the number of loads in flight
+    // path-dependent, and the compiler then waits for all of them wherever
+    // an earlier-loaded value is used.
+    f.e = adj_elem[f.a0 + min(k, f.nadj - 1)];
+  };
+  auto fetch3 = [&](Hex8Fetch &f) { f.nq = conn[(int64_t)f.e * 8 + j]; };
+  auto fetch4 = [&](Hex8Fetch &f) {
+    f.x0 = xyz[(int64_t)f.nq * 3];
+    f.x1 = xyz[(int64_t)f.nq * 3 + 1];
+    f.x2 = xyz[(int64_t)f.nq * 3 + 2];
+  };
+
+  // node c's work; the hooks issue the loads of the next nodes between its
+  // phases (at the start, after the geometry, after the contraction)
+  auto process = [&](const Hex8Fetch &c, auto &&at_start, auto &&after_geometry, auto &&after_blocks) {
+    const int npass = GENERAL ? (c.deg + SPP - 1) / SPP : 1;
+    const int nbatch = GENERAL ? (c.nadj + 7) / 8 : 1;
+    for (int ip = 0; ip < npass; ip++) {
+      const int s0 = ip * SPP;
+      const int s = s0 + lane / H, h = lane % H;  // slot of this lane, half of its block
+      // offsets of the lane's slot and of the next one in the rows (the width
+      // is taken where the rows are written: using loaded values early would
+      // wait for them)
+      const int32_t off = GENERAL ? nbr_off[c.nb0 + min(s, c.deg - 1)] : 0;
+      const int32_t offn = GENERAL ? nbr_off[c.nb0 + min(s + 1, c.deg - 1)] : 0;
+      double acc[RH];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const double *x = L.u.xe[k * 8 + q];
-        const double x0 = x[0], x1 = x[1], x2 = x[2];
-        const Hex8Shape h = hex8_shape(q, g);
-        J[0] += h.dx * x0; J[1] += h.dx * x1; J[2] += h.dx * x2;
-        J[3] += h.dy * x0; J[4] += h.dy * x1; J[5] += h.dy * x2;
-        J[6] += h.dz * x0; J[7] += h.dz * x1; J[8] += h.dz * x2;
-      }
-      const double det = J[0] * J[4] * J[8] + J[1] * J[5] * J[6] + J[2] * J[3] * J[7] -
-                         J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
-      if (!(det > 0.0)) atomicMin(bad, e);
-      const double id = 1.0 / det;
-      double G[9];
-      G[0] = (J[4] * J[8] - J[5] * J[7]) * id;
-      G[3] = -(J[3] * J[8] - J[5] * J[6]) * id;
-      G[6] = (J[3] * J[7] - J[4] * J[6]) * id;
-      G[1] = -(J[1] * J[8] - J[2] * J[7]) * id;
-      G[4] = (J[0] * J[8] - J[2] * J[6]) * id;
-      G[7] = -(J[0] * J[7] - J[1] * J[6]) * id;
-      G[2] = (J[1] * J[5] - J[2] * J[4]) * id;
-      G[5] = -(J[0] * J[5] - J[2] * J[3]) * id;
-      G[8] = (J[0] * J[4] - J[1] * J[3]) * id;
-      double *O = L.geo + lane * 10 + k;
+      for (int u = 0; u < RH; u++) acc[u] = 0.0;
+      for (int ib = 0; ib < nbatch; ib++) {
+        const int base = ib * 8;
+        const int nk = min(8, c.nadj - base);
+        const bool first = s0 == 0 && base == 0;  // the pass and batch of the prefetched inputs
+        // batch inputs: the coordinate chain prefetched for the first batch,
+        // loaded here for the others; the rest is used from phase 2 on
+        const int32_t p = c.a0 + base + min(k, nk - 1);
+        int32_t e = c.e;
+        double x0 = c.x0, x1 = c.x1, x2 = c.x2;
+        if (GENERAL && !first) {
+          e = adj_elem[p];
+          const int32_t nq = conn[(int64_t)e * 8 + j];
+          x0 = xyz[(int64_t)nq * 3];
+          x1 = xyz[(int64_t)nq * 3 + 1];
+          x2 = xyz[(int64_t)nq * 3 + 2];
+        }
+        const int32_t a = adj_local[p], slot = adj_slot[(int64_t)p * 8 + j];
+        const double pl = params[(int64_t)e * 4], pm = params[(int64_t)e * 4 + 1], pr = params[(int64_t)e * 4 + 2];
+        wave_sync();  // the previous batch's blocks are read
+        if (first) at_start();
+        if (lane < SPP) L.cb(lane) = ~0ull;
+        if (k < nk) {
+          L.u.xe[0][lane] = x0;
+          L.u.xe[1][lane] = x1;
+          L.u.xe[2][lane] = x2;
+        }
+        wave_sync();
+        // 1. geometry of (element k, point g = j): J(i,j) = sum_q dN_q/dxi_i x_q[j],
+        //    det by Sarrus' rule, inverse by cofactors, gradients invJ dN
+        //    (element/elementfem.cpp:158-210); derivative magnitudes
+        //    yz[sy][sz] = (f_y f_z) / 8 etc. of lane j as point g
+        double yz[2][2], xz[2][2], xy[2][2];
 #pragma unroll
-      for (int t = 0; t < 9; t++) O[t] = G[t];
-      O[9] = c_tab.w8[g] * det;
-    }
-    wave_sync();
-    // zero the row accumulators (they reuse the coordinate staging) and map
-    // every column of the rows to its neighbour slot
-    for (int q = lane; q < len; q += 64) {
+        for (int u = 0; u < 2; u++)
 #pragma unroll
-      for (int d = 0; d < ND; d++) L.u.kr[d][q] = 0.0;
-    }
-    for (int sl = lane; sl < deg; sl += 64) {
-      L.mr[sl] = 0.0;
-      for (int q = L.off[sl]; q < L.off[sl + 1]; q++) L.colslot[q] = (int16_t)sl;
-    }
-    // 2. unit (k, b): element k's block of the (node, neighbour b) pair, with
-    //    the element's own material law
-    double V[ND == 1 ? 2 : 10];
-    int slot = -1;
-    if (k < nadj) {
-      slot = find_slot(L.nb, deg, conn[e * 8 + b]);
-      double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mm = 0.0, kk = 0.0;
+          for (int v = 0; v < 2; v++) {
+            yz[u][v] = (h8_f(u, gyi) * h8_f(v, gzi)) * 0.125;
+            xz[u][v] = (h8_f(u, gxi) * h8_f(v, gzi)) * 0.125;
+            xy[u][v] = (h8_f(u, gxi) * h8_f(v, gyi)) * 0.125;
+          }
+        double G[9], wdj = 0.0;
+        if (k < nk) {
+          double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-      for (int gg = 0; gg < 8; gg++) {
-        const double *G = L.geo + (k * 8 + gg) * 10 + k;
-        const Hex8Shape ha = hex8_shape(a, gg), hb = hex8_shape(b, gg);
-        const double ax = ha.dx, ay = ha.dy, az = ha.dz;
-        const double bx = hb.dx, by = hb.dy, bz = hb.dz;
-        const double wd = G[9];
-        const double p0 = G[0] * ax + G[1] * ay + G[2] * az;
-        const double p1 = G[3] * ax + G[4] * ay + G[5] * az;
-        const double p2 = G[6] * ax + G[7] * ay + G[8] * az;
-        const double q0 = G[0] * bx + G[1] * by + G[2] * bz;
-        const double q1 = G[3] * bx + G[4] * by + G[5] * bz;
-        const double q2 = G[6] * bx + G[7] * by + G[8] * bz;
-        mm += ha.n * hb.n * wd;
-        const double w0 = wd * p0, w1 = wd * p1, w2 = wd * p2;
-        if constexpr (ND == 1) {
-          kk += w0 * q0 + w1 * q1 + w2 * q2;
-        } else {
-          P[0] += w0 * q0; P[1] += w0 * q1; P[2] += w0 * q2;
-          P[3] += w1 * q0; P[4] += w1 * q1; P[5] += w1 * q2;
-          P[6] += w2 * q0; P[7] += w2 * q1; P[8] += w2 * q2;
+          for (int q = 0; q < 8; q++) {
+            const double c0 = L.u.xe[0][k * 8 + q], c1 = L.u.xe[1][k * 8 + q], c2 = L.u.xe[2][k * 8 + q];
+            const double dx = (h8_sx(q) ? 1.0 : -1.0) * yz[h8_sy(q)][h8_sz(q)];
+            const double dy = (h8_sy(q) ? 1.0 : -1.0) * xz[h8_sx(q)][h8_sz(q)];
+            const double dz = (h8_sz(q) ? 1.0 : -1.0) * xy[h8_sx(q)][h8_sy(q)];
+            J[0] += dx * c0; J[1] += dx * c1; J[2] += dx * c2;
+            J[3] += dy * c0; J[4] += dy * c1; J[5] += dy * c2;
+            J[6] += dz * c0; J[7] += dz * c1; J[8] += dz * c2;
+          }
+          const double det = J[0] * J[4] * J[8] + J[1] * J[5] * J[6] + J[2] * J[3] * J[7] -
+                             J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
+          if (!(det > 0.0)) bad_elem = min(bad_elem, e);
+          const double id = 1.0 / det;
+          G[0] = (J[4] * J[8] - J[5] * J[7]) * id;
+          G[3] = -(J[3] * J[8] - J[5] * J[6]) * id;
+          G[6] = (J[3] * J[7] - J[4] * J[6]) * id;
+          G[1] = -(J[1] * J[8] - J[2] * J[7]) * id;
+          G[4] = (J[0] * J[8] - J[2] * J[6]) * id;
+          G[7] = -(J[0] * J[7] - J[1] * J[6]) * id;
+          G[2] = (J[1] * J[5] - J[2] * J[4]) * id;
+          G[5] = -(J[0] * J[5] - J[2] * J[3]) * id;
+          G[8] = (J[0] * J[4] - J[1] * J[3]) * id;
+          wdj = det;  // w = 1 at every point of the 2x2x2 rule
+        }
+        wave_sync();  // coordinates read: the gradients overwrite them
+        if (k < nk) {
+          const int row = k * 72 + j * 9;
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const double dx = (h8_sx(q) ? 1.0 : -1.0) * yz[h8_sy(q)][h8_sz(q)];
+            const double dy = (h8_sy(q) ? 1.0 : -1.0) * xz[h8_sx(q)][h8_sz(q)];
+            const double dz = (h8_sz(q) ? 1.0 : -1.0) * xy[h8_sx(q)][h8_sy(q)];
+            L.u.gr[0][row + q] = G[0] * dx + G[1] * dy + G[2] * dz;
+            L.u.gr[1][row + q] = G[3] * dx + G[4] * dy + G[5] * dz;
+            L.u.gr[2][row + q] = G[6] * dx + G[7] * dy + G[8] * dz;
+          }
+          L.wd(k, j) = wdj;
+        }
+        if (first) after_geometry();
+        wave_sync();
+        // 2. unit (k, b = j): element k's block of the (node, neighbour b) pair
+        double V[Hex8Lds<ND>::kRec];
+        if (k < nk) {
+          // N_a N_b / 64 at point (ix, iy, iz) = pxy[ix + 2 iy] * pz[iz]
+          const double px0 = h8_f(h8_sx(a), 0) * h8_f(h8_sx(j), 0), px1 = h8_f(h8_sx(a), 1) * h8_f(h8_sx(j), 1);
+          const double py0 = h8_f(h8_sy(a), 0) * h8_f(h8_sy(j), 0), py1 = h8_f(h8_sy(a), 1) * h8_f(h8_sy(j), 1);
+          const double pz[2] = {h8_f(h8_sz(a), 0) * h8_f(h8_sz(j), 0) * 0.015625,
+                                h8_f(h8_sz(a), 1) * h8_f(h8_sz(j), 1) * 0.015625};
+          const double pxy[4] = {px0 * py0, px1 * py0, px0 * py1, px1 * py1};
+          const double *gx = L.u.gr[0] + k * 72, *gy = L.u.gr[1] + k * 72, *gz = L.u.gr[2] + k * 72;
+          double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mm = 0.0, kk = 0.0;
+#pragma unroll
+          for (int gg = 0; gg < 8; gg++) {
+            const double ax = gx[gg * 9 + a], ay = gy[gg * 9 + a], az = gz[gg * 9 + a];
+            const double bx = gx[gg * 9 + j], by = gy[gg * 9 + j], bz = gz[gg * 9 + j];
+            const double wd = L.wd(k, gg);
+            const double w0 = wd * ax, w1 = wd * ay, w2 = wd * az;
+            mm += (pxy[gg & 3] * pz[gg >> 2]) * wd;
+            if constexpr (ND == 1) {
+              kk += w0 * bx + w1 * by + w2 * bz;
+            } else {
+              P[0] += w0 * bx; P[1] += w0 * by; P[2] += w0 * bz;
+              P[3] += w1 * bx; P[4] += w1 * by; P[5] += w1 * bz;
+              P[6] += w2 * bx; P[7] += w2 * by; P[8] += w2 * bz;
+            }
+            // one point's LDS reads at a time: hoisting all 8 would cost
+            // registers, and occupancy, for latency the other waves cover
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          // the element's own material law (params: Fluid8 rho, 1/c^2; Brick8
+          // lambda, mu, rho)
+          if constexpr (ND == 1) {
+            V[0] = kk / pl;
+            V[1] = pm * mm / pl;
+            V[2] = 0.0;
+          } else {
+            const double tr = P[0] + P[4] + P[8];
+            double Kb[9];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+              for (int dd = 0; dd < 3; dd++)
+                Kb[d * 3 + dd] = pl * P[d * 3 + dd] + pm * P[dd * 3 + d] + (d == dd ? pm * tr : 0.0);
+            const double M = pr * mm;
+            V[0] = Kb[0]; V[1] = Kb[1]; V[2] = Kb[2]; V[3] = Kb[3]; V[4] = Kb[4]; V[5] = M;
+            V[6] = Kb[5]; V[7] = Kb[6]; V[8] = Kb[7]; V[9] = Kb[8]; V[10] = 0.0; V[11] = M; V[12] = 0.0;
+          }
+        }
+        if (first) after_blocks();
+        wave_sync();
+        // 3. blocks to LDS, then the slot lanes add them in ascending element order
+        if (k < nk) {
+#pragma unroll
+          for (int u = 0; u < Hex8Lds<ND>::kRec - 1; u++) L.u.v[lane][u] = V[u];
+          if (slot >= s0 && slot < s0 + SPP) reinterpret_cast<uint8_t *>(&L.cb(slot - s0))[k] = (uint8_t)j;
+        }
+        wave_sync();
+        if (s < c.deg) {
+          const uint64_t c8 = L.cb(s - s0);
+#pragma unroll
+          for (int kk = 0; kk < 8; kk++) {
+            const int bb = (int)((c8 >> (8 * kk)) & 0xFF);
+            if (bb != 0xFF) {
+              const double *src = L.u.v[kk * 8 + bb] + h * RH;
+#pragma unroll
+              for (int u = 0; u < RH; u++) acc[u] += src[u];
+            }
+          }
         }
       }
-      const double p0m = params[e * 4], p1m = params[e * 4 + 1], p2m = params[e * 4 + 2];
-      if constexpr (ND == 1) {
-        V[0] = kk / p0m;
-        V[1] = p1m * mm / p0m;
-      } else {
-        const double tr = P[0] + P[4] + P[8];
+      // 4. the node's rows.  Common path: the slot lanes place their entries
+      //    in an LDS image of the rows (slot t is columns ND t..ND t+ND-1),
+      //    then the wavefront writes the rows with coalesced stores, a fixed
+      //    number of them (lanes past the row end repeat its last entry with
+      //    the same value) so that no store sits under a branch.
+      if constexpr (!GENERAL) {
+        constexpr int kCap = ND == 1 ? 64 : 96;  // row length of a node with one pass of slots
+        double *kr = L.u.gr[0];                  // [d * kCap + column]
+        double *mr = L.u.gr[0] + ND * kCap;      // mass per slot
+        wave_sync();  // blocks and slot table read
+        if (s < c.deg) {
+          if constexpr (ND == 1) {
+            kr[s] = acc[0];
+            mr[s] = acc[1];
+          } else {
 #pragma unroll
-        for (int d = 0; d < 3; d++)
+            for (int u = 0; u < 5; u++) {
+              const int en = h * 5 + u;  // entry (d, dd) of the 3x3 block, row-major
+              if (en < 9) kr[(en / 3) * kCap + 3 * s + en % 3] = acc[u];
+            }
+            if (h == 0) mr[s] = acc[5];
+          }
+        }
+        wave_sync();
 #pragma unroll
-          for (int dd = 0; dd < 3; dd++)
-            V[d * 3 + dd] = p0m * P[d * 3 + dd] + p1m * P[dd * 3 + d] + (d == dd ? p1m * tr : 0.0);
-        V[9] = p2m * mm;
-      }
-    }
-    wave_sync();
-    // 3. ordered accumulation: element k's 8 units (distinct slots) add in
-    //    round k, so every entry sums in ascending element id
-    for (int kk = 0; kk < nadj; kk++) {
-      if (k == kk) {
-        const int off = L.off[slot];
+        for (int d = 0; d < ND; d++)
+#pragma unroll
+          for (int it = 0; it < kCap / 64 + (kCap % 64 ? 1 : 0); it++) {
+            const int col = min(it * 64 + lane, c.len - 1);
+            const double kv = kr[d * kCap + col];
+            const double mv = ND == 1 ? mr[col] : (col % 3 == d ? mr[col / 3] : 0.0);
+            const int64_t rb = (int64_t)c.rowbase + (int64_t)d * c.len + col;
+            Kv[rb] = kv;
+            Mv[rb] = mv;
+            if constexpr (WITH_C) Cv[rb] = alpha * kv + beta * mv;
+          }
+      } else if (s < c.deg) {
+        // GENERAL: the slot lanes' entries straight into the rows; a slot that
+        // no element touches (an interface partner of another field) holds zeros
+        const int32_t wid = (s + 1 < c.deg ? offn : c.len) - off;
         if constexpr (ND == 1) {
-          L.u.kr[0][off] += V[0];
-          L.mr[slot] += V[1];
+          const int64_t rb = (int64_t)c.rowbase + off;
+          if (wid == 1) {
+            Kv[rb] = acc[0];
+            Mv[rb] = acc[1];
+            if constexpr (WITH_C) Cv[rb] = alpha * acc[0] + beta * acc[1];
+          } else {
+            for (int u = 0; u < wid; u++) {
+              Kv[rb + u] = 0.0;
+              Mv[rb + u] = 0.0;
+              if constexpr (WITH_C) Cv[rb + u] = 0.0;
+            }
+          }
         } else {
+          if (wid == 3) {
 #pragma unroll
-          for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int dd = 0; dd < 3; dd++) L.u.kr[d][off + dd] += V[d * 3 + dd];
-          L.mr[slot] += V[9];
+            for (int u = 0; u < 5; u++) {
+              const int en = h * 5 + u;  // entry (d, dd) of the 3x3 block, row-major
+              if (en < 9) {
+                const int d = en / 3, dd = en - 3 * (en / 3);
+                const int64_t rb = (int64_t)c.rowbase + (int64_t)d * c.len + off + dd;
+                const double kv = acc[u], mv = d == dd ? acc[5] : 0.0;
+                Kv[rb] = kv;
+                Mv[rb] = mv;
+                if constexpr (WITH_C) Cv[rb] = alpha * kv + beta * mv;
+              }
+            }
+          } else if (h == 0) {
+            for (int d = 0; d < 3; d++)
+              for (int u = 0; u < wid; u++) {
+                const int64_t rb = (int64_t)c.rowbase + (int64_t)d * c.len + off + u;
+                Kv[rb] = 0.0;
+                Mv[rb] = 0.0;
+                if constexpr (WITH_C) Cv[rb] = 0.0;
+              }
+          }
         }
       }
-      wave_sync();
     }
-    // 4. coalesced row writes
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-      const int64_t base = rowptr[r0 + d];
-      for (int q = lane; q < len; q += 64) {
-        const int sl = L.colslot[q];
-        const int dd = q - L.off[sl];
-        const double kv = L.u.kr[d][q];
-        const double mv = (dd == d) ? L.mr[sl] : 0.0;
-        Kv[base + q] = kv;
-        Mv[base + q] = mv;
-        if (Cv) Cv[base + q] = alpha * kv + beta * mv;
-      }
+  };
+  auto none = [] {};
+  if constexpr (GENERAL) {
+    for (; t < end; t += wx) {
+      Hex8Fetch c{};
+      fetch1(c, t);
+      fetch2(c);
+      fetch3(c);
+      fetch4(c);
+      process(c, none, none, none);
+    }
+  } else {
+    // two nodes ahead: X = node t (all inputs in), Y = t + wx (levels 1-2 in,
+    // 3-4 issued during t), Z = t + 2 wx (levels 1-2 issued during t); the
+    // three records rotate, so nothing is copied while its loads are in flight
+    // (past the end of the range the last node is fetched again, unused:
+    // the loads stay unconditional, see fetch2)
+    Hex8Fetch A{}, B{}, C{};
+    fetch1(A, t);
+    fetch2(A);
+    fetch3(A);
+    fetch4(A);
+    fetch1(B, min(t + wx, end - 1));
+    fetch2(B);
+    auto step = [&](const Hex8Fetch &X, Hex8Fetch &Y, Hex8Fetch &Z) {
+      process(
+          X,
+          [&] {
+            fetch3(Y);
+            fetch1(Z, min(t + 2 * wx, end - 1));
+          },
+          [&] { fetch4(Y); }, [&] { fetch2(Z); });
+      t += wx;
+      return t < end;
+    };
+    while (step(A, B, C) && step(B, C, A) && step(C, A, B)) {
     }
   }
+  if (bad_elem != INT32_MAX) atomicMin(bad, bad_elem);
+}
+
+// persistent wavefronts for n descriptors: as many as can be resident, a
+// multiple of the 8 XCDs
+template <int ND, bool GENERAL, bool WITH_C>
+static void launch_hex8_part(Model &m, KindBlock &kb, int64_t first, int64_t n, int *bad, hipStream_t s) {
+  if (n == 0) return;
+  int dev = 0, cus = 0, per_cu = 0;
+  ELP_HIP(hipGetDevice(&dev));
+  ELP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  ELP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hex8_node_kernel<ND, GENERAL, WITH_C>, 64, 0));
+  const int64_t resident = (int64_t)std::max(cus, 8) * std::max(per_cu, 1);
+  const int grid = (int)(8 * std::max<int64_t>(1, std::min((n + 7) / 8, resident / 8)));
+  hipLaunchKernelGGL((hex8_node_kernel<ND, GENERAL, WITH_C>), dim3(grid), dim3(64), 0, s, (int32_t)n,
+                     kb.d_h8_desc.p + first * 8, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_adj_slot.p, kb.d_conn.p,
+                     kb.d_params.p, m.d_xyz.p, m.d_nbr_off.p, m.d_K.p, m.d_M.p,
+                     m.has_rayleigh ? m.d_C.p : nullptr, m.ray_a, m.ray_b, bad);
+  ELP_HIP(hipGetLastError());
+}
+
+template <int ND, bool WITH_C>
+static void launch_hex8_kind(Model &m, KindBlock &kb, int *bad, hipStream_t s) {
+  const int64_t nn = (int64_t)kb.nodes.size(), ns = kb.h8_simple;
+  launch_hex8_part<ND, false, WITH_C>(m, kb, 0, ns, bad, s);
+  launch_hex8_part<ND, true, WITH_C>(m, kb, ns, nn - ns, bad, s);
 }
 
 static void launch_hex8_node(Model &m, KindBlock &kb, int *bad, hipStream_t s) {
-  const int32_t nn = (int32_t)kb.nodes.size();
-  if (nn == 0) return;
-  const int grid = (nn + 7) / 8 * 8;  // one wavefront per node, a multiple of the 8 XCDs
-  double *Cv = m.has_rayleigh ? m.d_C.p : nullptr;
-#define ELP_HEX8_ARGS                                                                                   \
-  nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, kb.d_params.p,   \
-      m.d_xyz.p, m.d_nbr_ptr.p, m.d_nbr.p, m.d_nbr_off.p, m.d_node_row.p, m.d_rowptr.p, m.d_K.p,     \
-      m.d_M.p, Cv, m.ray_a, m.ray_b, bad
-  if (kb.kind == ELP_FLUID8)
-    hipLaunchKernelGGL(hex8_node_kernel<1>, dim3(grid), dim3(64), 0, s, ELP_HEX8_ARGS);
-  else
-    hipLaunchKernelGGL(hex8_node_kernel<3>, dim3(grid), dim3(64), 0, s, ELP_HEX8_ARGS);
-#undef ELP_HEX8_ARGS
-  ELP_HIP(hipGetLastError());
+  if (kb.kind == ELP_FLUID8) {
+    if (m.has_rayleigh) launch_hex8_kind<1, true>(m, kb, bad, s);
+    else launch_hex8_kind<1, false>(m, kb, bad, s);
+  } else {
+    if (m.has_rayleigh) launch_hex8_kind<3, true>(m, kb, bad, s);
+    else launch_hex8_kind<3, false>(m, kb, bad, s);
+  }
 }
 
 static bool fused_kind(const KindBlock &kb) { return kb.npe == 8; }

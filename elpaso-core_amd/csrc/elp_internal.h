@@ -126,8 +126,18 @@ struct KindBlock {
   std::vector<int32_t> adj_ptr;   // per entry of `nodes`: adjacent elements (kind-local, ascending id)
   std::vector<int32_t> adj_elem;
   std::vector<uint8_t> adj_local; // local node index in the element
-  DArr<int32_t> d_conn, d_nodes, d_adj_ptr, d_adj_elem;
-  DArr<uint8_t> d_adj_local;
+  // fused Hex8 kernel: per entry of `nodes` {first adjacency entry, adjacent
+  // elements, first neighbour, neighbours, first CSR entry of the node's
+  // first row, row length, 0, 0}; per adjacency entry the neighbour slot of
+  // each of the element's 8 nodes
+  // Nodes in at most 8 elements with at most 32 (3 DOF) or 64 (1 DOF)
+  // neighbour slots, all of them nodes of the node's own field, come first,
+  // h8_simple of them, then the others.
+  std::vector<int32_t> h8_desc;
+  std::vector<uint8_t> adj_slot;
+  int64_t h8_simple = 0;
+  DArr<int32_t> d_conn, d_nodes, d_adj_ptr, d_adj_elem, d_h8_desc;
+  DArr<uint8_t> d_adj_local, d_adj_slot;
   DArr<double> d_params;
   DArr<double> d_Ke, d_Me;        // element matrices (nd npe)^2 and npe^2 per element
 };

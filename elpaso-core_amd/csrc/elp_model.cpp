@@ -292,6 +292,49 @@ void finalize_model(Model &m) {
       }
     }
   });
+  // ---- records of the fused Hex8 kernel (see KindBlock::h8_desc)
+  for (auto &kb : m.kinds) {
+    if (kb.npe != 8) continue;
+    const int64_t ns = (int64_t)kb.nodes.size();
+    const int32_t slots_per_pass = kb.nd == 1 ? 64 : 32;
+    // simple: at most 8 elements, one pass of slots, every neighbour a node of
+    // the same field (no interface partner), so slot t is columns nd t.. of the rows
+    std::vector<char> simple(ns);
+    parallel_for(ns, [&](int64_t a, int64_t b) {
+      for (int64_t s = a; s < b; s++) {
+        const int32_t i = kb.nodes[s];
+        bool ok = kb.adj_ptr[s + 1] - kb.adj_ptr[s] <= 8 && m.nbr_ptr[i + 1] - m.nbr_ptr[i] <= slots_per_pass;
+        for (int32_t p = m.nbr_ptr[i]; ok && p < m.nbr_ptr[i + 1]; p++) ok = m.node_nd[m.nbr[p]] == kb.nd;
+        simple[s] = ok;
+      }
+    });
+    std::vector<int64_t> at(ns);  // descriptor position: simple nodes first, in node order
+    kb.h8_simple = 0;
+    for (int64_t s = 0; s < ns; s++)
+      if (simple[s]) at[s] = kb.h8_simple++;
+    for (int64_t s = 0, o = kb.h8_simple; s < ns; s++)
+      if (!simple[s]) at[s] = o++;
+    kb.h8_desc.assign(ns * 8, 0);
+    kb.adj_slot.assign(kb.adj_elem.size() * 8, 0);
+    parallel_for(ns, [&](int64_t a, int64_t b) {
+      for (int64_t s = a; s < b; s++) {
+        const int32_t i = kb.nodes[s], r0 = m.node_row[i];
+        int32_t *d = &kb.h8_desc[at[s] * 8];
+        d[0] = kb.adj_ptr[s];
+        d[1] = kb.adj_ptr[s + 1] - kb.adj_ptr[s];
+        d[2] = m.nbr_ptr[i];
+        d[3] = m.nbr_ptr[i + 1] - m.nbr_ptr[i];
+        d[4] = m.rowptr[r0];
+        d[5] = m.rowptr[r0 + 1] - m.rowptr[r0];
+        const int32_t *nb = m.nbr.data() + m.nbr_ptr[i];
+        for (int32_t p = d[0]; p < d[0] + d[1]; p++)
+          for (int q = 0; q < 8; q++) {
+            const int32_t j = kb.conn[(int64_t)kb.adj_elem[p] * 8 + q];
+            kb.adj_slot[(int64_t)p * 8 + q] = (uint8_t)(std::lower_bound(nb, nb + d[3], j) - nb);
+          }
+      }
+    });
+  }
   auto full_pos = [&](int32_t r, int32_t c) -> int32_t {
     auto b = m.colind.begin() + m.rowptr[r], e = m.colind.begin() + m.rowptr[r + 1];
     auto it = std::lower_bound(b, e, c);
@@ -424,6 +467,8 @@ void finalize_model(Model &m) {
     kb.d_adj_ptr.upload(c, kb.adj_ptr);
     kb.d_adj_elem.upload(c, kb.adj_elem);
     kb.d_adj_local.upload(c, kb.adj_local);
+    kb.d_h8_desc.upload(c, kb.h8_desc);
+    kb.d_adj_slot.upload(c, kb.adj_slot);
     kb.d_params.upload(c, kb.params);
   }
   m.d_x_rowptr.upload(c, m.x_rowptr);

@@ -75,6 +75,26 @@ def test_coupled_small(ctx):
     _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_S))
 
 
+def _doubled(m):
+    """Every volume element twice (second copy with new ids): interior nodes
+    then lie in 16 Hex8 elements, two batches of the fused kernel."""
+    g0 = m.groups[0]
+    top = int(np.max(g0.elem_ids)) + 1
+    m.groups.append(MG.ElementGroup(g0.kind, g0.material_id, np.asarray(g0.elem_ids) + top, np.array(g0.conn)))
+    m.name += "-doubled"
+    return m
+
+
+@pytest.mark.parametrize("kind", ["brick8", "fluid8"])
+def test_hex8_nodes_in_more_than_8_elements(ctx, kind):
+    if kind == "brick8":
+        m = _doubled(MG.steel_block(6, 5, 3, L=(0.03, 0.025, 0.015), jitter=0.1, seed=7))
+        _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_C))
+    else:
+        m = _doubled(MG.acoustic_cavity(5, 4, 3))
+        _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_G))
+
+
 def test_shuffled_ids_and_groups(ctx):
     """Node/element ids out of order and split groups: numbering follows ids."""
     m = MG.acoustic_cavity(4, 3, 3)
