@@ -463,21 +463,19 @@ struct Hex8Lds {
   // 3 DOF: half 0 = K00 K01 K02 K10 K11 M, half 1 = K12 K20 K21 K22 0 M;
   // 1 DOF: K, M
   static constexpr int kRec = ND == 1 ? 3 : 13, kHalf = ND == 1 ? 2 : 6;
+  static constexpr int kRow = 25;  // per (element k, point g): 3 x 8 derivatives and w detJ
   union {
-    double xe[3][64];      // phase 0-1: coordinate c of node q of element k, [c][k*8+q]
-    double gr[3][8 * 72];  // phase 1-2: derivative c of N_q at point g of element k,
-                           // [c][k*72 + g*9 + q] (odd point stride: no bank conflicts);
-                           // the ninth double of each point row holds w detJ (c = 0) and
-                           // the slot table (c = 2)
-    double v[64][kRec];    // phase 3: the block of lane (k, b) (below gr[2])
+    double xe[3][64];       // phase 0-1: coordinate c of node q of element k, [c][k*8+q]
+    double gr[64 * kRow];   // phase 1-2: derivative c of N_q at point g of element k at
+                            // [(k*8+g)*25 + c*8 + q], w detJ at [(k*8+g)*25 + 24]; the odd
+                            // row stride keeps the lanes' rows in distinct LDS banks
+    double v[64][kRec];     // phase 3: the block of lane (k, b)
+    double rows[ND * 96 + 32];  // phase 4: image of the node's rows and the mass per slot
   } u;
-  __device__ double &wd(int k, int g) { return u.gr[0][k * 72 + g * 9 + 8]; }
-  // slot sp of the pass: byte k = b of batch element k touching it, 0xFF if none
-  __device__ uint64_t &cb(int sp) {
-    return *reinterpret_cast<uint64_t *>(&u.gr[2][(sp >> 3) * 72 + (sp & 7) * 9 + 8]);
-  }
+  uint64_t slots[64];  // slot sp of the pass: byte k = b of batch element k touching it, 0xFF if none
+  __device__ double *point(int k, int g) { return u.gr + (k * 8 + g) * kRow; }
+  __device__ uint64_t &cb(int sp) { return slots[sp]; }
 };
-static_assert(sizeof(double) * 64 * 13 <= sizeof(double) * 2 * 8 * 72, "blocks overlap the slot table");
 
 // The chain of one node's inputs that is loaded ahead (lane (k, j) holds
 // element k's share); what the node needs only from its phase 2 on (local
@@ -628,17 +626,17 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
         }
         wave_sync();  // coordinates read: the gradients overwrite them
         if (k < nk) {
-          const int row = k * 72 + j * 9;
+          double *row = L.point(k, j);
 #pragma unroll
           for (int q = 0; q < 8; q++) {
             const double dx = (h8_sx(q) ? 1.0 : -1.0) * yz[h8_sy(q)][h8_sz(q)];
             const double dy = (h8_sy(q) ? 1.0 : -1.0) * xz[h8_sx(q)][h8_sz(q)];
             const double dz = (h8_sz(q) ? 1.0 : -1.0) * xy[h8_sx(q)][h8_sy(q)];
-            L.u.gr[0][row + q] = G[0] * dx + G[1] * dy + G[2] * dz;
-            L.u.gr[1][row + q] = G[3] * dx + G[4] * dy + G[5] * dz;
-            L.u.gr[2][row + q] = G[6] * dx + G[7] * dy + G[8] * dz;
+            row[q] = G[0] * dx + G[1] * dy + G[2] * dz;
+            row[8 + q] = G[3] * dx + G[4] * dy + G[5] * dz;
+            row[16 + q] = G[6] * dx + G[7] * dy + G[8] * dz;
           }
-          L.wd(k, j) = wdj;
+          row[24] = wdj;
         }
         if (first) after_geometry();
         wave_sync();
@@ -651,13 +649,13 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
           const double pz[2] = {h8_f(h8_sz(a), 0) * h8_f(h8_sz(j), 0) * 0.015625,
                                 h8_f(h8_sz(a), 1) * h8_f(h8_sz(j), 1) * 0.015625};
           const double pxy[4] = {px0 * py0, px1 * py0, px0 * py1, px1 * py1};
-          const double *gx = L.u.gr[0] + k * 72, *gy = L.u.gr[1] + k * 72, *gz = L.u.gr[2] + k * 72;
           double P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, mm = 0.0, kk = 0.0;
 #pragma unroll
           for (int gg = 0; gg < 8; gg++) {
-            const double ax = gx[gg * 9 + a], ay = gy[gg * 9 + a], az = gz[gg * 9 + a];
-            const double bx = gx[gg * 9 + j], by = gy[gg * 9 + j], bz = gz[gg * 9 + j];
-            const double wd = L.wd(k, gg);
+            const double *row = L.point(k, gg);
+            const double ax = row[a], ay = row[8 + a], az = row[16 + a];
+            const double bx = row[j], by = row[8 + j], bz = row[16 + j];
+            const double wd = row[24];
             const double w0 = wd * ax, w1 = wd * ay, w2 = wd * az;
             mm += (pxy[gg & 3] * pz[gg >> 2]) * wd;
             if constexpr (ND == 1) {
@@ -719,8 +717,8 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
       //    the same value) so that no store sits under a branch.
       if constexpr (!GENERAL) {
         constexpr int kCap = ND == 1 ? 64 : 96;  // row length of a node with one pass of slots
-        double *kr = L.u.gr[0];                  // [d * kCap + column]
-        double *mr = L.u.gr[0] + ND * kCap;      // mass per slot
+        double *kr = L.u.rows;                   // [d * kCap + column]
+        double *mr = L.u.rows + ND * kCap;       // mass per slot
         wave_sync();  // blocks and slot table read
         if (s < c.deg) {
           if constexpr (ND == 1) {
@@ -832,8 +830,9 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
   if (bad_elem != INT32_MAX) atomicMin(bad, bad_elem);
 }
 
-// persistent wavefronts for n descriptors: as many as can be resident, a
-// multiple of the 8 XCDs
+// persistent wavefronts for n descriptors: one per CU fewer than can be
+// resident (a wave that does not find a free slot would run its whole share
+// after the others), a multiple of the 8 XCDs
 template <int ND, bool GENERAL, bool WITH_C>
 static void launch_hex8_part(Model &m, KindBlock &kb, int64_t first, int64_t n, int *bad, hipStream_t s) {
   if (n == 0) return;
@@ -841,7 +840,7 @@ static void launch_hex8_part(Model &m, KindBlock &kb, int64_t first, int64_t n, 
   ELP_HIP(hipGetDevice(&dev));
   ELP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   ELP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hex8_node_kernel<ND, GENERAL, WITH_C>, 64, 0));
-  const int64_t resident = (int64_t)std::max(cus, 8) * std::max(per_cu, 1);
+  const int64_t resident = (int64_t)std::max(cus, 8) * std::max(per_cu - 1, 1);
   const int grid = (int)(8 * std::max<int64_t>(1, std::min((n + 7) / 8, resident / 8)));
   hipLaunchKernelGGL((hex8_node_kernel<ND, GENERAL, WITH_C>), dim3(grid), dim3(64), 0, s, (int32_t)n,
                      kb.d_h8_desc.p + first * 8, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_adj_slot.p, kb.d_conn.p,
