@@ -456,7 +456,6 @@ __device__ inline double h8_f(int s, int ix) { return s != ix ? kH8Hi : kH8Lo; }
 // hides behind its arithmetic, and runs one batch and one pass.  The others
 // (GENERAL) load their inputs in place and run the batches once per group of
 // slots.
-constexpr int kMaxHex8Cols = 128;  // row length / neighbours of a Hex8 node (checked at finalize)
 template <int ND>
 struct Hex8Lds {
   // lane (k, b)'s block at an odd stride of doubles (no LDS bank conflicts):
@@ -836,10 +835,14 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
 template <int ND, bool GENERAL, bool WITH_C>
 static void launch_hex8_part(Model &m, KindBlock &kb, int64_t first, int64_t n, int *bad, hipStream_t s) {
   if (n == 0) return;
-  int dev = 0, cus = 0, per_cu = 0;
+  int dev = 0, cus = 0;
   ELP_HIP(hipGetDevice(&dev));
   ELP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  ELP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hex8_node_kernel<ND, GENERAL, WITH_C>, 64, 0));
+  static const int per_cu = [] {  // resident waves of this instantiation per CU (the same on every gfx950)
+    int v = 0;
+    ELP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, hex8_node_kernel<ND, GENERAL, WITH_C>, 64, 0));
+    return v;
+  }();
   const int64_t resident = (int64_t)std::max(cus, 8) * std::max(per_cu - 1, 1);
   const int grid = (int)(8 * std::max<int64_t>(1, std::min((n + 7) / 8, resident / 8)));
   hipLaunchKernelGGL((hex8_node_kernel<ND, GENERAL, WITH_C>), dim3(grid), dim3(64), 0, s, (int32_t)n,

@@ -253,13 +253,10 @@ void finalize_model(Model &m) {
     rowlen[r] = off;
     std::vector<int32_t>().swap(lists[r]);
   }
-  // max row length the gather kernel supports (LDS accumulators)
-  for (int64_t r = 0; r < N; r++) {
+  // max row length the gather kernel supports (LDS accumulators); the fused
+  // Hex8 kernel's per-adjacency slot bytes need at most 256 neighbours too
+  for (int64_t r = 0; r < N; r++)
     ELP_REQUIRE(rowlen[r] <= 256 && deg[r] <= 256, ELP_EINVAL, "node with more than 256 coupled columns");
-    // the fused Hex8 kernel keeps a node's rows in LDS
-    if (node_kind[r] >= 0 && m.kinds[node_kind[r]].npe == 8)
-      ELP_REQUIRE(rowlen[r] <= 128 && deg[r] <= 128, ELP_EINVAL, "Hex8 node with more than 128 coupled columns");
-  }
 
   // ---- CSR
   m.rowptr.assign(m.n + 1, 0);
