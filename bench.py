@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--fp64-sample", type=int, default=4, help="frequencies timed with the FP64 solver")
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true", help="skip re-measuring copy bandwidth and FP64 GEMM")
     ap.add_argument("--no-config0", action="store_true",
                     help="skip the 20x16x12 Fluid8 cavity leg (GPU sweep vs the CPU oracle's direct solves)")
     ap.add_argument("--cpu-procs", type=int, default=16,
@@ -179,6 +180,12 @@ def main():
     if args.copy_calibration:
         copy_calibration()
 
+    peaks = None
+    if not args.no_peaks:
+        peaks = measure_peaks()
+        peaks.update({"published_hbm_GBs": HBM_PEAK_GBS, "published_fp64_TFLOPs": 78.6})
+        log(f"[bench] measured peaks: {peaks}")
+
     serving = None
     if args.serve_requests > 0:
         serving = serve_stream(ctx, args.serve_requests)
@@ -207,7 +214,7 @@ def main():
         "iters_mean": float(iters.mean()), "iters_max": int(iters.max()),
         "sweep_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "fp64_baseline": fp64, "roofline": roof, "assembly": assembly, "serving": serving,
-        "cpu_baseline": cpu, "config0": config0,
+        "cpu_baseline": cpu, "config0": config0, "measured_peaks": peaks,
     }
     print(json.dumps(line), flush=True)
 
@@ -229,6 +236,69 @@ def copy_calibration(nbytes=1 << 30, repeats=3):
     hip.hipFree(a)
     hip.hipFree(b)
     log(f"[bench] copy calibration: {repeats} x {nbytes} B device-to-device")
+
+
+def measure_peaks(copy_bytes=1 << 30, n=8192, repeats=5):
+    """The GPU's attainable peaks re-measured on this box: device-to-device
+    copy of 1 GiB (HBM read + write bytes per second) and rocBLAS FP64 GEMM
+    (n x n x n).  Reported beside the published peaks the rooflines use."""
+    import ctypes
+    C = ctypes
+    hip = C.CDLL("libamdhip64.so")
+    out = {}
+    a, b = C.c_void_p(), C.c_void_p()
+    try:
+        for ptr in (a, b):
+            if hip.hipMalloc(C.byref(ptr), C.c_size_t(copy_bytes)) != 0:
+                raise RuntimeError("hipMalloc failed")
+        hip.hipMemset(a, 1, C.c_size_t(copy_bytes))
+        hip.hipMemcpy(b, a, C.c_size_t(copy_bytes), 3)
+        hip.hipDeviceSynchronize()
+        t = time.perf_counter()
+        for _ in range(repeats):
+            hip.hipMemcpy(b, a, C.c_size_t(copy_bytes), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t) / repeats
+        out["hbm_copy_GBs"] = round(2 * copy_bytes / dt / 1e9, 1)
+    finally:
+        for ptr in (a, b):
+            if ptr.value:
+                hip.hipFree(ptr)
+    try:
+        rb = C.CDLL("librocblas.so")
+    except OSError:
+        return out
+    h = C.c_void_p()
+    if rb.rocblas_create_handle(C.byref(h)) != 0:
+        return out
+    bufs = [C.c_void_p() for _ in range(3)]
+    try:
+        nb = n * n * 8
+        for ptr in bufs:
+            if hip.hipMalloc(C.byref(ptr), C.c_size_t(nb)) != 0:
+                raise RuntimeError("hipMalloc failed")
+            hip.hipMemset(ptr, 0x3F, C.c_size_t(nb))  # every double 4.8e-4
+        one, zero = C.c_double(1.0), C.c_double(0.0)
+        rb.rocblas_dgemm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.c_int]
+        args = (h, 111, 111, n, n, n, C.byref(one), bufs[0], n, bufs[1], n, C.byref(zero), bufs[2], n)
+        if rb.rocblas_dgemm(*args) != 0:
+            return out
+        hip.hipDeviceSynchronize()
+        t = time.perf_counter()
+        for _ in range(repeats):
+            rb.rocblas_dgemm(*args)
+        hip.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t) / repeats
+        out["fp64_dgemm_TFLOPs"] = round(2.0 * n ** 3 / dt / 1e12, 1)
+        out["dgemm_n"] = n
+    finally:
+        for ptr in bufs:
+            if ptr.value:
+                hip.hipFree(ptr)
+        rb.rocblas_destroy_handle(h)
+    return out
 
 
 def serve_stream(ctx, n_req, rate=20.0, seed=0):
