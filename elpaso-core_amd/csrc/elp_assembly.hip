@@ -444,18 +444,19 @@ __device__ inline double h8_f(int s, int ix) { return s != ix ? kH8Hi : kH8Lo; }
 //  3. the blocks go to LDS and the lanes owning a neighbour slot (two lanes
 //     per slot for 3 DOF, one per half of the block) add them into registers
 //     in ascending element id, the order of elPaSo's MatSetValues.
-// After the last batch the slot lanes write their entries of K, M (and C)
-// straight into the node's CSR rows.  No atomics on values: repeated runs are
-// bit-identical.
+// After the last batch the node's K, M (and C) entries go to its CSR rows:
+// on the common path through an LDS image of the rows and coalesced stores,
+// on the general path straight from the slot lanes.  No atomics on values:
+// repeated runs are bit-identical.
 //
 // The inputs of a node sit four dependent loads deep (descriptor ->
 // adjacency -> connectivity -> coordinates).  For the nodes in at most 8
-// elements with at most one pass of slots (32 for 3 DOF, 64 for 1: all nodes
-// of a structured mesh away from interfaces) the wavefront issues those of
-// its next node between the phases of the current one, so their latency
-// hides behind its arithmetic, and runs one batch and one pass.  The others
-// (GENERAL) load their inputs in place and run the batches once per group of
-// slots.
+// elements with at most one pass of slots (32 for 3 DOF, 64 for 1) and no
+// neighbour of another field (all nodes of a structured mesh away from
+// interfaces) the wavefront issues those of its next two nodes between the
+// phases of the current one, so their latency hides behind its arithmetic,
+// and runs one batch and one pass.  The others (GENERAL) load their inputs
+// in place and run the batches once per group of slots.
 template <int ND>
 struct Hex8Lds {
   // lane (k, b)'s block at an odd stride of doubles (no LDS bank conflicts):
