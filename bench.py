@@ -160,20 +160,6 @@ def main():
                     "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "kernel": "hex8_node_kernel<3, false, true>",
                     "traffic": PMC_TRAFFIC.get(("hex8_node_kernel<3, false, true>", args.assembly_nel))}
-        if not args.no_cpu:
-            # the oracle's K + M + C assembly of the same block on one core
-            # (element integrals in C, scatter in numpy), symbolic pattern apart
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import oracle as O
-            t = time.perf_counter()
-            snum = O.Numbering(sm)
-            spat = O.pattern(sm, snum)
-            t_sym = time.perf_counter() - t
-            t = time.perf_counter()
-            O.assemble_km(sm, snum, spat)
-            t_asm = time.perf_counter() - t
-            assembly["cpu_oracle_1core_s"] = {"assembly": round(t_asm, 2), "symbolic": round(t_sym, 2)}
-            assembly["gpu_vs_cpu_1core"] = round(t_asm / (ms * 1e-3), 0)
         log(f"[bench] assembly: {assembly}")
         sg.close()
 
@@ -191,14 +177,20 @@ def main():
         serving = serve_stream(ctx, args.serve_requests)
         log(f"[bench] serving: {serving}")
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(model, freqs)
-
     config0 = None
     if not args.no_config0:
-        config0 = config0_leg(ctx, MG, run_cpu=not args.no_cpu, procs=args.cpu_procs)
+        config0 = config0_leg(ctx, MG)
         log(f"[bench] config 0: {config0}")
+
+    cpu = None
+    if not args.no_cpu:
+        steel = MG.steel_block(*(int(v) for v in args.assembly_nel.split(","))) if assembly else None
+        cpu = cpu_baseline(model, freqs, steel=steel, config0=config0 is not None, procs=args.cpu_procs)
+        if assembly:
+            assembly["gpu_vs_cpu_1core"] = round(cpu["brick8_assembly_1core_s"]["assembly"] / (assembly["ms"] * 1e-3))
+        if config0:
+            config0["gpu_vs_cpu_parallel"] = round(config0["gpu_solves_per_s"] /
+                                                   cpu["config0"]["cpu_parallel_solves_per_s"], 1)
 
     line = {
         "metric": "freq-point solves/s at 1e-8 rel residual, mixed vs FP64; p99 sweep latency",
@@ -366,14 +358,11 @@ def _oracle_direct(args):
     return time.perf_counter() - t
 
 
-def config0_leg(ctx, MG, run_cpu=True, procs=16):
-    """The reference's own CPU-runnable case: Fluid8 cavity 20x16x12 (4,641
-    DOF), Robin face z = 0.6 with Z = rho c (1 + 0.5i), 100 frequencies
-    10-500 Hz.  GPU: one sweep on 32 slots (solutions stay on the device).
-    CPU: the oracle's restatement of elPaSo's per-frequency loop
-    (re-integration + scatter + BCs + SuperLU, standing in for MUMPS) on one
-    core for a sample of 10 frequencies, and over all 100 frequencies split
-    into contiguous shares across P processes."""
+def config0_leg(ctx, MG):
+    """The reference's own CPU-runnable case on the GPU: Fluid8 cavity
+    20x16x12 (4,641 DOF), Robin face z = 0.6 with Z = rho c (1 + 0.5i), 100
+    frequencies 10-500 Hz, one sweep on 32 slots (solutions stay on the
+    device).  Its CPU counterpart is timed in the cpu_baseline leg."""
     from elpaso_amd import GpuModel
     m = MG.acoustic_cavity()
     freqs = MG.linear_frequencies(10.0, 500.0, 100)
@@ -384,24 +373,33 @@ def config0_leg(ctx, MG, run_cpu=True, procs=16):
     _, rel, it, _ = g.sweep(freqs, want_x=False)
     dt = time.perf_counter() - t
     g.close()
-    out = {"workload": "acoustic cavity 1.0x0.8x0.6 m, Fluid8 20x16x12 (4,641 DOF), 100 freqs 10-500 Hz",
-           "gpu_solves_per_s": round(len(freqs) / dt, 1), "gpu_sweep_ms": round(1e3 * dt, 2),
-           "gpu_relres_max": float(rel.max()), "gpu_iters_max": int(it.max())}
-    if not run_cpu:
-        return out
+    return {"workload": "acoustic cavity 1.0x0.8x0.6 m, Fluid8 20x16x12 (4,641 DOF), 100 freqs 10-500 Hz",
+            "gpu_solves_per_s": round(len(freqs) / dt, 1), "gpu_sweep_ms": round(1e3 * dt, 2),
+            "gpu_relres_max": float(rel.max()), "gpu_iters_max": int(it.max())}
+
+
+def _cpu_config0(procs):
+    """CPU side of the reference case: the oracle's restatement of elPaSo's
+    per-frequency loop (re-integration + scatter + BCs + SuperLU, standing in
+    for MUMPS) on one core for 10 of the frequencies, and over all 100
+    frequencies split into contiguous shares across P processes."""
+    sys.path.insert(0, os.path.join(ROOT, "elpaso-core_amd"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    from elpaso_amd import meshgen as MG
+    from threadpoolctl import threadpool_limits
+    m = MG.acoustic_cavity()
+    freqs = MG.linear_frequencies(10.0, 500.0, 100)
     num = O.Numbering(m)
     pat = O.pattern(m, num)
-    from threadpoolctl import threadpool_limits
     tm = []
     with threadpool_limits(limits=1):  # one core: SuperLU's BLAS calls single-threaded
         t = time.perf_counter()
         O.solve_direct(m, freqs[::10], num, pat, timings=tm)
         dt1 = time.perf_counter() - t
-    out["cpu_1core_solves_per_s"] = round(10 / dt1, 3)
-    out["cpu_1core_ms_per_freq"] = {"assembly": round(1e3 * float(np.mean([a for a, _ in tm])), 1),
-                                    "lu_solve": round(1e3 * float(np.mean([b for _, b in tm])), 1)}
+    out = {"cpu_1core_solves_per_s": round(10 / dt1, 3),
+           "cpu_1core_ms_per_freq": {"assembly": round(1e3 * float(np.mean([a for a, _ in tm])), 1),
+                                     "lu_solve": round(1e3 * float(np.mean([b for _, b in tm])), 1)}}
     p = max(1, min(procs, os.cpu_count() or 1))
     import multiprocessing as mp
     shares = [freqs[i * len(freqs) // p:(i + 1) * len(freqs) // p] for i in range(p)]
@@ -423,18 +421,19 @@ def config0_leg(ctx, MG, run_cpu=True, procs=16):
                 os.environ[k] = v
     out["cpu_procs"] = p
     out["cpu_parallel_solves_per_s"] = round(len(freqs) / dtp, 2)
-    out["gpu_vs_cpu_parallel"] = round(out["gpu_solves_per_s"] / out["cpu_parallel_solves_per_s"], 1)
-    out["cpu_sample"] = ("1 core: 10 of the frequencies; parallel: all 100, contiguous shares per process "
-                         "(process start-up and per-process symbolic set-up included)")
+    out["sample"] = ("1 core: 10 of the 100 frequencies; parallel: all 100, contiguous shares per process "
+                     "(process start-up and per-process symbolic set-up included)")
     return out
 
 
-def cpu_baseline(model, freqs):
+def cpu_baseline(model, freqs, steel=None, config0=False, procs=16):
     """elPaSo's per-frequency flow restated on the CPU (oracle), one core:
     re-integrate every element at w, scatter into CSR (SetupKM).  The direct
     LU solve (MUMPS in elPaSo, SuperLU here) of the 1.98M-DOF Hex20 system
     does not fit a bounded sample, so the value is an upper bound on CPU
-    solves/s (assembly time only)."""
+    solves/s (assembly time only).  Also timed here: the oracle's K + M + C
+    assembly of the Brick8 block (one core; element integrals in C, scatter
+    in numpy) and the reference case's full per-frequency loop."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     t = time.perf_counter()
@@ -445,12 +444,25 @@ def cpu_baseline(model, freqs):
     O.dynamic_matrix(model, 2 * np.pi * float(freqs[len(freqs) // 2]), num, pat)
     t_asm = time.perf_counter() - t
     log(f"[bench] cpu oracle: symbolic {t_sym:.1f}s, one-frequency assembly {t_asm:.1f}s")
-    return {"value": round(1.0 / t_asm, 5), "unit": "freq-point solves/s (upper bound: assembly only)",
-            "cores": 1, "kind": "port",
-            "sample": f"1 frequency ({float(freqs[len(freqs) // 2]):.1f} Hz) of the same model: elPaSo's "
-                      f"per-frequency element re-integration + CSR scatter + BCs ({t_asm:.1f} s); the "
-                      f"direct LU of this system is beyond a bounded sample and is not included"}
-
+    out = {"value": round(1.0 / t_asm, 5), "unit": "freq-point solves/s (upper bound: assembly only)",
+           "cores": 1, "kind": "port",
+           "sample": f"1 frequency ({float(freqs[len(freqs) // 2]):.1f} Hz) of the same model: elPaSo's "
+                     f"per-frequency element re-integration + CSR scatter + BCs ({t_asm:.1f} s); the "
+                     f"direct LU of this system is beyond a bounded sample and is not included"}
+    if steel is not None:
+        t = time.perf_counter()
+        snum = O.Numbering(steel)
+        spat = O.pattern(steel, snum)
+        t_ssym = time.perf_counter() - t
+        t = time.perf_counter()
+        O.assemble_km(steel, snum, spat)
+        out["brick8_assembly_1core_s"] = {"assembly": round(time.perf_counter() - t, 2),
+                                          "symbolic": round(t_ssym, 2)}
+        log(f"[bench] cpu oracle Brick8 assembly: {out['brick8_assembly_1core_s']}")
+    if config0:
+        out["config0"] = _cpu_config0(procs)
+        log(f"[bench] cpu oracle config 0: {out['config0']}")
+    return out
 
 if __name__ == "__main__":
     main()
