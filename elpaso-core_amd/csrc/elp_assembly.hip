@@ -91,6 +91,22 @@ __device__ inline bool point_geometry(ElemLds<NPE, NG> &L, int g) {
 
 constexpr int kWaves = 4;  // wavefronts (elements / nodes) per workgroup
 
+// Entry t of the upper triangle (a <= b) of a symmetric N x N matrix, N even:
+// rows p and N-1-p form one run of N+1 entries, so t decodes with one
+// division by a constant.
+template <int N>
+__device__ inline void tri_entry(int t, int &a, int &b) {
+  static_assert(N % 2 == 0, "paired rows need an even order");
+  const int p = t / (N + 1), r = t - p * (N + 1);
+  if (r < N - p) {
+    a = p;
+    b = p + r;
+  } else {
+    a = N - 1 - p;
+    b = a + (r - (N - p));
+  }
+}
+
 // Element matrices of fluid elements: K_ab = sum_g wdJ grad_a.grad_b / rho,
 // M_ab = sum_g (1/c^2) N_a N_b wdJ / rho  (elementfluid3d.cpp:54-167, with the
 // frequency-independent 1/rho of elPaSo's 1/(rho w^2) scaling, elementfluid.cpp:74-76).
@@ -98,11 +114,16 @@ constexpr int kWaves = 4;  // wavefronts (elements / nodes) per workgroup
 // lambda p_d q_e + mu p_e q_d + mu delta_de p.q, M_ab = rho sum_g N_a N_b wdJ
 // (elementstructurebrick.cpp:470-554).  params: fluid (rho, 1/c^2), brick
 // (lambda, mu, rho).  bad receives the smallest element index with detJ <= 0.
+// K and M are symmetric: each lane computes entries of the upper triangle and
+// writes them to both halves.
 template <int NPE, int ND, int NG>
 __global__ __launch_bounds__(64 * kWaves) void element_kernel(
     const double *__restrict__ xyz, const int32_t *__restrict__ conn, const double *__restrict__ params,
     int64_t n_elem, double *__restrict__ Ke, double *__restrict__ Me, int *bad, int xyz_direct) {
   __shared__ ElemLds<NPE, NG> lds[kWaves];
+  __shared__ double sN[NG][NPE];  // shape values at the points, shared by the workgroup's elements
+  for (int i = threadIdx.x; i < NG * NPE; i += blockDim.x) sN[i / NPE][i % NPE] = tab_n<NPE>(i / NPE, 0)[i % NPE];
+  __syncthreads();
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int64_t e = (int64_t)blockIdx.x * kWaves + w;
   if (e >= n_elem) return;  // whole wavefront leaves together
@@ -125,8 +146,9 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
   double *Kout = Ke + e * (int64_t)NDF * NDF;
   double *Mout = Me + e * (int64_t)NPE * NPE;
   if constexpr (ND == 1) {
-    for (int t = lane; t < NPE * NPE; t += 64) {
-      const int a = t / NPE, b = t % NPE;
+    for (int t = lane; t < NPE * (NPE + 1) / 2; t += 64) {
+      int a, b;
+      tri_entry<NPE>(t, a, b);
       double k = 0.0, m = 0.0;
 #pragma unroll 3
       for (int g = 0; g < NG; g++) {
@@ -134,16 +156,16 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
         double r = L.grad[g][0][a] * L.grad[g][0][b] + L.grad[g][1][a] * L.grad[g][1][b] +
                    L.grad[g][2][a] * L.grad[g][2][b];
         k += r * wd;
-        const double *N = tab_n<NPE>(g, 0);
-        m += p1 * N[a] * N[b] * wd;
+        m += p1 * sN[g][a] * sN[g][b] * wd;
       }
-      Kout[t] = k / p0;
-      Mout[t] = m / p0;
+      Kout[a * NPE + b] = Kout[b * NPE + a] = k / p0;
+      Mout[a * NPE + b] = Mout[b * NPE + a] = m / p0;
     }
   } else {
     const double lam = p0, mu = p1, rho = p2;
-    for (int t = lane; t < NDF * NDF; t += 64) {
-      const int row = t / NDF, col = t % NDF;
+    for (int t = lane; t < NDF * (NDF + 1) / 2; t += 64) {
+      int row, col;
+      tri_entry<NDF>(t, row, col);
       const int a = row / 3, d = row % 3, b = col / 3, dd = col % 3;
       double k = 0.0;
 #pragma unroll 3
@@ -156,17 +178,15 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
                      L.grad[g][2][a] * L.grad[g][2][b]);
         k += v * L.wdj[g];
       }
-      Kout[t] = k;
+      Kout[row * NDF + col] = Kout[col * NDF + row] = k;
     }
-    for (int t = lane; t < NPE * NPE; t += 64) {
-      const int a = t / NPE, b = t % NPE;
+    for (int t = lane; t < NPE * (NPE + 1) / 2; t += 64) {
+      int a, b;
+      tri_entry<NPE>(t, a, b);
       double m = 0.0;
 #pragma unroll 3
-      for (int g = 0; g < NG; g++) {
-        const double *N = tab_n<NPE>(g, 0);
-        m += rho * N[a] * N[b] * L.wdj[g];
-      }
-      Mout[t] = m;
+      for (int g = 0; g < NG; g++) m += rho * sN[g][a] * sN[g][b] * L.wdj[g];
+      Mout[a * NPE + b] = Mout[b * NPE + a] = m;
     }
   }
 }
