@@ -737,6 +737,7 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
       //    the same value) so that no store sits under a branch.
       if constexpr (!GENERAL) {
         constexpr int kCap = ND == 1 ? 64 : 96;  // row length of a node with one pass of slots
+        static_assert(kCap <= 128, "two columns per lane");
         double *kr = L.u.rows;                   // [d * kCap + column]
         double *mr = L.u.rows + ND * kCap;       // mass per slot
         wave_sync();  // blocks and slot table read
@@ -754,18 +755,20 @@ __global__ __launch_bounds__(64) void hex8_node_kernel(
           }
         }
         wave_sync();
+        // two columns per lane (one 16-byte store each; rows hold at most
+        // 2 x 64 entries here); lanes past the row end repeat its last pair
+        const int c0 = min(2 * lane, c.len - 2);
 #pragma unroll
-        for (int d = 0; d < ND; d++)
-#pragma unroll
-          for (int it = 0; it < kCap / 64 + (kCap % 64 ? 1 : 0); it++) {
-            const int col = min(it * 64 + lane, c.len - 1);
-            const double kv = kr[d * kCap + col];
-            const double mv = ND == 1 ? mr[col] : (col % 3 == d ? mr[col / 3] : 0.0);
-            const int64_t rb = (int64_t)c.rowbase + (int64_t)d * c.len + col;
-            Kv[rb] = kv;
-            Mv[rb] = mv;
-            if constexpr (WITH_C) Cv[rb] = alpha * kv + beta * mv;
-          }
+        for (int d = 0; d < ND; d++) {
+          const double k0 = kr[d * kCap + c0], k1 = kr[d * kCap + c0 + 1];
+          const double m0 = ND == 1 ? mr[c0] : (c0 % 3 == d ? mr[c0 / 3] : 0.0);
+          const double m1 = ND == 1 ? mr[c0 + 1] : ((c0 + 1) % 3 == d ? mr[(c0 + 1) / 3] : 0.0);
+          const int64_t rb = (int64_t)c.rowbase + (int64_t)d * c.len + c0;
+          *reinterpret_cast<double2 *>(Kv + rb) = make_double2(k0, k1);
+          *reinterpret_cast<double2 *>(Mv + rb) = make_double2(m0, m1);
+          if constexpr (WITH_C)
+            *reinterpret_cast<double2 *>(Cv + rb) = make_double2(alpha * k0 + beta * m0, alpha * k1 + beta * m1);
+        }
       } else if (s < c.deg) {
         // GENERAL: the slot lanes' entries straight into the rows; a slot that
         // no element touches (an interface partner of another field) holds zeros
