@@ -192,14 +192,23 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
 }
 
 // Deterministic gather of element rows into the node's CSR rows (K, M and
-// optionally C = alpha K + beta M), one wavefront per node.
+// optionally C = alpha K + beta M), one wavefront per node.  The node's
+// adjacent elements are taken in batches: the lanes load every (element,
+// neighbour) item of a batch at once (its slot in the node's rows, its block
+// of the element matrices), then the blocks are added into the rows in LDS
+// element by element, in ascending element id.
 constexpr int kMaxCols = 256;
 template <int NPE, int ND>
 struct GatherLds {
+  static constexpr int BK = ND == 1 ? 8 : 4;  // elements per batch
+  static constexpr int NI = BK * NPE;         // (element, neighbour) items per batch
   double k[ND][kMaxCols];
   double m[kMaxCols];
   int32_t nb[kMaxCols];
   int32_t off[kMaxCols + 1];
+  double val[NI][ND * ND];  // item (k, b): rows of the node's DOFs, columns of node b's
+  double mv[NI];
+  int16_t slot[NI];
 };
 
 __device__ inline int find_slot(const int32_t *nb, int deg, int32_t j) {
@@ -243,23 +252,39 @@ __global__ __launch_bounds__(64 * kWaves) void gather_kernel(
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  constexpr int NDF = NPE * ND;
-  for (int32_t p = adj_ptr[s_node]; p < adj_ptr[s_node + 1]; p++) {
-    const int64_t e = adj_elem[p];
-    const int a = adj_local[p];
-    const double *kr = Ke + e * (int64_t)NDF * NDF + (int64_t)a * ND * NDF;
-    for (int t = lane; t < ND * NDF; t += 64) {
-      const int d = t / NDF, c = t % NDF, b = c / ND, dd = c % ND;
-      const int s = find_slot(L.nb, deg, conn[e * NPE + b]);
-      L.k[d][L.off[s] + dd] += kr[d * NDF + c];
-    }
-    if (lane < NPE) {
-      const int s = find_slot(L.nb, deg, conn[e * NPE + lane]);
-      L.m[s] += Me[e * NPE * NPE + a * NPE + lane];
+  constexpr int NDF = NPE * ND, BK = GatherLds<NPE, ND>::BK;
+  const int32_t pa0 = adj_ptr[s_node], pa1 = adj_ptr[s_node + 1];
+  for (int32_t pb = pa0; pb < pa1; pb += BK) {
+    const int nk = min(BK, pa1 - pb);
+    // 1. every item of the batch at once: slot and element-matrix block
+    for (int it = lane; it < nk * NPE; it += 64) {
+      const int kk = it / NPE, b = it % NPE;
+      const int64_t e = adj_elem[pb + kk];
+      const int a = adj_local[pb + kk];
+      L.slot[it] = (int16_t)find_slot(L.nb, deg, conn[e * NPE + b]);
+      const double *kr = Ke + e * (int64_t)NDF * NDF + (int64_t)a * ND * NDF + b * ND;
+#pragma unroll
+      for (int d = 0; d < ND; d++)
+#pragma unroll
+        for (int dd = 0; dd < ND; dd++) L.val[it][d * ND + dd] = kr[d * NDF + dd];
+      L.mv[it] = Me[e * NPE * NPE + a * NPE + b];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // 2. ordered accumulation: element by element (the items of one element
+    //    go to distinct slots)
+    for (int kk = 0; kk < nk; kk++) {
+      for (int u = lane; u < NPE * ND * ND; u += 64) {
+        const int b = u / (ND * ND), c = u % (ND * ND);
+        const int it = kk * NPE + b, sl = L.slot[it];
+        L.k[c / ND][L.off[sl] + c % ND] += L.val[it][c];
+        if (c == 0) L.m[sl] += L.mv[it];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
   for (int d = 0; d < ND; d++) {
     const int64_t base = rowptr[r0 + d];
