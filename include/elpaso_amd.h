@@ -148,7 +148,9 @@ int elp_model_set_rayleigh(elp_model *m, double alpha, double beta);
  * Enables the multigrid preconditioner. */
 int elp_model_set_box(elp_model *m, const double *origin, const double *length,
                       const int32_t *nel, int order, int robin_sides);
-/* DOF numbering, symbolic phase (sparsity pattern, adjacency), upload */
+/* DOF numbering, symbolic phase (sparsity pattern, adjacency), upload.
+ * ELP_EINVAL for a node coupled to more than 256 neighbour nodes or columns,
+ * ELP_EDETJ (at assembly) for an element with detJ <= 0. */
 int elp_model_finalize(elp_model *m);
 int elp_model_info(const elp_model *m, int64_t *n_dof, int64_t *nnz, int64_t *device_bytes);
 /* /Solution/Maps/mtxDofMap: row -> (node id, dof enum) */
