@@ -752,6 +752,28 @@ __global__ void unpack_col_kernel(int64_t n, int B, int c, const double2 *__rest
   out[r] = x[r * B + c];
 }
 
+// FP64 refinement of a few frozen columns: their x64 columns gathered into a
+// narrow [row][CB] block (xc[r][f] = x64[r][map[f]]), and r64[r][map[f]] =
+// F[r] - (A x)[r][f] scattered back for the nfz frozen columns (the same
+// values as b - A x computed on all B columns: each column is summed alone)
+__global__ void gather_cols_kernel(int64_t n, int B, int CB, const int *__restrict__ map,
+                                   const double2 *__restrict__ x, double2 *__restrict__ xc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * CB) return;
+  xc[i] = x[(i / CB) * B + map[i % CB]];
+}
+__global__ void scatter_residual_kernel(int64_t n, int B, int CB, int nfz, const int *__restrict__ map,
+                                        const double2 *__restrict__ F, const double2 *__restrict__ y,
+                                        double2 *__restrict__ r64) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * CB) return;
+  const int f = (int)(i % CB);
+  if (f >= nfz) return;
+  const int64_t r = i / CB;
+  const double2 fv = F[r], yv = y[i];
+  r64[r * B + map[f]] = {fv.x - yv.x, fv.y - yv.y};
+}
+
 // columns taking a new frequency: x64 = 0, r64 = b
 __global__ void slot_fresh_kernel(int64_t n, int B, const int *__restrict__ fresh, const double2 *__restrict__ F,
                                   double2 *__restrict__ x64, double2 *__restrict__ r64) {
@@ -1503,6 +1525,14 @@ struct Solver {
     hipLaunchKernelGGL((op_kernel<TT, B, MODE>), dim3(grid), dim3(256), 0, s, L.template args<TT>(), coef, x, b, y);
     ELP_HIP(hipGetLastError());
   }
+  // y = A x in FP64 on CB columns (x, y: [row][CB]); coef holds CB entries
+  template <int CB>
+  void op_fp64_narrow(const OpCoef *coef, const double2 *x, double2 *y) {
+    const LevelData &L = *st.levels[0];
+    hipLaunchKernelGGL((op_kernel<double, CB, 0>), dim3(xcd_grid(nblk(L.n, OpShape<CB>::RPB))), dim3(256), 0, s,
+                       L.template args<double>(), coef, x, nullptr, y);
+    ELP_HIP(hipGetLastError());
+  }
   template <bool ADD>
   void transfer(int64_t n, const int32_t *rp, const int32_t *ci, const T *v, const V *x, V *y) {
     hipLaunchKernelGGL((transfer_kernel<T, B, ADD>), dim3(xcd_grid(nblk(n, OpShape<B>::RPB))), dim3(256), 0, s, n, rp,
@@ -1964,6 +1994,14 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   for (int k = 0; k < B; k++) set_coefs(k, 0);
   std::vector<int> act(B, 0), fresh(B, 0);
   std::vector<double> scale(B, 0.0), wgt(B, 0.0);
+  // narrow FP64 residual of the frozen columns (allocated at first use)
+  std::vector<int> fmap(B, 0);
+  std::vector<OpCoef> cAn(B);
+  DArr<int> d_fmap;
+  DArr<OpCoef> d_cAn;
+  DArr<double2> xn, yn;
+  d_fmap.alloc(m.ctx, B);
+  d_cAn.alloc(m.ctx, B);
   DArr<int> d_fresh;
   DArr<double> d_scale, d_wgt;
   d_fresh.alloc(m.ctx, B);
@@ -2107,7 +2145,31 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
                        iperm);
-    S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+    // r64 = b - A x64 on the frozen columns: on a narrow block of 8 or 16
+    // columns when few are frozen (the gathers of x shrink with the block)
+    int nfz = 0;
+    for (int k = 0; k < B; k++)
+      if (sl[k].state == 2) fmap[nfz++] = k;
+    if (B == 32 && nfz <= 16) {
+      const int CB = nfz <= 8 ? 8 : 16;
+      for (int f = nfz; f < CB; f++) fmap[f] = fmap[0];  // padding columns repeat a frozen one
+      for (int f = 0; f < CB; f++) cAn[f] = cA[fmap[f]];
+      if (!xn.p) {
+        xn.alloc(m.ctx, m.n * 16);
+        yn.alloc(m.ctx, m.n * 16);
+      }
+      ELP_HIP(hipMemcpyAsync(d_fmap.p, fmap.data(), sizeof(int) * CB, hipMemcpyHostToDevice, s));
+      ELP_HIP(hipMemcpyAsync(d_cAn.p, cAn.data(), sizeof(OpCoef) * CB, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(gather_cols_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, d_fmap.p, x64.p,
+                         xn.p);
+      if (CB == 8) S.template op_fp64_narrow<8>(d_cAn.p, xn.p, yn.p);
+      else S.template op_fp64_narrow<16>(d_cAn.p, xn.p, yn.p);
+      hipLaunchKernelGGL(scatter_residual_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, nfz,
+                         d_fmap.p, F.p, yn.p, r64.p);
+      ELP_HIP(hipGetLastError());
+    } else {
+      S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+    }
     S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
     S.fetch(S.red_out.p);
     std::vector<double> rel(B);
@@ -2188,6 +2250,7 @@ int64_t solver_bytes_estimate(const Model &m, int block, int precision) {
   b += m.nnz * 18;                                     // permuted columns, row-padded operator copy
   b += m.n * 512;                                      // box products A P, padded P rows, lattice
   b += (int64_t)m.n * block * (5 * vb * 2 + 4 * 16);   // level-0 + Krylov + FP64 outer vectors
+  if (precision != ELP_PREC_FP64) b += m.n * 16 * 16 * 2;  // narrow FP64 refinement blocks (x, A x)
   return b + b / 4;                                    // coarse levels (about 1/8 each) and slack
 }
 
