@@ -200,13 +200,27 @@ class GpuModel:
             _check(L.elp_model_set_rayleigh(self._h, a, b))
         box = box if box is not None else getattr(model, "box", None)
         if box is not None:
-            o = np.ascontiguousarray(box["origin"], dtype=np.float64)
-            ln = np.ascontiguousarray(box["length"], dtype=np.float64)
-            ne = np.ascontiguousarray(box["nel"], dtype=np.int32)
-            _check(L.elp_model_set_box(self._h, _ptr(o), _ptr(ln), _ptr(ne), int(box["order"]),
-                                       int(box.get("robin_sides", 0))))
+            self.set_box(box)
         if finalize:
             self.finalize()
+
+    @classmethod
+    def empty(cls, ctx: Context):
+        """An unfinalized model with nothing in it (for readers that fill the
+        handle through the C-ABI themselves, e.g. the HDF5 input)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.model = None
+        self._h = C.c_void_p()
+        _check(load_library().elp_model_create(ctx.handle, C.byref(self._h)))
+        return self
+
+    def set_box(self, box):
+        o = np.ascontiguousarray(box["origin"], dtype=np.float64)
+        ln = np.ascontiguousarray(box["length"], dtype=np.float64)
+        ne = np.ascontiguousarray(box["nel"], dtype=np.int32)
+        _check(load_library().elp_model_set_box(self._h, _ptr(o), _ptr(ln), _ptr(ne), int(box["order"]),
+                                                int(box.get("robin_sides", 0))))
 
     def finalize(self):
         L = load_library()

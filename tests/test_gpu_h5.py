@@ -1,0 +1,165 @@
+"""elPaSo HDF5 input on the GPU: models read by elp_h5_read_model assemble to
+the same CSR as the same model given through the C-ABI calls, and to the
+oracle's matrices (1e-12), for
+- the reference's own 2-element Fluid8 input fixture (nodes with gaps in
+  their ids, an acoustic Dirichlet constraint), its contents read
+  independently with the HDF5 tools (h5dump), and
+- a coupled Brick8 plate / Fluid8 cavity written in elPaSo's input layout
+  (viscoelastic material, InterfaceMindlin elements, clamped structure
+  constraints, a point force and a fluid source).
+A solved sweep written with elp_h5_write_solution reads back bit for bit."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from elpaso_amd import FLUID8, ElementGroup, GpuModel, Material, Model, h5
+from elpaso_amd import _lib
+from elpaso_amd import meshgen as MG
+
+from h5_elpaso_writer import write_input
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FLUID8_2ELEM = os.path.join(ROOT, "tests", "golden", "acousticmedium_fluid8_2elem_AF_LIN_UAF_ISO_DIR.hdf5")
+H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+def _data_block(text):
+    """The DATA { ... } block of the first dataset in h5dump output (its
+    attributes follow it and are left out)."""
+    i = text.index("DATA {") + len("DATA {")
+    depth = 1
+    for j in range(i, len(text)):
+        depth += {"{": 1, "}": -1}.get(text[j], 0)
+        if depth == 0:
+            return text[i:j]
+    raise ValueError("unbalanced h5dump output")
+
+
+def _numbers(text):
+    body = _data_block(text)
+    return [float(t) for t in re.findall(r"(?<![\w(])-?\d+(?:\.\d*)?(?:[eE][-+]?\d+)?(?![\w)])", body)]
+
+
+def _dump(*args):
+    return subprocess.run([H5DUMP, "-m", "%.17g", *args], capture_output=True, text=True, check=True).stdout
+
+
+def _same_csr(a: GpuModel, b: GpuModel, which):
+    assert (a.n, a.nnz) == (b.n, b.nnz)
+    ra, ca = a.pattern()
+    rb, cb = b.pattern()
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    ia, da = a.dof_map()
+    ib, db = b.dof_map()
+    assert np.array_equal(ia, ib) and np.array_equal(da, db)
+    for w in which:
+        assert np.array_equal(a.values(w), b.values(w)), f"matrix {w}"
+
+
+def _oracle_close(model, g, which):
+    num = O.Numbering(model)
+    rp, ci = O.pattern(model, num)
+    grp, gci = g.pattern()
+    assert np.array_equal(rp, grp) and np.array_equal(ci, gci)
+    ref = O.assemble_km(model, num, (rp, ci))
+    refs = {_lib.MAT_K: ref.K, _lib.MAT_M: ref.M, _lib.MAT_S: ref.S}
+    for w in which:
+        got, r = g.values(w), refs[w]
+        assert np.linalg.norm(got - r) <= TOL * np.linalg.norm(r), f"matrix {w}"
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="h5dump (HDF5 tools) not installed")
+def test_reference_fluid8_fixture(ctx):
+    print("[h5] reference Fluid8 2-element fixture", flush=True)
+    g, an, counts = h5.read_model(ctx, FLUID8_2ELEM)
+    assert counts == {"nodes": 12, "materials": 1, "element_groups": 1, "elements": 2, "constraints": 1,
+                      "node_loads": 0, "interfaces": 0}
+    assert an.type == "frequency-basic" and an.steps == 10
+    # the same model from the file's contents as the HDF5 tools print them
+    nodes = np.array(_numbers(_dump("-d", "/Nodes/mtxFemNodes", FLUID8_2ELEM))).reshape(-1, 4)
+    rows = np.array(_numbers(_dump("-d", "/Elements/mtxFemElemGroup1", FLUID8_2ELEM)), np.int64).reshape(-1, 9)
+    attrs = _dump("-A", "-d", "/Materials/material1", FLUID8_2ELEM)
+    c = float(re.search(r'ATTRIBUTE "c".*?\(0\): "([^"]+)"', attrs, re.S).group(1))
+    rho = float(re.search(r'ATTRIBUTE "rho".*?\(0\): "([^"]+)"', attrs, re.S).group(1))
+    assert (c, rho) == (343.0, 1.21)
+    m = Model(node_ids=nodes[:, 0].astype(np.int64), xyz=np.ascontiguousarray(nodes[:, 1:]), name="fluid8_2elem")
+    m.materials[1] = Material("fluid", rho=rho, c=c)
+    m.groups.append(ElementGroup(FLUID8, 1, rows[:, 0], rows[:, 1:]))
+    m.dirichlet.append((7, 16, 1.0))  # nodeConstraint1_7: acoustic, NodeId 7, valp 1
+    ref = GpuModel(ctx, m)
+    for x in (g, ref):
+        x.assemble()
+    _same_csr(g, ref, (_lib.MAT_K, _lib.MAT_M))
+    _oracle_close(m, g, (_lib.MAT_K, _lib.MAT_M))
+    ids, dofs = g.dof_map()
+    np.testing.assert_array_equal(ids, np.sort(nodes[:, 0]).astype(np.int64))
+    assert (dofs == 16).all()
+    # the constrained row: identity in A(w), prescribed value in F (MatZeroRows)
+    rp, ci = g.pattern()
+    a, f = g.dynamic_matrix(2 * np.pi * 10.0)
+    r = int(np.nonzero(ids == 7)[0][0])
+    row = a[rp[r]:rp[r + 1]]
+    assert np.array_equal(row, np.where(ci[rp[r]:rp[r + 1]] == r, 1.0, 0.0))
+    assert f[r] == 1.0 and np.count_nonzero(f) == 1
+
+
+def test_coupled_model_written_in_elpaso_layout(ctx, tmp_path):
+    print("[h5] coupled plate/cavity 4x4x(2+3)", flush=True)
+    m = MG.coupled_plate_cavity(n_xy=4, n_plate_z=2, n_cav_z=3)
+    m.sources.append((int(m.node_ids[5]), 0.5))
+    path = tmp_path / "coupled.hdf5"
+    write_input(path, m, start=20.0, delta=980.0 / 511, steps=512)
+    g, an, counts = h5.read_model(ctx, path)
+    assert counts["interfaces"] == len(m.if_ids)
+    assert counts["constraints"] == len(m.dirichlet)
+    assert counts["node_loads"] == 2 and counts["element_groups"] == 2
+    np.testing.assert_array_equal(an.frequencies(), 20.0 + (980.0 / 511) * np.arange(512))
+    ref = GpuModel(ctx, m)
+    for x in (g, ref):
+        x.assemble()
+    _same_csr(g, ref, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_S))
+    _oracle_close(m, g, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_S))
+    for f in (20.0, 500.0):
+        a1, f1 = g.dynamic_matrix(2 * np.pi * f)
+        a2, f2 = ref.dynamic_matrix(2 * np.pi * f)
+        assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
+
+
+def test_unsupported_content_is_rejected(ctx, tmp_path):
+    m = MG.acoustic_cavity(2, 2, 2)
+    m.materials[1] = Material("fluid", rho=1.21, c=343.0)
+    path = tmp_path / "bad.hdf5"
+    write_input(path, m, revision=5)
+    with pytest.raises(_lib.ElpError, match="revision 5"):
+        h5.read_model(ctx, path)
+
+
+def test_sweep_written_and_read_back(ctx, tmp_path):
+    print("[h5] sweep of the reference fixture -> solution file", flush=True)
+    g, an, _ = h5.read_model(ctx, FLUID8_2ELEM)
+    g.assemble()
+    freqs = an.frequencies()
+    # the sweep solver takes zero prescribed values; the dynamic matrix keeps
+    # the fixture's value 1: solve a few frequencies with numpy instead
+    rp, ci = g.pattern()
+    X = np.zeros((len(freqs), g.n), complex)
+    for k, f in enumerate(freqs):
+        a, fv = g.dynamic_matrix(2 * np.pi * f)
+        A = np.zeros((g.n, g.n), complex)
+        for r in range(g.n):
+            A[r, ci[rp[r]:rp[r + 1]]] = a[rp[r]:rp[r + 1]]
+        X[k] = np.linalg.solve(A, fv)
+    ids, dofs = g.dof_map()
+    out = tmp_path / "eGenOutput_fixture.hdf5"
+    h5.write_solution(out, freqs, X, ids, dofs)
+    for k in range(len(freqs)):
+        np.testing.assert_array_equal(h5.read_complex_vector(out, f"/Solution/State/vecFemStep{k + 1}", g.n), X[k])
+    r = int(np.nonzero(ids == 7)[0][0])
+    assert np.allclose(X[:, r], 1.0)  # the prescribed pressure
