@@ -335,6 +335,28 @@ int elp_dynamic_matrix(elp_model *mm, double omega, double z_re, double z_im, do
   ELP_CATCH
 }
 
+int elp_results_sizes(const elp_model *mm, int64_t *n_nodes, int64_t *n_brick_elements) {
+  ELP_TRY
+  ELP_ARG(mm && mm->m && n_nodes && n_brick_elements, "null argument");
+  const auto &m = *mm->m;
+  ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
+  *n_nodes = m.n_nodes;
+  *n_brick_elements = elp::brick_element_count(m);
+  ELP_CATCH
+}
+
+int elp_node_results(elp_model *mm, int64_t n_freq, const double *freqs_hz, const double *X, double *cells,
+                     double *nodes) {
+  ELP_TRY
+  auto &m = M(mm);
+  ELP_ARG(n_freq > 0 && freqs_hz && X && (cells || nodes), "invalid arguments");
+  ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
+  ELP_HIP(hipSetDevice(mm->m->ctx->device));
+  std::lock_guard<std::mutex> lk(m.mu);
+  elp::node_results(m, n_freq, freqs_hz, X, cells, nodes);
+  ELP_CATCH
+}
+
 int elp_element_matrices(elp_ctx *ctx, int kind, int64_t n, const double *xyz, double p0, double p1, double p2,
                          double *Ke, double *Me) {
   ELP_TRY

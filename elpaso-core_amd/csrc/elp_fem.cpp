@@ -1,6 +1,9 @@
 // Host construction of the quadrature and shape-function tables (see elp_fem.h).
 #include "elp_fem.h"
 
+#include <array>
+#include <vector>
+
 namespace elp {
 
 const int kHex8Nat[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
@@ -140,6 +143,22 @@ static HexTables build() {
 const HexTables &host_tables() {
   static const HexTables t = build();
   return t;
+}
+
+std::vector<double> hex_derivs_at(int npe, const std::vector<std::array<double, 3>> &pts) {
+  std::vector<double> T(pts.size() * 3 * npe);
+  for (size_t a = 0; a < pts.size(); a++)
+    for (int d = 0; d < 3; d++)
+      for (int f = 0; f < npe; f++)
+        T[(a * 3 + d) * npe + f] = npe == 8 ? hex8_fn(f, d + 1, pts[a].data()) : hex20_fn(f, d + 1, pts[a].data());
+  return T;
+}
+
+std::vector<double> hex_nodal_derivs(int npe) {
+  std::vector<std::array<double, 3>> pts(npe);
+  for (int a = 0; a < npe; a++)
+    for (int i = 0; i < 3; i++) pts[a][i] = npe == 8 ? kHex8Nat[a][i] : kHex20Nat[a][i];
+  return hex_derivs_at(npe, pts);
 }
 
 }  // namespace elp

@@ -10,7 +10,9 @@
 // FP64 and copied to device constant memory.
 #pragma once
 
+#include <array>
 #include <cstdint>
+#include <vector>
 
 namespace elp {
 
@@ -33,6 +35,13 @@ struct HexTables {
 };
 
 const HexTables &host_tables();
+
+// shape-function derivatives d/dxi, d/deta, d/dzeta of the npe-node brick
+// (8 or 20) at the given natural coordinates: T[point][deriv][node]
+std::vector<double> hex_derivs_at(int npe, const std::vector<std::array<double, 3>> &pts);
+// ... at the element's own nodes (elPaSo evaluates nodal gradients there,
+// element/fluid/elementfluid3d.cpp:407-430)
+std::vector<double> hex_nodal_derivs(int npe);
 
 // natural coordinates of local nodes
 extern const int kHex8Nat[8][3];

@@ -1,0 +1,354 @@
+// Results at elements and nodes from a block of solutions, on the device:
+// the pressure gradient of the fluid elements at their nodes
+// (cElementFluid3d::computeStressesNodes / computeGradP / computeGradPPoint,
+// element/fluid/elementfluid3d.cpp:380-540) and the stresses of the bricks at
+// their centroid (cElementStructureBrick::computeStressesCells /
+// computeStressesNodes, element/structure/linear/brick/elementstructurebrick.cpp:48-279),
+// collected per node as cAnalysis::computeStressesNodes does
+// (analysis/analysis.cpp:869-937): 9 values per node (sigma 0..5, grad p 6..8).
+//
+// Layout: solutions X[f][row] (complex), f over a chunk of frequencies.  Per
+// chunk: one thread per (brick, frequency) for the centroid stresses, one per
+// (node, frequency) summing them over the node's bricks, and one per fluid
+// node applying each of its elements' gradient operators to all of the
+// chunk's frequencies.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "elp_fem.h"
+#include "elp_internal.h"
+
+namespace elp {
+
+namespace {
+
+constexpr int kPostThreads = 128;
+
+// J(i,j) = sum_k dN_k/dxi_i x_k[j]; det by Sarrus' rule, inverse by cofactors
+// (element/elementfem.cpp:158-210)
+__device__ inline double jac_inverse(const double J[3][3], double inv[3][3]) {
+  const double det = J[0][0] * J[1][1] * J[2][2] + J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] -
+                     J[0][2] * J[1][1] * J[2][0] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2];
+  const double r = 1.0 / det;
+  inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * r;
+  inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+  inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+  inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * r;
+  inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+  inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+  inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * r;
+  inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+  inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+  return det;
+}
+
+// sum of v[0..n) in elPaSo's order: sorted by magnitude, largest first
+// (std::sort with abs_order, then summed from the back; misc/mytypes.h:226).
+// Odd-even transposition network: N rounds of fixed compare-exchanges, all
+// indices static so v stays in registers; it swaps only strictly larger
+// magnitudes, so equal magnitudes keep their order.
+template <int N>
+__device__ inline double abs_order_sum(double (&v)[N]) {
+#pragma unroll
+  for (int round = 0; round < N; round++) {
+#pragma unroll
+    for (int i = round & 1; i + 1 < N; i += 2) {
+      const double a = v[i], b = v[i + 1];
+      const bool sw = fabs(a) > fabs(b);
+      v[i] = sw ? b : a;
+      v[i + 1] = sw ? a : b;
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int i = N - 1; i >= 0; i--) s += v[i];
+  return s;
+}
+
+// grad p at the fluid nodes: one thread per node of a fluid kind; for each
+// of the node's elements in ascending id the element's gradient operator at
+// the node (J^-1 dN, frequency independent) is formed once and applied to the
+// chunk's nf solutions, each added to the node's slots 6..8 of out[f][r][9]
+// (the node's own slots: no other thread writes them).
+template <int NPE>
+__global__ __launch_bounds__(kPostThreads) void fluid_node_gradp_kernel(
+    int64_t n_kn, const int32_t *__restrict__ knodes, const int32_t *__restrict__ adj_ptr,
+    const int32_t *__restrict__ adj_elem, const uint8_t *__restrict__ adj_local, const int32_t *__restrict__ conn,
+    const double *__restrict__ xyz, const int32_t *__restrict__ node_row, const double *__restrict__ T, int nf,
+    int64_t n, int64_t n_nodes, const double2 *__restrict__ X, double2 *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_kn) return;
+  const int32_t r = knodes[i];
+  for (int32_t j = adj_ptr[i]; j < adj_ptr[i + 1]; j++) {
+    const int64_t e = adj_elem[j];
+    const double *Ta = T + (int)adj_local[j] * 3 * NPE;
+    double x[NPE][3];
+    int32_t row[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; k++) {
+      const int32_t q = conn[e * NPE + k];
+      x[k][0] = xyz[(int64_t)q * 3];
+      x[k][1] = xyz[(int64_t)q * 3 + 1];
+      x[k][2] = xyz[(int64_t)q * 3 + 2];
+      row[k] = node_row[q];
+    }
+    double J[3][3], inv[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NPE; k++) s += Ta[a * NPE + k] * x[k][b];
+        J[a][b] = s;
+      }
+    jac_inverse(J, inv);
+    double jh[3][NPE];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int k = 0; k < NPE; k++)
+        jh[a][k] = inv[a][0] * Ta[k] + inv[a][1] * Ta[NPE + k] + inv[a][2] * Ta[2 * NPE + k];
+    for (int f = 0; f < nf; f++) {
+      const double2 *Xf = X + (int64_t)f * n;
+      double2 p[NPE];
+#pragma unroll
+      for (int k = 0; k < NPE; k++) p[k] = Xf[row[k]];
+      double2 *o = out + ((int64_t)f * n_nodes + r) * 9 + 6;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        double re[NPE], im[NPE];
+#pragma unroll
+        for (int k = 0; k < NPE; k++) {
+          re[k] = jh[a][k] * p[k].x;
+          im[k] = jh[a][k] * p[k].y;
+        }
+        const double2 g = double2{abs_order_sum(re), abs_order_sum(im)};
+        double2 v = o[a];
+        v.x += g.x;
+        v.y += g.y;
+        o[a] = v;
+      }
+    }
+  }
+}
+
+// stresses of one brick at its centroid from its 8 corner nodes:
+// out[f][e][6] = scale_f * D * B u (strain order xx, yy, zz, xy, yz, xz as
+// the B matrix of elementstructurebrick.cpp:97-107)
+__global__ __launch_bounds__(kPostThreads) void brick_centroid_kernel(int64_t n_elem, int npe,
+                                                                     const int32_t *__restrict__ conn,
+                                                                     const double *__restrict__ xyz,
+                                                                     const int32_t *__restrict__ node_row,
+                                                                     const double *__restrict__ params,
+                                                                     const double *__restrict__ T,
+                                                                     const double2 *__restrict__ scale, int64_t n,
+                                                                     const double2 *__restrict__ X,
+                                                                     double2 *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (e >= n_elem) return;
+  double x[8][3];
+  int32_t row[8];
+  for (int k = 0; k < 8; k++) {
+    const int32_t r = conn[e * npe + k];
+    for (int j = 0; j < 3; j++) x[k][j] = xyz[(int64_t)r * 3 + j];
+    row[k] = node_row[r];
+  }
+  double J[3][3], inv[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double s = 0.0;
+      for (int k = 0; k < 8; k++) s += T[i * 8 + k] * x[k][j];
+      J[i][j] = s;
+    }
+  jac_inverse(J, inv);
+  double2 eps[6] = {};
+  const double2 *Xf = X + (int64_t)f * n;
+  for (int k = 0; k < 8; k++) {
+    const double nx = inv[0][0] * T[k] + inv[0][1] * T[8 + k] + inv[0][2] * T[16 + k];
+    const double ny = inv[1][0] * T[k] + inv[1][1] * T[8 + k] + inv[1][2] * T[16 + k];
+    const double nz = inv[2][0] * T[k] + inv[2][1] * T[8 + k] + inv[2][2] * T[16 + k];
+    const double2 u0 = Xf[row[k]], u1 = Xf[row[k] + 1], u2 = Xf[row[k] + 2];
+    eps[0].x += nx * u0.x; eps[0].y += nx * u0.y;
+    eps[1].x += ny * u1.x; eps[1].y += ny * u1.y;
+    eps[2].x += nz * u2.x; eps[2].y += nz * u2.y;
+    eps[3].x += ny * u0.x; eps[3].y += ny * u0.y;
+    eps[3].x += nx * u1.x; eps[3].y += nx * u1.y;
+    eps[4].x += nz * u1.x; eps[4].y += nz * u1.y;
+    eps[4].x += ny * u2.x; eps[4].y += ny * u2.y;
+    eps[5].x += nz * u0.x; eps[5].y += nz * u0.y;
+    eps[5].x += nx * u2.x; eps[5].y += nx * u2.y;
+  }
+  // isotropic D (materialstructureisotrop.cpp:130-147): lambda, mu per element
+  const double lam = params[e * 4], mu = params[e * 4 + 1];
+  const double c11 = lam + 2.0 * mu;
+  double2 sig[6];
+  for (int c = 0; c < 2; c++) {
+    const double e0 = c ? eps[0].y : eps[0].x, e1 = c ? eps[1].y : eps[1].x, e2 = c ? eps[2].y : eps[2].x;
+    const double s0 = c11 * e0 + lam * e1 + lam * e2;
+    const double s1 = lam * e0 + c11 * e1 + lam * e2;
+    const double s2 = lam * e0 + lam * e1 + c11 * e2;
+    if (c) { sig[0].y = s0; sig[1].y = s1; sig[2].y = s2; }
+    else { sig[0].x = s0; sig[1].x = s1; sig[2].x = s2; }
+  }
+  for (int i = 3; i < 6; i++) sig[i] = double2{mu * eps[i].x, mu * eps[i].y};
+  const double2 sc = scale[f];  // 1, or the loss factor's 1 + i eta (omega)
+  double2 *o = out + ((int64_t)f * n_elem + e) * 6;
+  for (int i = 0; i < 6; i++)
+    o[i] = double2{sc.x * sig[i].x - sc.y * sig[i].y, sc.x * sig[i].y + sc.y * sig[i].x};
+}
+
+struct PostKind {
+  const int32_t *adj_ptr = nullptr, *adj_elem = nullptr;
+  const uint8_t *adj_local = nullptr;
+  const double2 *res = nullptr;  // brick centroid stresses [f][e][6]
+  int64_t n_elem = 0;
+  int npe = 0, fluid = 0;
+};
+struct PostKinds {
+  PostKind k[4];
+};
+
+// per node: the stresses of its bricks in ascending id, each scaled by
+// 1 / (elements at the node) before it is added (elementstructurebrick.cpp:268-277);
+// all other slots 0 (fluid_node_gradp_kernel adds grad p afterwards,
+// elementfluid3d.cpp:395-404)
+__global__ __launch_bounds__(kPostThreads) void node_results_kernel(int64_t n_nodes, const int8_t *__restrict__ slot,
+                                                                   const int32_t *__restrict__ pos, PostKinds K,
+                                                                   double2 *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (r >= n_nodes) return;
+  double2 acc[9] = {};
+  const int s = slot[r];
+  if (s >= 0 && !K.k[s].fluid) {
+    const PostKind &kb = K.k[s];
+    const int32_t j0 = kb.adj_ptr[pos[r]], j1 = kb.adj_ptr[pos[r] + 1];
+    const double w = 1.0 / (double)(j1 - j0);
+    for (int32_t j = j0; j < j1; j++) {
+      const double2 *g = kb.res + ((int64_t)f * kb.n_elem + kb.adj_elem[j]) * 6;
+      for (int i = 0; i < 6; i++) {
+        acc[i].x += g[i].x * w;
+        acc[i].y += g[i].y * w;
+      }
+    }
+  }
+  double2 *o = out + ((int64_t)f * n_nodes + r) * 9;
+  for (int i = 0; i < 9; i++) o[i] = acc[i];
+}
+
+}  // namespace
+
+void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes) {
+  ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
+  Ctx *c = m.ctx;
+  hipStream_t s = c->stream;
+  const int64_t N = m.n_nodes, n = m.n;
+  // node -> (kind slot, entry of the kind's node list)
+  std::vector<int8_t> slot(N, -1);
+  std::vector<int32_t> pos(N, 0);
+  for (size_t k = 0; k < m.kinds.size(); k++)
+    for (size_t i = 0; i < m.kinds[k].nodes.size(); i++) {
+      slot[m.kinds[k].nodes[i]] = (int8_t)k;
+      pos[m.kinds[k].nodes[i]] = (int32_t)i;
+    }
+  DArr<int8_t> d_slot;
+  DArr<int32_t> d_pos;
+  d_slot.upload(c, slot);
+  d_pos.upload(c, pos);
+  // derivative tables: fluid kinds at their nodes, bricks at the centroid
+  DArr<double> d_T8, d_T20, d_Tc;
+  d_T8.upload(c, hex_nodal_derivs(8));
+  d_T20.upload(c, hex_nodal_derivs(20));
+  const std::vector<std::array<double, 3>> centroid(1, std::array<double, 3>{0.0, 0.0, 0.0});
+  d_Tc.upload(c, hex_derivs_at(8, centroid));
+  // frequency chunk: solutions, node results and brick stresses within
+  // about 4 GiB (the fluid gradient operators are reused across a chunk)
+  int64_t per_f = n * 16 + N * 9 * 16;
+  for (auto &kb : m.kinds)
+    if (!is_fluid(kb.kind)) per_f += kb.n_elem * 6 * 16;
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nf, (int64_t)(4LL << 30) / per_f, 65535}));
+  DArr<double2> dX, dN, dScale;
+  std::vector<DArr<double2>> dres(m.kinds.size());
+  dX.alloc(c, chunk * n);
+  dN.alloc(c, chunk * N * 9);
+  dScale.alloc(c, chunk);
+  for (size_t k = 0; k < m.kinds.size(); k++)
+    if (!is_fluid(m.kinds[k].kind)) dres[k].alloc(c, chunk * m.kinds[k].n_elem * 6);
+  int64_t brick_cols = 0, brick_kinds = 0;  // brick elements, ascending id
+  for (auto &kb : m.kinds)
+    if (!is_fluid(kb.kind)) brick_cols += kb.n_elem, brick_kinds++;
+  ELP_REQUIRE(!cells || brick_kinds <= 1, ELP_EINVAL,
+              "element stresses of a model mixing Brick8 and Brick20 elements are not supported");
+  std::vector<double2> scale(chunk);
+  for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
+    const int nc = (int)std::min(chunk, nf - f0);
+    ELP_HIP(hipMemcpyAsync(dX.p, X + 2 * f0 * n, sizeof(double2) * nc * n, hipMemcpyHostToDevice, s));
+    for (int f = 0; f < nc; f++) {
+      // E_vis = E (1 + i eta) (type 1) or E (1 + i omega eta) (type 2)
+      // (materialstructurevisco.cpp:283-285); D is linear in E
+      const double w = 2.0 * M_PI * freqs[f0 + f];
+      const double im = m.has_visco ? (m.visco_type == 2 ? w * m.eta : m.eta) : 0.0;
+      scale[f] = double2{1.0, im};
+    }
+    ELP_HIP(hipMemcpyAsync(dScale.p, scale.data(), sizeof(double2) * nc, hipMemcpyHostToDevice, s));
+    PostKinds K;
+    for (size_t k = 0; k < m.kinds.size(); k++) {
+      KindBlock &kb = m.kinds[k];
+      K.k[k] = PostKind{kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, dres[k].p, kb.n_elem, kb.npe,
+                        (int)is_fluid(kb.kind)};
+      if (kb.n_elem == 0 || is_fluid(kb.kind)) continue;
+      dim3 grid((unsigned)((kb.n_elem + kPostThreads - 1) / kPostThreads), (unsigned)nc);
+      hipLaunchKernelGGL(brick_centroid_kernel, grid, dim3(kPostThreads), 0, s, kb.n_elem, kb.npe, kb.d_conn.p,
+                         m.d_xyz.p, m.d_node_row.p, kb.d_params.p, d_Tc.p, dScale.p, n, dX.p, dres[k].p);
+      ELP_HIP(hipGetLastError());
+    }
+    if (nodes) {
+      dim3 grid((unsigned)((N + kPostThreads - 1) / kPostThreads), (unsigned)nc);
+      hipLaunchKernelGGL(node_results_kernel, grid, dim3(kPostThreads), 0, s, N, d_slot.p, d_pos.p, K, dN.p);
+      ELP_HIP(hipGetLastError());
+      for (auto &kb : m.kinds) {
+        const int64_t nk = (int64_t)kb.nodes.size();
+        if (!is_fluid(kb.kind) || nk == 0) continue;
+        dim3 g1((unsigned)((nk + kPostThreads - 1) / kPostThreads));
+        if (kb.kind == ELP_FLUID8)
+          hipLaunchKernelGGL(fluid_node_gradp_kernel<8>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p,
+                             kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p,
+                             m.d_node_row.p, d_T8.p, nc, n, N, dX.p, dN.p);
+        else
+          hipLaunchKernelGGL(fluid_node_gradp_kernel<20>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p,
+                             kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p,
+                             m.d_node_row.p, d_T20.p, nc, n, N, dX.p, dN.p);
+        ELP_HIP(hipGetLastError());
+      }
+      ELP_HIP(hipMemcpyAsync(nodes + 2 * f0 * N * 9, dN.p, sizeof(double2) * nc * N * 9, hipMemcpyDeviceToHost, s));
+    }
+    if (cells && brick_cols) {
+      // [f][brick element][6]
+      for (int f = 0; f < nc; f++) {
+        int64_t col = 0;
+        for (size_t k = 0; k < m.kinds.size(); k++) {
+          KindBlock &kb = m.kinds[k];
+          if (is_fluid(kb.kind) || kb.n_elem == 0) continue;
+          ELP_HIP(hipMemcpyAsync(cells + 2 * 6 * ((f0 + f) * brick_cols + col), dres[k].p + (int64_t)f * kb.n_elem * 6,
+                                 sizeof(double2) * kb.n_elem * 6, hipMemcpyDeviceToHost, s));
+          col += kb.n_elem;
+        }
+      }
+    }
+    ELP_HIP(hipStreamSynchronize(s));
+  }
+}
+
+int64_t brick_element_count(const Model &m) {
+  int64_t b = 0;
+  for (auto &kb : m.kinds)
+    if (!is_fluid(kb.kind)) b += kb.n_elem;
+  return b;
+}
+
+}  // namespace elp

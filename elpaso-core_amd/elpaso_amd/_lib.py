@@ -21,6 +21,7 @@ EXPORTED_SYMBOLS = [
     "elp_model_add_dirichlet", "elp_model_set_rayleigh", "elp_model_set_box", "elp_model_finalize",
     "elp_model_info", "elp_model_dof_map", "elp_assemble", "elp_assemble_timed", "elp_get_pattern",
     "elp_get_values", "elp_get_device_csr", "elp_dynamic_matrix", "elp_element_matrices", "elp_sweep",
+    "elp_results_sizes", "elp_node_results",
     "elp_server_create", "elp_server_destroy", "elp_server_submit", "elp_server_poll", "elp_server_wait",
     "elp_server_stats",
 ]
@@ -94,6 +95,8 @@ def load_library():
     L.elp_dynamic_matrix.argtypes = [vp, dbl, dbl, dbl, vp, vp]
     L.elp_element_matrices.argtypes = [vp, C.c_int, i64, vp, dbl, dbl, dbl, vp, vp]
     L.elp_sweep.argtypes = [vp, i64, vp, dbl, dbl, P(SolveOpts), vp, vp, vp, P(SweepStats)]
+    L.elp_results_sizes.argtypes = [vp, P(i64), P(i64)]
+    L.elp_node_results.argtypes = [vp, i64, vp, vp, vp, vp]
     L.elp_server_create.argtypes = [vp, i64, C.c_int, P(vp)]
     L.elp_server_destroy.argtypes = [vp]
     L.elp_server_destroy.restype = None
@@ -296,6 +299,24 @@ class GpuModel:
         _check(load_library().elp_sweep(self._h, nf, _ptr(freqs), Z.real, Z.imag, C.byref(o),
                                         _ptr(X) if X is not None else None, _ptr(rel), _ptr(it), C.byref(st)))
         return X, rel, it, st
+
+
+    def node_results(self, X, freqs, cells=True, nodes=True):
+        """Brick centroid stresses [nf, bricks, 6] and node results [nf,
+        nodes, 9] (sigma 0..5, grad p 6..8; nodes in ascending id) of
+        solutions X [nf, n] (elp_node_results)."""
+        L = load_library()
+        X = np.ascontiguousarray(X, dtype=np.complex128)
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        if X.shape != (len(freqs), self.n):
+            raise ValueError("X must be n_freq x n")
+        nn, nb = C.c_int64(), C.c_int64()
+        _check(L.elp_results_sizes(self._h, C.byref(nn), C.byref(nb)))
+        c = np.zeros((len(freqs), nb.value, 6), np.complex128) if cells else None
+        nd = np.zeros((len(freqs), nn.value, 9), np.complex128) if nodes else None
+        _check(L.elp_node_results(self._h, len(freqs), _ptr(freqs), _ptr(X), _ptr(c) if c is not None else None,
+                                  _ptr(nd) if nd is not None else None))
+        return c, nd
 
 
 class Server:

@@ -189,6 +189,33 @@ int elp_sweep(elp_model *m, int64_t n_freq, const double *freqs_hz, double z_re,
               double z_im, const elp_solve_opts *opts, double *X, double *relres,
               int32_t *iters, elp_sweep_stats *stats);
 
+/* ---- results at elements and nodes ----------------------------------------- */
+/* Sizes of elp_node_results' outputs: nodes, and Brick8/Brick20 elements. */
+int elp_results_sizes(const elp_model *m, int64_t *n_nodes, int64_t *n_brick_elements);
+/* From solutions X (n_freq x n interleaved complex, as elp_sweep returns
+ * them), on the device:
+ *  cells (may be null): n_freq x n_brick_elements x 6 complex, the stresses
+ *    D B u of each brick at its centroid from its 8 corner nodes, bricks in
+ *    ascending element id, components in the order of the strain vector
+ *    (xx, yy, zz, xy, yz, xz) as elPaSo stores them in slots 0..5
+ *    (cElementStructureBrick::computeStressesCells,
+ *    element/structure/linear/brick/elementstructurebrick.cpp:48-158);
+ *    the loss factor scales D as in the solve (E(1 + i eta) or E(1 + i w eta))
+ *  nodes (may be null): n_freq x n_nodes x 9 complex in ascending node id,
+ *    elPaSo's m_StressesNodes layout (eStresses, misc/mytypes.h:79-93):
+ *    slots 0..5 the centroid stresses of the node's bricks, each divided by
+ *    the number of elements at the node, summed
+ *    (elementstructurebrick.cpp:161-279); slots 6..8 grad p of each of the
+ *    node's fluid elements evaluated at the node, summed over the elements
+ *    (cElementFluid3d::computeStressesNodes / computeGradPPoint,
+ *    element/fluid/elementfluid3d.cpp:380-540; per-element sums in elPaSo's
+ *    magnitude order); other slots 0.
+ * Replaces cAnalysis::computeStressesCells / computeStressesNodes
+ * (analysis/analysis.cpp:869-937), which run per frequency on the host.
+ * ELP_EINVAL for cells of a model mixing Brick8 and Brick20. */
+int elp_node_results(elp_model *m, int64_t n_freq, const double *freqs_hz, const double *X,
+                     double *cells, double *nodes);
+
 /* ---- serving: continuous batching of sweep requests ----------------------------- */
 int elp_server_create(elp_ctx *ctx, int64_t hbm_budget_bytes, int max_slots, elp_server **out);
 void elp_server_destroy(elp_server *s);

@@ -454,3 +454,122 @@ def solve_direct(model, freqs, num=None, pat=None, Z=None, timings=None):
         if timings is not None:
             timings.append((t1 - t0, t2 - t1))
     return X
+
+
+# ---------------------------------------------------------------------------
+# results at elements and nodes (analysis.cpp:869-937)
+# ---------------------------------------------------------------------------
+def _natural_nodes(nnod):
+    """Natural coordinates of the local nodes: the lattice point where N_k = 1
+    (getLocalCoordinatesOfElementsNodes)."""
+    pts = [np.array([x, y, z], float) for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)]
+    out = np.zeros((nnod, 3))
+    for k in range(nnod):
+        hit = [p for p in pts if abs(shape_hex(nnod, k, 0, p) - 1.0) < 1e-14]
+        assert len(hit) == 1, (nnod, k)
+        out[k] = hit[0]
+    return out
+
+
+def _inv_jacobian(nnod, ep, xyz):
+    """dN/dxi at ep, J(i,j) = sum_k dN_k/dxi_i x_k[j], and J^-1 by cofactors
+    over Sarrus' determinant (elementfem.cpp:158-210)."""
+    dN = np.array([[shape_hex(nnod, k, d, ep) for k in range(nnod)] for d in (1, 2, 3)])
+    J = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            s = 0.0
+            for k in range(nnod):
+                s += dN[i, k] * xyz[k, j]
+            J[i, j] = s
+    det = (J[0, 0] * J[1, 1] * J[2, 2] + J[0, 1] * J[1, 2] * J[2, 0] + J[0, 2] * J[1, 0] * J[2, 1]
+           - J[0, 2] * J[1, 1] * J[2, 0] - J[0, 0] * J[1, 2] * J[2, 1] - J[0, 1] * J[1, 0] * J[2, 2])
+    inv = np.array([[J[1, 1] * J[2, 2] - J[1, 2] * J[2, 1], J[0, 2] * J[2, 1] - J[0, 1] * J[2, 2],
+                     J[0, 1] * J[1, 2] - J[0, 2] * J[1, 1]],
+                    [J[1, 2] * J[2, 0] - J[1, 0] * J[2, 2], J[0, 0] * J[2, 2] - J[0, 2] * J[2, 0],
+                     J[0, 2] * J[1, 0] - J[0, 0] * J[1, 2]],
+                    [J[1, 0] * J[2, 1] - J[1, 1] * J[2, 0], J[0, 1] * J[2, 0] - J[0, 0] * J[2, 1],
+                     J[0, 0] * J[1, 1] - J[0, 1] * J[1, 0]]]) / det
+    return dN, inv
+
+
+def _abs_order_sum(vals):
+    """std::sort(abs_order) then summed from the back (elementfluid3d.cpp:521-551)."""
+    s = 0.0
+    for v in sorted(vals, key=abs, reverse=True):
+        s += v
+    return s
+
+
+def node_results(model, X, freqs, num=None):
+    """elPaSo's element and node results of solutions X [nf, n]:
+    cells [nf, bricks, 6] (computeStressesCells, elementstructurebrick.cpp:48-158:
+    D B u at the centroid from the 8 corner nodes) and nodes [nf, nodes, 9]
+    (computeStressesNodes: brick stresses / elements at the node, summed,
+    elementstructurebrick.cpp:161-279; grad p of each fluid element at the
+    node, summed, elementfluid3d.cpp:380-540), elements in ascending id."""
+    num = num or Numbering(model)
+    X = np.asarray(X)
+    nf, N = len(freqs), len(num.sorted_ids)
+    xyz = num.xyz_by_rank(model)
+    items = _elements_in_id_order(model, num)
+    numele = np.zeros(N, dtype=np.int64)
+    for gi, ids, conn, edofs in items:
+        np.add.at(numele, conn.ravel(), 1)
+    # one pass over the elements in map order (ascending id over all groups)
+    seq = []
+    for gi, ids, conn, edofs in items:
+        for k in range(len(ids)):
+            seq.append((int(ids[k]), gi, k))
+    seq.sort()
+    n_brick = sum(len(ids) for gi, ids, conn, edofs in items if model.groups[gi].kind in (BRICK8, BRICK20))
+    cells = np.zeros((nf, n_brick, 6), complex)
+    nodes = np.zeros((nf, N, 9), complex)
+    nat = {8: _natural_nodes(8), 20: _natural_nodes(20)}
+    cell = 0
+    for eid, gi, k in seq:
+        g = model.groups[gi]
+        conn = items[gi][2][k]
+        nnod = NPE[g.kind]
+        x = xyz[conn]
+        if g.kind in (FLUID8, FLUID20):
+            rows = num.rows[conn, 16]
+            derivs = [_inv_jacobian(nnod, nat[nnod][a], x) for a in range(nnod)]
+            for f in range(nf):
+                p = X[f, rows]
+                for a in range(nnod):
+                    dN, inv = derivs[a]
+                    JH = [[inv[i, 0] * dN[0, m] + inv[i, 1] * dN[1, m] + inv[i, 2] * dN[2, m] for m in range(nnod)]
+                          for i in range(3)]
+                    for i in range(3):
+                        t = [JH[i][m] * p[m] for m in range(nnod)]
+                        nodes[f, conn[a], 6 + i] += complex(_abs_order_sum([v.real for v in t]),
+                                                            _abs_order_sum([v.imag for v in t]))
+        else:
+            m = model.materials[g.material_id]
+            dN, inv = _inv_jacobian(8, np.zeros(3), x[:8])
+            B = np.zeros((6, 24))
+            for c in range(8):
+                nx, ny, nz = (inv[i, 0] * dN[0, c] + inv[i, 1] * dN[1, c] + inv[i, 2] * dN[2, c] for i in range(3))
+                B[0, 3 * c] = nx
+                B[1, 3 * c + 1] = ny
+                B[2, 3 * c + 2] = nz
+                B[3, 3 * c], B[3, 3 * c + 1] = ny, nx
+                B[4, 3 * c + 1], B[4, 3 * c + 2] = nz, ny
+                B[5, 3 * c], B[5, 3 * c + 2] = nz, nx
+            rows = num.rows[conn[:8]][:, [0, 1, 2]].ravel()
+            for f in range(nf):
+                w = 2 * np.pi * freqs[f]
+                eta = getattr(m, "eta", 0.0) or 0.0
+                E = m.E * (1 + 1j * (w * eta if getattr(m, "visco_type", 1) == 2 else eta))
+                Cm = setup_c(E, m.nu)
+                u = X[f, rows]
+                strain = [sum(B[i, j] * u[j] for j in range(24)) for i in range(6)]
+                stress = [sum(Cm[i, j] * strain[j] for j in range(6)) for i in range(6)]
+                cells[f, cell] = stress
+                for a in range(nnod):
+                    s = 1.0 / numele[conn[a]]
+                    for i in range(6):
+                        nodes[f, conn[a], i] += stress[i] * s
+            cell += 1
+    return cells, nodes
