@@ -12,7 +12,9 @@ assembly on tiny meshes and against the exact nnz of structured meshes;
 assembled matrices against closed forms (tests/test_oracle_*.py).  The solve
 is scipy's SuperLU (complex128), standing in for elPaSo's MUMPS LU, which is
 not available here: MUMPS' own bits are parity unpinned; solutions are pinned
-by the FP64 residual ||b - A x|| / ||b||.
+by the FP64 residual ||b - A x|| / ||b||.  Element and node results
+(node_results) are pinned by exact linear/quadratic fields, rigid-body
+motion and the loss-factor scale (tests/test_oracle_post.py).
 """
 from __future__ import annotations
 
