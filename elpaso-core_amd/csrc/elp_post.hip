@@ -47,19 +47,28 @@ __device__ inline double jac_inverse(const double J[3][3], double inv[3][3]) {
 
 // sum of v[0..n) in elPaSo's order: sorted by magnitude, largest first
 // (std::sort with abs_order, then summed from the back; misc/mytypes.h:226).
-// Odd-even transposition network: N rounds of fixed compare-exchanges, all
-// indices static so v stays in registers; it swaps only strictly larger
-// magnitudes, so equal magnitudes keep their order.
+// The sort is Batcher's odd-even merge network for N inputs (103
+// compare-exchanges for 20, 19 for 8; every loop bound is a compile-time
+// constant, so the network unrolls and v stays in registers).  Like
+// std::sort it does not keep the order of equal magnitudes.
 template <int N>
 __device__ inline double abs_order_sum(double (&v)[N]) {
 #pragma unroll
-  for (int round = 0; round < N; round++) {
+  for (int p = 1; p < N; p <<= 1) {
 #pragma unroll
-    for (int i = round & 1; i + 1 < N; i += 2) {
-      const double a = v[i], b = v[i + 1];
-      const bool sw = fabs(a) > fabs(b);
-      v[i] = sw ? b : a;
-      v[i + 1] = sw ? a : b;
+    for (int k = p; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+          if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+            const double a = v[i + j], b = v[i + j + k];
+            const bool sw = fabs(a) > fabs(b);
+            v[i + j] = sw ? b : a;
+            v[i + j + k] = sw ? a : b;
+          }
+        }
+      }
     }
   }
   double s = 0.0;
