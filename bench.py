@@ -58,6 +58,8 @@ def main():
                     help="processes of the CPU oracle's frequency-parallel run (capped by the host's cores)")
     ap.add_argument("--assembly-nel", type=str, default="200,200,4")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--node-results", type=int, default=13,
+                    help="frequencies of the sweep whose element/node results are timed (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 device-to-device copies of 1 GiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -134,6 +136,27 @@ def main():
                 "speedup_mixed_batched_vs_fp64": value / (len(idx) / dt64)}
         log(f"[bench] fp64 sample: {fp64}")
 
+    # element and node results (pressure gradients at the nodes) of a sample
+    # of the sweep's own solutions
+    node_res = None
+    if args.node_results > 0:
+        idx = np.linspace(0, len(freqs) - 1, args.node_results).round().astype(int)
+        Xs, _, _, _ = g.sweep(freqs[idx], opts=opts, want_x=True)
+        g.node_results(Xs[:1], freqs[idx[:1]])  # warm-up
+        tm = {}
+        t = time.perf_counter()
+        _, nd = g.node_results(Xs, freqs[idx], cells=False, timing=tm)
+        wall = time.perf_counter() - t
+        assert np.isfinite(nd).all()
+        node_res = {"workload": f"grad p at the nodes of the Hex20 cavity, {len(idx)} of the sweep's solutions",
+                    "kernel_ms_per_freq": round(tm["kernel_ms"] / len(idx), 3),
+                    "wall_ms_per_freq": round(1e3 * wall / len(idx), 2),
+                    "kernel": "fluid_node_gradp_kernel<20> + node_results_kernel",
+                    "bound": "instruction issue (magnitude-ordered sums of 20 terms)",
+                    "note": "wall time includes the host copies of the solutions in and the 9 results per node out"}
+        del Xs, nd
+        log(f"[bench] node results: {node_res}")
+
     per_launch_ms = spmv_ms / max(1, spmv_calls)
     achieved = spmv_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     traffic = PMC_TRAFFIC.get(("row_op_kernel<0>", args.nel)) if B == 32 else None
@@ -206,6 +229,7 @@ def main():
         "iters_mean": float(iters.mean()), "iters_max": int(iters.max()),
         "sweep_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "fp64_baseline": fp64, "roofline": roof, "assembly": assembly, "serving": serving,
+        "node_results": node_res,
         "cpu_baseline": cpu, "config0": config0, "measured_peaks": peaks,
     }
     print(json.dumps(line), flush=True)

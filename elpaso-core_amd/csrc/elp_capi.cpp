@@ -346,14 +346,14 @@ int elp_results_sizes(const elp_model *mm, int64_t *n_nodes, int64_t *n_brick_el
 }
 
 int elp_node_results(elp_model *mm, int64_t n_freq, const double *freqs_hz, const double *X, double *cells,
-                     double *nodes) {
+                     double *nodes, double *kernel_ms) {
   ELP_TRY
   auto &m = M(mm);
   ELP_ARG(n_freq > 0 && freqs_hz && X && (cells || nodes), "invalid arguments");
   ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
   ELP_HIP(hipSetDevice(mm->m->ctx->device));
   std::lock_guard<std::mutex> lk(m.mu);
-  elp::node_results(m, n_freq, freqs_hz, X, cells, nodes);
+  elp::node_results(m, n_freq, freqs_hz, X, cells, nodes, kernel_ms);
   ELP_CATCH
 }
 

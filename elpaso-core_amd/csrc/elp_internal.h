@@ -222,7 +222,8 @@ void element_matrices_batch(Ctx *ctx, int kind, int64_t n, const double *xyz, do
                             double p1, double p2, double *Ke, double *Me);
 void dynamic_matrix(Model &m, double omega, std::complex<double> Z, double *a, double *f);
 // element stresses (bricks) and node results (sigma, grad p) of solutions X
-void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes);
+void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes,
+                  double *kernel_ms);
 int64_t brick_element_count(const Model &m);
 
 }  // namespace elp

@@ -252,7 +252,8 @@ __global__ __launch_bounds__(kPostThreads) void node_results_kernel(int64_t n_no
 
 }  // namespace
 
-void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes) {
+void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes,
+                  double *kernel_ms) {
   ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
   Ctx *c = m.ctx;
   hipStream_t s = c->stream;
@@ -294,6 +295,12 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
   ELP_REQUIRE(!cells || brick_kinds <= 1, ELP_EINVAL,
               "element stresses of a model mixing Brick8 and Brick20 elements are not supported");
   std::vector<double2> scale(chunk);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (kernel_ms) {
+    *kernel_ms = 0.0;
+    ELP_HIP(hipEventCreate(&ev0));
+    ELP_HIP(hipEventCreate(&ev1));
+  }
   for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
     const int nc = (int)std::min(chunk, nf - f0);
     ELP_HIP(hipMemcpyAsync(dX.p, X + 2 * f0 * n, sizeof(double2) * nc * n, hipMemcpyHostToDevice, s));
@@ -305,6 +312,7 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
       scale[f] = double2{1.0, im};
     }
     ELP_HIP(hipMemcpyAsync(dScale.p, scale.data(), sizeof(double2) * nc, hipMemcpyHostToDevice, s));
+    if (kernel_ms) ELP_HIP(hipEventRecord(ev0, s));
     PostKinds K;
     for (size_t k = 0; k < m.kinds.size(); k++) {
       KindBlock &kb = m.kinds[k];
@@ -334,8 +342,10 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
                              m.d_node_row.p, d_T20.p, nc, n, N, dX.p, dN.p);
         ELP_HIP(hipGetLastError());
       }
-      ELP_HIP(hipMemcpyAsync(nodes + 2 * f0 * N * 9, dN.p, sizeof(double2) * nc * N * 9, hipMemcpyDeviceToHost, s));
     }
+    if (kernel_ms) ELP_HIP(hipEventRecord(ev1, s));
+    if (nodes)
+      ELP_HIP(hipMemcpyAsync(nodes + 2 * f0 * N * 9, dN.p, sizeof(double2) * nc * N * 9, hipMemcpyDeviceToHost, s));
     if (cells && brick_cols) {
       // [f][brick element][6]
       for (int f = 0; f < nc; f++) {
@@ -350,7 +360,14 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
       }
     }
     ELP_HIP(hipStreamSynchronize(s));
+    if (kernel_ms) {
+      float ms = 0.f;
+      ELP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+      *kernel_ms += ms;
+    }
   }
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
 }
 
 int64_t brick_element_count(const Model &m) {

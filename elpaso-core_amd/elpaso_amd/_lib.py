@@ -96,7 +96,7 @@ def load_library():
     L.elp_element_matrices.argtypes = [vp, C.c_int, i64, vp, dbl, dbl, dbl, vp, vp]
     L.elp_sweep.argtypes = [vp, i64, vp, dbl, dbl, P(SolveOpts), vp, vp, vp, P(SweepStats)]
     L.elp_results_sizes.argtypes = [vp, P(i64), P(i64)]
-    L.elp_node_results.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.elp_node_results.argtypes = [vp, i64, vp, vp, vp, vp, P(dbl)]
     L.elp_server_create.argtypes = [vp, i64, C.c_int, P(vp)]
     L.elp_server_destroy.argtypes = [vp]
     L.elp_server_destroy.restype = None
@@ -301,10 +301,11 @@ class GpuModel:
         return X, rel, it, st
 
 
-    def node_results(self, X, freqs, cells=True, nodes=True):
+    def node_results(self, X, freqs, cells=True, nodes=True, timing=None):
         """Brick centroid stresses [nf, bricks, 6] and node results [nf,
         nodes, 9] (sigma 0..5, grad p 6..8; nodes in ascending id) of
-        solutions X [nf, n] (elp_node_results)."""
+        solutions X [nf, n] (elp_node_results).  timing: a dict that gets
+        the kernels' device time in "kernel_ms"."""
         L = load_library()
         X = np.ascontiguousarray(X, dtype=np.complex128)
         freqs = np.ascontiguousarray(freqs, dtype=np.float64)
@@ -314,8 +315,11 @@ class GpuModel:
         _check(L.elp_results_sizes(self._h, C.byref(nn), C.byref(nb)))
         c = np.zeros((len(freqs), nb.value, 6), np.complex128) if cells else None
         nd = np.zeros((len(freqs), nn.value, 9), np.complex128) if nodes else None
+        kms = C.c_double()
         _check(L.elp_node_results(self._h, len(freqs), _ptr(freqs), _ptr(X), _ptr(c) if c is not None else None,
-                                  _ptr(nd) if nd is not None else None))
+                                  _ptr(nd) if nd is not None else None, C.byref(kms)))
+        if timing is not None:
+            timing["kernel_ms"] = kms.value
         return c, nd
 
 

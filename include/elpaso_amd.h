@@ -212,9 +212,11 @@ int elp_results_sizes(const elp_model *m, int64_t *n_nodes, int64_t *n_brick_ele
  *    magnitude order); other slots 0.
  * Replaces cAnalysis::computeStressesCells / computeStressesNodes
  * (analysis/analysis.cpp:869-937), which run per frequency on the host.
+ * kernel_ms (may be null): device time of the kernels (HIP events), without
+ * the host copies.
  * ELP_EINVAL for cells of a model mixing Brick8 and Brick20. */
 int elp_node_results(elp_model *m, int64_t n_freq, const double *freqs_hz, const double *X,
-                     double *cells, double *nodes);
+                     double *cells, double *nodes, double *kernel_ms);
 
 /* ---- serving: continuous batching of sweep requests ----------------------------- */
 int elp_server_create(elp_ctx *ctx, int64_t hbm_budget_bytes, int max_slots, elp_server **out);

@@ -24,5 +24,6 @@ def test_flags_of_the_measurement_commands():
     r = _run("--help")
     assert r.returncode == 0
     for flag in ("--gpus", "--steps", "--warmup", "--nfreq", "--fp64-sample", "--no-cpu", "--no-config0",
-                 "--no-peaks", "--serve-requests", "--copy-calibration", "--assembly-nel", "--cpu-procs"):
+                 "--no-peaks", "--serve-requests", "--copy-calibration", "--assembly-nel", "--cpu-procs",
+                 "--node-results"):
         assert flag in r.stdout, flag
