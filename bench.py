@@ -184,6 +184,23 @@ def main():
                     "kernel": "hex8_node_kernel<3, false, true>",
                     "traffic": PMC_TRAFFIC.get(("hex8_node_kernel<3, false, true>", args.assembly_nel))}
         log(f"[bench] assembly: {assembly}")
+        if args.node_results > 0:
+            # brick centroid stresses and their node averages for synthetic
+            # displacements of the block (timing only)
+            rng = np.random.default_rng(0)
+            nfr = args.node_results
+            Xb = rng.standard_normal((nfr, sg.n)) + 1j * rng.standard_normal((nfr, sg.n))
+            fb = np.linspace(10.0, 2000.0, nfr)
+            sg.node_results(Xb[:1], fb[:1])  # warm-up
+            tm = {}
+            t = time.perf_counter()
+            sg.node_results(Xb, fb, timing=tm)
+            wall = time.perf_counter() - t
+            assembly["node_results"] = {
+                "workload": f"brick centroid stresses + node averages, {nfr} synthetic displacement fields",
+                "kernel_ms_per_freq": round(tm["kernel_ms"] / nfr, 4), "wall_ms_per_freq": round(1e3 * wall / nfr, 2),
+                "kernel": "brick_centroid_kernel + node_results_kernel"}
+            del Xb
         sg.close()
 
     if args.copy_calibration:
