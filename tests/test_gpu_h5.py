@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from elpaso_amd import FLUID8, ElementGroup, GpuModel, Material, Model, h5
+from elpaso_amd import BRICK20, FLUID8, ElementGroup, GpuModel, Material, Model, h5
 from elpaso_amd import _lib
 from elpaso_amd import meshgen as MG
 
@@ -130,6 +130,35 @@ def test_coupled_model_written_in_elpaso_layout(ctx, tmp_path):
         a1, f1 = g.dynamic_matrix(2 * np.pi * f)
         a2, f2 = ref.dynamic_matrix(2 * np.pi * f)
         assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("kind", ["fluid20", "brick20"])
+def test_twenty_node_models_round_trip(ctx, tmp_path, kind):
+    print(f"[h5] {kind} model written and read back", flush=True)
+    if kind == "fluid20":
+        m = MG.acoustic_cavity(3, 2, 2, quadratic=True)
+        m.imp_elem_ids = m.imp_faces = None  # Robin faces are not part of elPaSo's input
+        m.box = None
+        m.sources.append((int(m.node_ids[3]), 1.0))
+    else:
+        ids, xyz, eid, conn, _ = MG.box(2, 2, 1, (0.1, 0.1, 0.02), quadratic=True)
+        m = Model(node_ids=ids, xyz=xyz, name="brick20_2x2x1")
+        m.materials[1] = Material("structure", rho=7850.0, E=210e9, nu=0.3, eta=0.01, visco_type=1)
+        m.groups.append(ElementGroup(BRICK20, 1, eid, conn))
+        m.forces.append((int(ids[-1]), 0.0, 0.0, 1.0))
+        m.dirichlet.extend((int(n), d, 0.0) for n in ids[np.isclose(xyz[:, 0], 0.0)] for d in (0, 1, 2))
+    which = (_lib.MAT_K, _lib.MAT_M)
+    path = tmp_path / f"{kind}.hdf5"
+    write_input(path, m)
+    g, _, counts = h5.read_model(ctx, path)
+    assert counts["elements"] == m.n_elements
+    ref = GpuModel(ctx, m)
+    for x in (g, ref):
+        x.assemble()
+    _same_csr(g, ref, which)
+    a1, f1 = g.dynamic_matrix(2 * np.pi * 100.0)
+    a2, f2 = ref.dynamic_matrix(2 * np.pi * 100.0)
+    assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
 
 
 def test_unsupported_content_is_rejected(ctx, tmp_path):
