@@ -116,12 +116,17 @@ __device__ inline void tri_entry(int t, int &a, int &b) {
 // (lambda, mu, rho).  bad receives the smallest element index with detJ <= 0.
 // K and M are symmetric: each lane computes entries of the upper triangle and
 // writes them to both halves.
-template <int NPE, int ND, int NG>
+// ROWS: instead of [element][row][col] blocks, local node a's rows go to the
+// record row_at[e][a] of Ke (records of RS = ND NDF + NPE doubles: its ND rows
+// of K, then its row of M), so that a node's records are contiguous.
+template <int NPE, int ND, int NG, bool ROWS = false>
 __global__ __launch_bounds__(64 * kWaves) void element_kernel(
     const double *__restrict__ xyz, const int32_t *__restrict__ conn, const double *__restrict__ params,
-    int64_t n_elem, double *__restrict__ Ke, double *__restrict__ Me, int *bad, int xyz_direct) {
+    int64_t n_elem, double *__restrict__ Ke, double *__restrict__ Me, int *bad, int xyz_direct,
+    const int32_t *__restrict__ row_at) {
   __shared__ ElemLds<NPE, NG> lds[kWaves];
   __shared__ double sN[NG][NPE];  // shape values at the points, shared by the workgroup's elements
+  __shared__ int32_t sj[kWaves][NPE];
   for (int i = threadIdx.x; i < NG * NPE; i += blockDim.x) sN[i / NPE][i % NPE] = tab_n<NPE>(i / NPE, 0)[i % NPE];
   __syncthreads();
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -131,6 +136,9 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
   for (int t = lane; t < NPE * 3; t += 64) {
     int k = t / 3, d = t % 3;
     L.x[t] = xyz_direct ? xyz[e * NPE * 3 + t] : xyz[(int64_t)conn[e * NPE + k] * 3 + d];
+  }
+  if constexpr (ROWS) {
+    if (lane < NPE) sj[w][lane] = row_at[e * NPE + lane];
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -158,8 +166,15 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
         k += r * wd;
         m += p1 * sN[g][a] * sN[g][b] * wd;
       }
-      Kout[a * NPE + b] = Kout[b * NPE + a] = k / p0;
-      Mout[a * NPE + b] = Mout[b * NPE + a] = m / p0;
+      if constexpr (ROWS) {
+        constexpr int RS = NPE + NPE;
+        double *Ra = Ke + (int64_t)sj[w][a] * RS, *Rb = Ke + (int64_t)sj[w][b] * RS;
+        Ra[b] = Rb[a] = k / p0;
+        Ra[NPE + b] = Rb[NPE + a] = m / p0;
+      } else {
+        Kout[a * NPE + b] = Kout[b * NPE + a] = k / p0;
+        Mout[a * NPE + b] = Mout[b * NPE + a] = m / p0;
+      }
     }
   } else {
     const double lam = p0, mu = p1, rho = p2;
@@ -178,7 +193,13 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
                      L.grad[g][2][a] * L.grad[g][2][b]);
         k += v * L.wdj[g];
       }
-      Kout[row * NDF + col] = Kout[col * NDF + row] = k;
+      if constexpr (ROWS) {
+        constexpr int RS = ND * NDF + NPE;
+        Ke[(int64_t)sj[w][a] * RS + d * NDF + col] = k;
+        Ke[(int64_t)sj[w][b] * RS + dd * NDF + row] = k;
+      } else {
+        Kout[row * NDF + col] = Kout[col * NDF + row] = k;
+      }
     }
     for (int t = lane; t < NPE * (NPE + 1) / 2; t += 64) {
       int a, b;
@@ -186,7 +207,12 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
       double m = 0.0;
 #pragma unroll 3
       for (int g = 0; g < NG; g++) m += rho * sN[g][a] * sN[g][b] * L.wdj[g];
-      Mout[a * NPE + b] = Mout[b * NPE + a] = m;
+      if constexpr (ROWS) {
+        constexpr int RS = ND * NDF + NPE;
+        Ke[(int64_t)sj[w][a] * RS + ND * NDF + b] = Ke[(int64_t)sj[w][b] * RS + ND * NDF + a] = m;
+      } else {
+        Mout[a * NPE + b] = Mout[b * NPE + a] = m;
+      }
     }
   }
 }
@@ -221,10 +247,14 @@ __device__ inline int find_slot(const int32_t *nb, int deg, int32_t j) {
   return lo;
 }
 
-template <int NPE, int ND>
+// ROWS: the element matrices are in node-ordered records (element_kernel's
+// ROWS layout) and adj_slot holds each item's slot, so a batch's items are
+// read contiguously and need neither the element's nodes nor a slot search.
+template <int NPE, int ND, bool ROWS = false>
 __global__ __launch_bounds__(64 * kWaves) void gather_kernel(
     int32_t n_nodes, const int32_t *__restrict__ nodes, const int32_t *__restrict__ adj_ptr,
     const int32_t *__restrict__ adj_elem, const uint8_t *__restrict__ adj_local,
+    const uint8_t *__restrict__ adj_slot,
     const int32_t *__restrict__ conn, const double *__restrict__ Ke, const double *__restrict__ Me,
     const int32_t *__restrict__ nbr_ptr, const int32_t *__restrict__ nbr,
     const int32_t *__restrict__ nbr_off, const int32_t *__restrict__ node_row,
@@ -259,15 +289,27 @@ __global__ __launch_bounds__(64 * kWaves) void gather_kernel(
     // 1. every item of the batch at once: slot and element-matrix block
     for (int it = lane; it < nk * NPE; it += 64) {
       const int kk = it / NPE, b = it % NPE;
-      const int64_t e = adj_elem[pb + kk];
-      const int a = adj_local[pb + kk];
-      L.slot[it] = (int16_t)find_slot(L.nb, deg, conn[e * NPE + b]);
-      const double *kr = Ke + e * (int64_t)NDF * NDF + (int64_t)a * ND * NDF + b * ND;
+      if constexpr (ROWS) {
+        const int64_t j = pb + kk;
+        constexpr int RS = ND * NDF + NPE;
+        L.slot[it] = (int16_t)adj_slot[j * NPE + b];
+        const double *kr = Ke + j * RS + b * ND;
 #pragma unroll
-      for (int d = 0; d < ND; d++)
+        for (int d = 0; d < ND; d++)
 #pragma unroll
-        for (int dd = 0; dd < ND; dd++) L.val[it][d * ND + dd] = kr[d * NDF + dd];
-      L.mv[it] = Me[e * NPE * NPE + a * NPE + b];
+          for (int dd = 0; dd < ND; dd++) L.val[it][d * ND + dd] = kr[d * NDF + dd];
+        L.mv[it] = Ke[j * RS + ND * NDF + b];
+      } else {
+        const int64_t e = adj_elem[pb + kk];
+        const int a = adj_local[pb + kk];
+        L.slot[it] = (int16_t)find_slot(L.nb, deg, conn[e * NPE + b]);
+        const double *kr = Ke + e * (int64_t)NDF * NDF + (int64_t)a * ND * NDF + b * ND;
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+#pragma unroll
+          for (int dd = 0; dd < ND; dd++) L.val[it][d * ND + dd] = kr[d * NDF + dd];
+        L.mv[it] = Me[e * NPE * NPE + a * NPE + b];
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -406,28 +448,39 @@ __global__ void segmented_sum_kernel(int64_t n, const int32_t *__restrict__ ptr,
 
 static int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
+// rows: write the model's node-ordered records (kb.d_row_at) instead of
+// [element][row][col] blocks (assembly of the 20-node kinds)
 static void launch_element(const KindBlock &kb, const double *xyz, int xyz_direct, double *Ke,
-                           double *Me, int *bad, hipStream_t s) {
+                           double *Me, int *bad, hipStream_t s, bool rows = false) {
   const int grid = blocks_for(kb.n_elem, kWaves);
   if (grid == 0) return;
   const int32_t *conn = kb.d_conn.p;
   const double *par = kb.d_params.p;
+  const int32_t *ra = kb.d_row_at.p;
   switch (kb.kind) {
     case ELP_FLUID8:
       hipLaunchKernelGGL((element_kernel<8, 1, 8>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
-                         kb.n_elem, Ke, Me, bad, xyz_direct);
+                         kb.n_elem, Ke, Me, bad, xyz_direct, nullptr);
       break;
     case ELP_BRICK8:
       hipLaunchKernelGGL((element_kernel<8, 3, 8>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
-                         kb.n_elem, Ke, Me, bad, xyz_direct);
+                         kb.n_elem, Ke, Me, bad, xyz_direct, nullptr);
       break;
     case ELP_FLUID20:
-      hipLaunchKernelGGL((element_kernel<20, 1, 27>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
-                         kb.n_elem, Ke, Me, bad, xyz_direct);
+      if (rows)
+        hipLaunchKernelGGL((element_kernel<20, 1, 27, true>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
+                           kb.n_elem, Ke, Me, bad, xyz_direct, ra);
+      else
+        hipLaunchKernelGGL((element_kernel<20, 1, 27>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
+                           kb.n_elem, Ke, Me, bad, xyz_direct, nullptr);
       break;
     default:
-      hipLaunchKernelGGL((element_kernel<20, 3, 27>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
-                         kb.n_elem, Ke, Me, bad, xyz_direct);
+      if (rows)
+        hipLaunchKernelGGL((element_kernel<20, 3, 27, true>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
+                           kb.n_elem, Ke, Me, bad, xyz_direct, ra);
+      else
+        hipLaunchKernelGGL((element_kernel<20, 3, 27>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
+                           kb.n_elem, Ke, Me, bad, xyz_direct, nullptr);
       break;
   }
   ELP_HIP(hipGetLastError());
@@ -438,10 +491,10 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
   const int grid = blocks_for(nn, kWaves);
   if (grid == 0) return;
   double *Cv = m.has_rayleigh ? m.d_C.p : nullptr;
-#define ELP_GATHER_ARGS                                                                          \
-  nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, kb.d_Ke.p,   \
-      kb.d_Me.p, m.d_nbr_ptr.p, m.d_nbr.p, m.d_nbr_off.p, m.d_node_row.p, m.d_rowptr.p, m.d_K.p, \
-      m.d_M.p, Cv, m.ray_a, m.ray_b
+#define ELP_GATHER_ARGS                                                                                          \
+  nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_adj_slot.p, kb.d_conn.p, kb.d_Ke.p, \
+      kb.d_Me.p, m.d_nbr_ptr.p, m.d_nbr.p, m.d_nbr_off.p, m.d_node_row.p, m.d_rowptr.p, m.d_K.p, m.d_M.p, Cv,    \
+      m.ray_a, m.ray_b
   switch (kb.kind) {
     case ELP_FLUID8:
       hipLaunchKernelGGL((gather_kernel<8, 1>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
@@ -450,10 +503,10 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
       hipLaunchKernelGGL((gather_kernel<8, 3>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
       break;
     case ELP_FLUID20:
-      hipLaunchKernelGGL((gather_kernel<20, 1>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
+      hipLaunchKernelGGL((gather_kernel<20, 1, true>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
       break;
     default:
-      hipLaunchKernelGGL((gather_kernel<20, 3>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
+      hipLaunchKernelGGL((gather_kernel<20, 3, true>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
       break;
   }
 #undef ELP_GATHER_ARGS
@@ -924,7 +977,7 @@ static void launch_volume(Model &m, KindBlock &kb, int *bad, hipStream_t s) {
   if (fused_kind(kb)) {
     launch_hex8_node(m, kb, bad, s);
   } else {
-    launch_element(kb, m.d_xyz.p, 0, kb.d_Ke.p, kb.d_Me.p, bad, s);
+    launch_element(kb, m.d_xyz.p, 0, kb.d_Ke.p, kb.d_Me.p, bad, s, true);
     launch_gather(m, kb, s);
   }
 }
@@ -938,9 +991,9 @@ void assemble_model(Model &m, hipStream_t s) {
     if (m.has_rayleigh) m.d_C.alloc(c, m.nnz);
     for (auto &kb : m.kinds) {
       if (fused_kind(kb)) continue;  // no element matrices in HBM
+      // node-ordered records of K rows and M rows (element_kernel's ROWS layout)
       const int64_t ndf = (int64_t)kb.nd * kb.npe;
-      kb.d_Ke.alloc(c, kb.n_elem * ndf * ndf);
-      kb.d_Me.alloc(c, kb.n_elem * (int64_t)kb.npe * kb.npe);
+      kb.d_Ke.alloc(c, kb.n_elem * (ndf * ndf + (int64_t)kb.npe * kb.npe));
     }
     const int64_t fb = m.n_faces * m.nfn * m.nfn + m.n_if * 48;
     m.d_fbuf.alloc(c, fb);

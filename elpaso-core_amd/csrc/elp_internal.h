@@ -136,6 +136,12 @@ struct KindBlock {
   std::vector<int32_t> h8_desc;
   std::vector<uint8_t> adj_slot;
   int64_t h8_simple = 0;
+  // 20-node kinds: adj_slot holds the neighbour slot of each of the element's
+  // 20 nodes per adjacency entry, and row_at (per element and local node) the
+  // adjacency entry whose record receives that node's rows of the element
+  // matrices (records in node order, so the gather reads them contiguously)
+  std::vector<int32_t> row_at;
+  DArr<int32_t> d_row_at;
   DArr<int32_t> d_conn, d_nodes, d_adj_ptr, d_adj_elem, d_h8_desc;
   DArr<uint8_t> d_adj_local, d_adj_slot;
   DArr<double> d_params;

@@ -332,6 +332,30 @@ void finalize_model(Model &m) {
       }
     });
   }
+  // ---- 20-node kinds: neighbour slots per adjacency entry and the record of
+  // each (element, local node) (see KindBlock::row_at)
+  for (auto &kb : m.kinds) {
+    if (kb.npe != 20) continue;
+    const int64_t ns = (int64_t)kb.nodes.size();
+    kb.adj_slot.assign(kb.adj_elem.size() * 20, 0);
+    kb.row_at.assign(kb.n_elem * 20, -1);
+    parallel_for(ns, [&](int64_t a, int64_t b) {
+      for (int64_t s = a; s < b; s++) {
+        const int32_t i = kb.nodes[s];
+        const int32_t *nb = m.nbr.data() + m.nbr_ptr[i];
+        const int32_t dg = m.nbr_ptr[i + 1] - m.nbr_ptr[i];
+        for (int32_t p = kb.adj_ptr[s]; p < kb.adj_ptr[s + 1]; p++) {
+          const int64_t e = kb.adj_elem[p];
+          kb.row_at[e * 20 + kb.adj_local[p]] = p;  // each (element, local node) once
+          for (int q = 0; q < 20; q++) {
+            const int32_t j = kb.conn[e * 20 + q];
+            kb.adj_slot[(int64_t)p * 20 + q] = (uint8_t)(std::lower_bound(nb, nb + dg, j) - nb);
+          }
+        }
+      }
+    });
+    for (int32_t v : kb.row_at) ELP_REQUIRE(v >= 0, ELP_EINVAL, "element node without adjacency entry");
+  }
   auto full_pos = [&](int32_t r, int32_t c) -> int32_t {
     auto b = m.colind.begin() + m.rowptr[r], e = m.colind.begin() + m.rowptr[r + 1];
     auto it = std::lower_bound(b, e, c);
@@ -466,6 +490,7 @@ void finalize_model(Model &m) {
     kb.d_adj_local.upload(c, kb.adj_local);
     kb.d_h8_desc.upload(c, kb.h8_desc);
     kb.d_adj_slot.upload(c, kb.adj_slot);
+    kb.d_row_at.upload(c, kb.row_at);
     kb.d_params.upload(c, kb.params);
   }
   m.d_x_rowptr.upload(c, m.x_rowptr);
@@ -487,6 +512,7 @@ int64_t model_device_bytes_estimate(const Model &m) {
   for (auto &kb : m.kinds) {
     int64_t nd = (int64_t)kb.nd * kb.npe;
     b += kb.n_elem * (nd * nd + (int64_t)kb.npe * kb.npe) * 8;
+    if (kb.npe == 20) b += kb.n_elem * 20 * (20 + 4);  // neighbour slots, record positions
   }
   return b;
 }
