@@ -153,7 +153,69 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
   constexpr int NDF = NPE * ND;
   double *Kout = Ke + e * (int64_t)NDF * NDF;
   double *Mout = Me + e * (int64_t)NPE * NPE;
-  if constexpr (ND == 1) {
+  if constexpr (ND == 1 && NPE == 20 && ROWS) {
+    // K = G diag(w) G^T and M = N diag(p1 w) N^T on the FP64 matrix cores:
+    // G is 20 x 81 (gradient component i at point g in column 3 g + i), N is
+    // 20 x 27.  v_mfma_f64_16x16x4f64 tiles: lane l holds A[l % 16][l / 16]
+    // and B[l / 16][l % 16] and receives D[l / 16 + 4 r][l % 16] in element r.
+    // Tiles (0,0), (0,1), (1,1) of the padded 32 x 32 result (rows 20..31
+    // zero); each entry a <= b is taken from one product and mirrored, so the
+    // result is exactly symmetric.
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    const int li = lane % 16, lk = lane / 16;
+    v4d k00 = {0, 0, 0, 0}, k01 = {0, 0, 0, 0}, k11 = {0, 0, 0, 0};
+    v4d m00 = {0, 0, 0, 0}, m01 = {0, 0, 0, 0}, m11 = {0, 0, 0, 0};
+#pragma unroll 3
+    for (int st = 0; st < 21; st++) {
+      const int kk = 4 * st + lk;
+      const bool kv = kk < 3 * NG;
+      const int g = kv ? kk / 3 : 0, i = kk % 3;
+      const double wk = kv ? L.wdj[g] : 0.0;
+      const double a0 = kv ? L.grad[g][i][li] : 0.0;
+      const double a1 = (kv && li < 4) ? L.grad[g][i][16 + li] : 0.0;
+      k00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a0, k00, 0, 0, 0);
+      k01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a1, k01, 0, 0, 0);
+      k11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, wk * a1, k11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int st = 0; st < 7; st++) {
+      const int g = 4 * st + lk;
+      const bool gv = g < NG;
+      const double wk = gv ? p1 * L.wdj[g] : 0.0;
+      const double a0 = gv ? sN[g][li] : 0.0;
+      const double a1 = (gv && li < 4) ? sN[g][16 + li] : 0.0;
+      m00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a0, m00, 0, 0, 0);
+      m01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a1, m01, 0, 0, 0);
+      m11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, wk * a1, m11, 0, 0, 0);
+    }
+    // stage the symmetric K and M in the wavefront's LDS (the gradients are
+    // no longer needed), then write each local node's record contiguously
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double(*Kt)[NPE] = reinterpret_cast<double(*)[NPE]>(&L.grad[0][0][0]);
+    double(*Mt)[NPE] = Kt + NPE;
+    static_assert(2 * NPE * NPE <= NG * 3 * NPE, "staging tile exceeds the gradient area");
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int io = lk + 4 * r;  // row within the tile
+      auto put = [&](int a, int b, double kv, double mv) {
+        Kt[a][b] = Kt[b][a] = kv;
+        Mt[a][b] = Mt[b][a] = mv;
+      };
+      if (io <= li) put(io, li, k00[r] / p0, m00[r] / p0);
+      if (li < 4) put(io, 16 + li, k01[r] / p0, m01[r] / p0);
+      if (io < 4 && li < 4 && io <= li) put(16 + io, 16 + li, k11[r] / p0, m11[r] / p0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int RS = NPE + NPE;
+    for (int t = lane; t < NPE * RS; t += 64) {
+      const int a = t / RS, c = t % RS;
+      Ke[(int64_t)sj[w][a] * RS + c] = c < NPE ? Kt[a][c] : Mt[a][c - NPE];
+    }
+  } else if constexpr (ND == 1) {
     for (int t = lane; t < NPE * (NPE + 1) / 2; t += 64) {
       int a, b;
       tri_entry<NPE>(t, a, b);
