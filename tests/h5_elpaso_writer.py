@@ -120,8 +120,13 @@ class _File:
 _KIND_NAMES = {1: "Fluid8", 2: "Brick8", 3: "Fluid20", 4: "Brick20"}
 
 
-def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6):
-    """Writes model (elpaso_amd.Model) as an elPaSo input file."""
+def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6, element_type=None,
+                material_type=None, elem_load=False, interface_type="InterfaceMindlin"):
+    """Writes model (elpaso_amd.Model) as an elPaSo input file.  element_type
+    / material_type replace the first group's ElementType / the first
+    material's MaterialType, elem_load adds a "vn" element load, and
+    interface_type names the interface elements' type (content the library
+    must reject)."""
     F = _File(path)
     g = F.group("/Analysis")
     F.attr(g, "type", "frequency-basic")
@@ -151,7 +156,9 @@ def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6):
         d = F.empty(g, f"material{mid}")
         F.attr(d, "Id", ("u64", mid))
         F.attr(d, "Name", "name")
-        if m.kind == "fluid":
+        if material_type is not None and mid == next(iter(model.materials)):
+            F.attr(d, "MaterialType", material_type)
+        elif m.kind == "fluid":
             F.attr(d, "MaterialType", "AF_LIN_UAF_ISO_DIR")
             F.attr(d, "c", repr(float(m.c)))
             F.attr(d, "rho", repr(float(m.rho)))
@@ -171,12 +178,19 @@ def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6):
         rows = np.concatenate([np.asarray(eg.elem_ids, np.uint64)[:, None], np.asarray(eg.conn, np.uint64)], axis=1)
         rows = np.ascontiguousarray(rows)
         d = F.dataset(g, f"mtxFemElemGroup{k + 1}", _t("H5T_STD_U64LE_g"), _t("H5T_NATIVE_ULLONG_g"), rows)
-        F.attr(d, "ElementType", _KIND_NAMES[eg.kind])
+        F.attr(d, "ElementType", element_type if (element_type and k == 0) else _KIND_NAMES[eg.kind])
         F.attr(d, "Id", ("u64", k + 1))
         F.attr(d, "MaterialId", ("i64", eg.material_id))
         F.attr(d, "N", ("u64", len(eg.elem_ids)))
         F.attr(d, "Name", f"Block_{k + 1}")
-    F.group("/ElemLoads")
+    g = F.group("/ElemLoads")
+    if elem_load:
+        d = F.empty(g, "elemLoad1")
+        F.attr(d, "LoadType", "vn")
+        F.attr(d, "Id", ("u64", 1))
+        F.attr(d, "ElementId", ("u64", int(model.groups[0].elem_ids[0])))
+        F.attr(d, "Face", ("u64", 1))
+        F.attr(d, "vn", ("f64", 1.0))
     # constraints: one dataset per (node, dof) with the flag and value of that dof
     flags = ["u1", "u2", "u3", "w1", "w2", "w3", "p0", "p1", "p2", "p3", "xd3", "wd1", "wd2", "dwdx", "dwdy",
              "dwdxy", "fluid", "z_1", "z_3", "x1_2", "x2_2"]
@@ -221,7 +235,7 @@ def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6):
                                np.asarray(model.if_fluid_nodes, np.int64)], axis=1)
         rows = np.ascontiguousarray(rows)
         d = F.dataset(g, "InterfaceElements0", _t("H5T_STD_I64LE_g"), _t("H5T_NATIVE_LLONG_g"), rows)
-        F.attr(d, "type", "InterfaceMindlin")
+        F.attr(d, "type", interface_type)
         F.attr(d, "NfluidNodes", ("u64", 4))
         F.attr(d, "NstructNodes", ("u64", 4))
         F.attr(d, "matF", ("i64", 1))

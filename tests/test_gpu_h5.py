@@ -161,12 +161,26 @@ def test_twenty_node_models_round_trip(ctx, tmp_path, kind):
     assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
 
 
-def test_unsupported_content_is_rejected(ctx, tmp_path):
-    m = MG.acoustic_cavity(2, 2, 2)
-    m.materials[1] = Material("fluid", rho=1.21, c=343.0)
-    path = tmp_path / "bad.hdf5"
-    write_input(path, m, revision=5)
-    with pytest.raises(_lib.ElpError, match="revision 5"):
+@pytest.mark.parametrize("case,match", [
+    ("revision", "revision 5"),
+    ("element", "element type DSG4 is not supported"),
+    ("material", "material type pointmass is not supported"),
+    ("elemload", "element loads are not supported"),
+    ("interface", "interface type InterfaceKirch is not supported"),
+])
+def test_unsupported_content_is_rejected(ctx, tmp_path, case, match):
+    """Content the library does not assemble is an error naming the item,
+    never dropped."""
+    if case == "interface":
+        m = MG.coupled_plate_cavity(n_xy=2, n_plate_z=1, n_cav_z=2)
+    else:
+        m = MG.acoustic_cavity(2, 2, 2)
+    path = tmp_path / f"bad_{case}.hdf5"
+    kw = {"revision": {"revision": 5}, "element": {"element_type": "DSG4"},
+          "material": {"material_type": "pointmass"}, "elemload": {"elem_load": True},
+          "interface": {"interface_type": "InterfaceKirch"}}[case]
+    write_input(path, m, **kw)
+    with pytest.raises(_lib.ElpError, match=match):
         h5.read_model(ctx, path)
 
 
