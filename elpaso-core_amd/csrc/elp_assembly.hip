@@ -153,69 +153,7 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
   constexpr int NDF = NPE * ND;
   double *Kout = Ke + e * (int64_t)NDF * NDF;
   double *Mout = Me + e * (int64_t)NPE * NPE;
-  if constexpr (ND == 1 && NPE == 20 && ROWS) {
-    // K = G diag(w) G^T and M = N diag(p1 w) N^T on the FP64 matrix cores:
-    // G is 20 x 81 (gradient component i at point g in column 3 g + i), N is
-    // 20 x 27.  v_mfma_f64_16x16x4f64 tiles: lane l holds A[l % 16][l / 16]
-    // and B[l / 16][l % 16] and receives D[l / 16 + 4 r][l % 16] in element r.
-    // Tiles (0,0), (0,1), (1,1) of the padded 32 x 32 result (rows 20..31
-    // zero); each entry a <= b is taken from one product and mirrored, so the
-    // result is exactly symmetric.
-    typedef double v4d __attribute__((ext_vector_type(4)));
-    const int li = lane % 16, lk = lane / 16;
-    v4d k00 = {0, 0, 0, 0}, k01 = {0, 0, 0, 0}, k11 = {0, 0, 0, 0};
-    v4d m00 = {0, 0, 0, 0}, m01 = {0, 0, 0, 0}, m11 = {0, 0, 0, 0};
-#pragma unroll 3
-    for (int st = 0; st < 21; st++) {
-      const int kk = 4 * st + lk;
-      const bool kv = kk < 3 * NG;
-      const int g = kv ? kk / 3 : 0, i = kk % 3;
-      const double wk = kv ? L.wdj[g] : 0.0;
-      const double a0 = kv ? L.grad[g][i][li] : 0.0;
-      const double a1 = (kv && li < 4) ? L.grad[g][i][16 + li] : 0.0;
-      k00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a0, k00, 0, 0, 0);
-      k01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a1, k01, 0, 0, 0);
-      k11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, wk * a1, k11, 0, 0, 0);
-    }
-#pragma unroll
-    for (int st = 0; st < 7; st++) {
-      const int g = 4 * st + lk;
-      const bool gv = g < NG;
-      const double wk = gv ? p1 * L.wdj[g] : 0.0;
-      const double a0 = gv ? sN[g][li] : 0.0;
-      const double a1 = (gv && li < 4) ? sN[g][16 + li] : 0.0;
-      m00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a0, m00, 0, 0, 0);
-      m01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a1, m01, 0, 0, 0);
-      m11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, wk * a1, m11, 0, 0, 0);
-    }
-    // stage the symmetric K and M in the wavefront's LDS (the gradients are
-    // no longer needed), then write each local node's record contiguously
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    double(*Kt)[NPE] = reinterpret_cast<double(*)[NPE]>(&L.grad[0][0][0]);
-    double(*Mt)[NPE] = Kt + NPE;
-    static_assert(2 * NPE * NPE <= NG * 3 * NPE, "staging tile exceeds the gradient area");
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int io = lk + 4 * r;  // row within the tile
-      auto put = [&](int a, int b, double kv, double mv) {
-        Kt[a][b] = Kt[b][a] = kv;
-        Mt[a][b] = Mt[b][a] = mv;
-      };
-      if (io <= li) put(io, li, k00[r] / p0, m00[r] / p0);
-      if (li < 4) put(io, 16 + li, k01[r] / p0, m01[r] / p0);
-      if (io < 4 && li < 4 && io <= li) put(16 + io, 16 + li, k11[r] / p0, m11[r] / p0);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    constexpr int RS = NPE + NPE;
-    for (int t = lane; t < NPE * RS; t += 64) {
-      const int a = t / RS, c = t % RS;
-      Ke[(int64_t)sj[w][a] * RS + c] = c < NPE ? Kt[a][c] : Mt[a][c - NPE];
-    }
-  } else if constexpr (ND == 1) {
+  if constexpr (ND == 1) {
     for (int t = lane; t < NPE * (NPE + 1) / 2; t += 64) {
       int a, b;
       tri_entry<NPE>(t, a, b);
@@ -276,6 +214,123 @@ __global__ __launch_bounds__(64 * kWaves) void element_kernel(
         Mout[a * NPE + b] = Mout[b * NPE + a] = m;
       }
     }
+  }
+}
+
+// Fluid20 element matrices of the assembly on the FP64 matrix cores, in the
+// node-ordered records of element_kernel's ROWS layout: K = G diag(w) G^T
+// and M = N diag(p1 w) N^T, G (20 x 81) the gradient components at the 27
+// points (column 3 g + i), N (20 x 27) the shape values, w = w_g detJ_g.
+// Only the 27 inverse Jacobians are kept per element (2.7 KB of LDS per
+// wavefront, the shape tables are shared by the workgroup); each MFMA
+// operand G[a][3 g + i] = sum_j invJ_g(i, j) dN_a/dxi_j is formed in the
+// loop.  v_mfma_f64_16x16x4f64 tiles: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16] and receives D[l / 16 + 4 r][l % 16] in element r.
+// Tiles (0,0), (0,1), (1,1) of the padded 32 x 32 result; each entry a <= b
+// comes from one product and is written to both halves (exactly symmetric).
+struct Fluid20Lds {
+  double x[20 * 3];
+  double wdj[27];
+  double ij[27][9];
+  int32_t rec[20];
+};
+
+__global__ __launch_bounds__(64 * kWaves) void fluid20_mfma_kernel(
+    const double *__restrict__ xyz, const int32_t *__restrict__ conn, const double *__restrict__ params,
+    int64_t n_elem, double *__restrict__ R, int *bad, const int32_t *__restrict__ row_at) {
+  constexpr int NPE = 20, NG = 27, RS = 2 * NPE;
+  __shared__ Fluid20Lds lds[kWaves];
+  __shared__ double sN[NG][NPE];
+  __shared__ double sD[NG][3][NPE];
+  for (int i = threadIdx.x; i < NG * NPE; i += blockDim.x) sN[i / NPE][i % NPE] = tab_n<NPE>(i / NPE, 0)[i % NPE];
+  for (int i = threadIdx.x; i < NG * 3 * NPE; i += blockDim.x) {
+    const int g = i / (3 * NPE), d = (i / NPE) % 3, k = i % NPE;
+    sD[g][d][k] = tab_n<NPE>(g, d + 1)[k];
+  }
+  __syncthreads();
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t e = (int64_t)blockIdx.x * kWaves + w;
+  if (e >= n_elem) return;  // whole wavefront leaves together
+  Fluid20Lds &L = lds[w];
+  for (int t = lane; t < NPE * 3; t += 64) L.x[t] = xyz[(int64_t)conn[e * NPE + t / 3] * 3 + t % 3];
+  if (lane < NPE) L.rec[lane] = row_at[e * NPE + lane];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < NG) {
+    // Jacobian, Sarrus determinant, cofactor inverse as point_geometry
+    const int g = lane;
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+    for (int k = 0; k < NPE; k++) {
+      const double x0 = L.x[3 * k], x1 = L.x[3 * k + 1], x2 = L.x[3 * k + 2];
+      const double a = sD[g][0][k], b = sD[g][1][k], c = sD[g][2][k];
+      J[0] += a * x0; J[1] += a * x1; J[2] += a * x2;
+      J[3] += b * x0; J[4] += b * x1; J[5] += b * x2;
+      J[6] += c * x0; J[7] += c * x1; J[8] += c * x2;
+    }
+    const double det = J[0] * J[4] * J[8] + J[1] * J[5] * J[6] + J[2] * J[3] * J[7] -
+                       J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
+    double *iJ = L.ij[g];
+    iJ[0] = (J[4] * J[8] - J[5] * J[7]) / det;
+    iJ[3] = -(J[3] * J[8] - J[5] * J[6]) / det;
+    iJ[6] = (J[3] * J[7] - J[4] * J[6]) / det;
+    iJ[1] = -(J[1] * J[8] - J[2] * J[7]) / det;
+    iJ[4] = (J[0] * J[8] - J[2] * J[6]) / det;
+    iJ[7] = -(J[0] * J[7] - J[1] * J[6]) / det;
+    iJ[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+    iJ[5] = -(J[0] * J[5] - J[2] * J[3]) / det;
+    iJ[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+    L.wdj[g] = tab_w<NPE>(g) * det;
+    if (!(det > 0.0)) atomicMin(bad, (int)e);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double p0 = params[e * 4], p1 = params[e * 4 + 1];
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  const int li = lane % 16, lk = lane / 16;
+  v4d k00 = {0, 0, 0, 0}, k01 = {0, 0, 0, 0}, k11 = {0, 0, 0, 0};
+  v4d m00 = {0, 0, 0, 0}, m01 = {0, 0, 0, 0}, m11 = {0, 0, 0, 0};
+#pragma unroll 3
+  for (int st = 0; st < 21; st++) {
+    const int kk = 4 * st + lk;
+    const bool kv = kk < 3 * NG;
+    const int g = kv ? kk / 3 : 0, i = kk % 3;
+    const double *q = &L.ij[g][3 * i];
+    const double wk = kv ? L.wdj[g] : 0.0;
+    const double a0 = kv ? q[0] * sD[g][0][li] + q[1] * sD[g][1][li] + q[2] * sD[g][2][li] : 0.0;
+    const int l1 = li < 4 ? 16 + li : li;  // in-range index; the value is zeroed below
+    const double a1v = q[0] * sD[g][0][l1] + q[1] * sD[g][1][l1] + q[2] * sD[g][2][l1];
+    const double a1 = (kv && li < 4) ? a1v : 0.0;
+    k00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a0, k00, 0, 0, 0);
+    k01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a1, k01, 0, 0, 0);
+    k11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, wk * a1, k11, 0, 0, 0);
+  }
+#pragma unroll
+  for (int st = 0; st < 7; st++) {
+    const int g = 4 * st + lk;
+    const bool gv = g < NG;
+    const int gc = gv ? g : 0;
+    const double wk = gv ? p1 * L.wdj[gc] : 0.0;
+    const int l1 = li < 4 ? 16 + li : li;  // in-range index; the value is zeroed below
+    const double a0 = gv ? sN[gc][li] : 0.0;
+    const double a1 = (gv && li < 4) ? sN[gc][l1] : 0.0;
+    m00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a0, m00, 0, 0, 0);
+    m01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, wk * a1, m01, 0, 0, 0);
+    m11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, wk * a1, m11, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int io = lk + 4 * r;  // row within the tile
+    auto put = [&](int a, int b, double kv, double mv) {
+      double *Ra = R + (int64_t)L.rec[a] * RS, *Rb = R + (int64_t)L.rec[b] * RS;
+      Ra[b] = Rb[a] = kv;
+      Ra[NPE + b] = Rb[NPE + a] = mv;
+    };
+    if (io <= li) put(io, li, k00[r] / p0, m00[r] / p0);
+    if (li < 4) put(io, 16 + li, k01[r] / p0, m01[r] / p0);
+    if (io < 4 && li < 4 && io <= li) put(16 + io, 16 + li, k11[r] / p0, m11[r] / p0);
   }
 }
 
@@ -530,8 +585,8 @@ static void launch_element(const KindBlock &kb, const double *xyz, int xyz_direc
       break;
     case ELP_FLUID20:
       if (rows)
-        hipLaunchKernelGGL((element_kernel<20, 1, 27, true>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
-                           kb.n_elem, Ke, Me, bad, xyz_direct, ra);
+        hipLaunchKernelGGL(fluid20_mfma_kernel, dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par, kb.n_elem, Ke,
+                           bad, ra);
       else
         hipLaunchKernelGGL((element_kernel<20, 1, 27>), dim3(grid), dim3(64 * kWaves), 0, s, xyz, conn, par,
                            kb.n_elem, Ke, Me, bad, xyz_direct, nullptr);
