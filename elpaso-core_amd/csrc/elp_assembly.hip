@@ -334,136 +334,91 @@ __global__ __launch_bounds__(64 * kWaves) void fluid20_mfma_kernel(
   }
 }
 
-// Deterministic gather of element rows into the node's CSR rows (K, M and
-// optionally C = alpha K + beta M), one wavefront per node.  The node's
-// adjacent elements are taken in batches: the lanes load every (element,
-// neighbour) item of a batch at once (its slot in the node's rows, its block
-// of the element matrices), then the blocks are added into the rows in LDS
-// element by element, in ascending element id.
+// Deterministic gather of the element rows into the node's CSR rows (K, M
+// and optionally C = alpha K + beta M), one wavefront per node of a 20-node
+// kind.  The node's records (element_kernel's / fluid20_mfma_kernel's ROWS
+// layout) are contiguous, in ascending element id, with the neighbour slot of
+// each record column precomputed (adj_slot).  Each lane owns neighbour slots
+// lane, lane + 64, ...: per batch of up to 8 records the wavefront builds the
+// inverse map (record, slot) -> record column in LDS, then every lane adds
+// its slots' values record by record, in ascending element id, in registers,
+// and finally writes its slots' columns of the node's rows.
 constexpr int kMaxCols = 256;
+constexpr int kGatherBatch = 8;
 template <int NPE, int ND>
-struct GatherLds {
-  static constexpr int BK = ND == 1 ? 8 : 4;  // elements per batch
-  static constexpr int NI = BK * NPE;         // (element, neighbour) items per batch
-  double k[ND][kMaxCols];
-  double m[kMaxCols];
-  int32_t nb[kMaxCols];
-  int32_t off[kMaxCols + 1];
-  double val[NI][ND * ND];  // item (k, b): rows of the node's DOFs, columns of node b's
-  double mv[NI];
-  int16_t slot[NI];
-};
-
-__device__ inline int find_slot(const int32_t *nb, int deg, int32_t j) {
-  int lo = 0, hi = deg - 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (nb[mid] < j) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// ROWS: the element matrices are in node-ordered records (element_kernel's
-// ROWS layout) and adj_slot holds each item's slot, so a batch's items are
-// read contiguously and need neither the element's nodes nor a slot search.
-template <int NPE, int ND, bool ROWS = false>
 __global__ __launch_bounds__(64 * kWaves) void gather_kernel(
     int32_t n_nodes, const int32_t *__restrict__ nodes, const int32_t *__restrict__ adj_ptr,
-    const int32_t *__restrict__ adj_elem, const uint8_t *__restrict__ adj_local,
-    const uint8_t *__restrict__ adj_slot,
-    const int32_t *__restrict__ conn, const double *__restrict__ Ke, const double *__restrict__ Me,
-    const int32_t *__restrict__ nbr_ptr, const int32_t *__restrict__ nbr,
-    const int32_t *__restrict__ nbr_off, const int32_t *__restrict__ node_row,
-    const int32_t *__restrict__ rowptr, double *__restrict__ Kv, double *__restrict__ Mv,
-    double *__restrict__ Cv, double alpha, double beta) {
-  __shared__ GatherLds<NPE, ND> lds[kWaves];
+    const uint8_t *__restrict__ adj_slot, const double *__restrict__ R, const int32_t *__restrict__ nbr_ptr,
+    const int32_t *__restrict__ nbr_off, const int32_t *__restrict__ node_row, const int32_t *__restrict__ rowptr,
+    double *__restrict__ Kv, double *__restrict__ Mv, double *__restrict__ Cv, double alpha, double beta) {
+  constexpr int NDF = NPE * ND, RS = ND * NDF + NPE, SP = kMaxCols / 64;
+  __shared__ uint8_t inv[kWaves][kGatherBatch][kMaxCols];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int32_t s_node = blockIdx.x * kWaves + w;
   if (s_node >= n_nodes) return;
-  GatherLds<NPE, ND> &L = lds[w];
   const int32_t i = nodes[s_node];
   const int32_t nb0 = nbr_ptr[i], deg = nbr_ptr[i + 1] - nb0;
-  for (int s = lane; s < deg; s += 64) {
-    L.nb[s] = nbr[nb0 + s];
-    L.off[s] = nbr_off[nb0 + s];
-  }
   const int32_t r0 = node_row[i];
   const int32_t len = rowptr[r0 + 1] - rowptr[r0];
-  if (lane == 0) L.off[deg] = len;
-  for (int q = lane; q < len; q += 64) {
-#pragma unroll
-    for (int d = 0; d < ND; d++) L.k[d][q] = 0.0;
-  }
-  for (int s = lane; s < deg; s += 64) L.m[s] = 0.0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  constexpr int NDF = NPE * ND, BK = GatherLds<NPE, ND>::BK;
   const int32_t pa0 = adj_ptr[s_node], pa1 = adj_ptr[s_node + 1];
-  for (int32_t pb = pa0; pb < pa1; pb += BK) {
-    const int nk = min(BK, pa1 - pb);
-    // 1. every item of the batch at once: slot and element-matrix block
-    for (int it = lane; it < nk * NPE; it += 64) {
-      const int kk = it / NPE, b = it % NPE;
-      if constexpr (ROWS) {
-        const int64_t j = pb + kk;
-        constexpr int RS = ND * NDF + NPE;
-        L.slot[it] = (int16_t)adj_slot[j * NPE + b];
-        const double *kr = Ke + j * RS + b * ND;
+  double ak[SP][ND][ND], am[SP];
+#pragma unroll
+  for (int p = 0; p < SP; p++) {
+    am[p] = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++)
+#pragma unroll
+      for (int dd = 0; dd < ND; dd++) ak[p][d][dd] = 0.0;
+  }
+  for (int32_t pb = pa0; pb < pa1; pb += kGatherBatch) {
+    const int nk = min(kGatherBatch, pa1 - pb);
+    for (int t = lane; t < nk * kMaxCols; t += 64) inv[w][t / kMaxCols][t % kMaxCols] = 0xff;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // the slots of one record are distinct
+    for (int it = lane; it < nk * NPE; it += 64) inv[w][it / NPE][adj_slot[(int64_t)pb * NPE + it]] = (uint8_t)(it % NPE);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int kk = 0; kk < nk; kk++) {
+      const double *rec = R + (int64_t)(pb + kk) * RS;
+#pragma unroll
+      for (int p = 0; p < SP; p++) {
+        const int sl = lane + 64 * p;
+        if (sl >= deg) continue;
+        const int b = inv[w][kk][sl];
+        if (b == 0xff) continue;
 #pragma unroll
         for (int d = 0; d < ND; d++)
 #pragma unroll
-          for (int dd = 0; dd < ND; dd++) L.val[it][d * ND + dd] = kr[d * NDF + dd];
-        L.mv[it] = Ke[j * RS + ND * NDF + b];
-      } else {
-        const int64_t e = adj_elem[pb + kk];
-        const int a = adj_local[pb + kk];
-        L.slot[it] = (int16_t)find_slot(L.nb, deg, conn[e * NPE + b]);
-        const double *kr = Ke + e * (int64_t)NDF * NDF + (int64_t)a * ND * NDF + b * ND;
-#pragma unroll
-        for (int d = 0; d < ND; d++)
-#pragma unroll
-          for (int dd = 0; dd < ND; dd++) L.val[it][d * ND + dd] = kr[d * NDF + dd];
-        L.mv[it] = Me[e * NPE * NPE + a * NPE + b];
+          for (int dd = 0; dd < ND; dd++) ak[p][d][dd] += rec[d * NDF + b * ND + dd];
+        am[p] += rec[ND * NDF + b];
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // 2. ordered accumulation: element by element (the items of one element
-    //    go to distinct slots)
-    for (int kk = 0; kk < nk; kk++) {
-      for (int u = lane; u < NPE * ND * ND; u += 64) {
-        const int b = u / (ND * ND), c = u % (ND * ND);
-        const int it = kk * NPE + b, sl = L.slot[it];
-        L.k[c / ND][L.off[sl] + c % ND] += L.val[it][c];
-        if (c == 0) L.m[sl] += L.mv[it];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
   }
-  for (int d = 0; d < ND; d++) {
-    const int64_t base = rowptr[r0 + d];
-    for (int q = lane; q < len; q += 64) {
-      // slot of column q and the DOF within the neighbour
-      const int s = [&] {
-        int lo = 0, hi = deg - 1;
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (L.off[mid] <= q) lo = mid;
-          else hi = mid - 1;
-        }
-        return lo;
-      }();
-      const int dd = q - L.off[s];
-      const double kv = L.k[d][q];
-      const double mv = (dd == d) ? L.m[s] : 0.0;
-      Kv[base + q] = kv;
-      Mv[base + q] = mv;
-      if (Cv) Cv[base + q] = alpha * kv + beta * mv;
+#pragma unroll
+  for (int p = 0; p < SP; p++) {
+    const int sl = lane + 64 * p;
+    if (sl >= deg) continue;
+    const int32_t o0 = nbr_off[nb0 + sl], o1 = sl + 1 < deg ? nbr_off[nb0 + sl + 1] : len;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const int64_t base = rowptr[r0 + d];
+      for (int32_t q = o0; q < o1; q++) {
+        const int dd = q - o0;  // DOF within the neighbour
+        double kv = 0.0;
+#pragma unroll
+        for (int e2 = 0; e2 < ND; e2++)
+          if (dd == e2) kv = ak[p][d][e2];
+        const double mv = (dd == d) ? am[p] : 0.0;
+        Kv[base + q] = kv;
+        Mv[base + q] = mv;
+        if (Cv) Cv[base + q] = alpha * kv + beta * mv;
+      }
     }
   }
 }
@@ -609,23 +564,12 @@ static void launch_gather(Model &m, KindBlock &kb, hipStream_t s) {
   if (grid == 0) return;
   double *Cv = m.has_rayleigh ? m.d_C.p : nullptr;
 #define ELP_GATHER_ARGS                                                                                          \
-  nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_adj_slot.p, kb.d_conn.p, kb.d_Ke.p, \
-      kb.d_Me.p, m.d_nbr_ptr.p, m.d_nbr.p, m.d_nbr_off.p, m.d_node_row.p, m.d_rowptr.p, m.d_K.p, m.d_M.p, Cv,    \
-      m.ray_a, m.ray_b
-  switch (kb.kind) {
-    case ELP_FLUID8:
-      hipLaunchKernelGGL((gather_kernel<8, 1>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
-      break;
-    case ELP_BRICK8:
-      hipLaunchKernelGGL((gather_kernel<8, 3>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
-      break;
-    case ELP_FLUID20:
-      hipLaunchKernelGGL((gather_kernel<20, 1, true>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
-      break;
-    default:
-      hipLaunchKernelGGL((gather_kernel<20, 3, true>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
-      break;
-  }
+  nn, kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_slot.p, kb.d_Ke.p, m.d_nbr_ptr.p, m.d_nbr_off.p, m.d_node_row.p, \
+      m.d_rowptr.p, m.d_K.p, m.d_M.p, Cv, m.ray_a, m.ray_b
+  if (kb.kind == ELP_FLUID20)
+    hipLaunchKernelGGL((gather_kernel<20, 1>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
+  else
+    hipLaunchKernelGGL((gather_kernel<20, 3>), dim3(grid), dim3(64 * kWaves), 0, s, ELP_GATHER_ARGS);
 #undef ELP_GATHER_ARGS
   ELP_HIP(hipGetLastError());
 }
