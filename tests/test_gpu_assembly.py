@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from elpaso_amd import BRICK8, BRICK20, FLUID8, FLUID20, GpuModel
+from elpaso_amd import BRICK8, BRICK20, FLUID8, FLUID20, ElementGroup, GpuModel, Material, Model
 from elpaso_amd import _lib
 from elpaso_amd import meshgen as MG
 
@@ -148,9 +148,21 @@ def test_element_operators_match_oracle(ctx):
                 assert np.abs(M3 - M).max() <= TOL * np.abs(M).max()
 
 
-def test_inverted_element_rejected(ctx):
-    m = MG.acoustic_cavity(2, 2, 2)
-    m.groups[0].conn[0] = m.groups[0].conn[0][[4, 5, 6, 7, 0, 1, 2, 3]]  # mirrored
+@pytest.mark.parametrize("kind", [FLUID8, FLUID20, BRICK8, BRICK20])
+def test_inverted_element_rejected(ctx, kind):
+    """An element whose node order is mirrored (det J < 0 at every point) is
+    reported as ELP_EDETJ by every element kernel."""
+    quad = kind in (FLUID20, BRICK20)
+    ids, xyz, eid, conn, _ = MG.box(2, 2, 2, quadratic=quad)
+    m = Model(node_ids=ids, xyz=xyz, name=f"inverted_{kind}")
+    m.materials[1] = (Material("fluid", rho=1.21, c=343.0) if kind in (FLUID8, FLUID20)
+                      else Material("structure", rho=7850.0, E=210e9, nu=0.3))
+    nat = MG.HEX20_NAT if quad else MG.HEX8_NAT
+    # node k -> the node at the mirrored natural coordinates (-xi, eta, zeta)
+    perm = [int(np.nonzero((nat == nat[k] * np.array([-1, 1, 1])).all(axis=1))[0][0]) for k in range(len(nat))]
+    conn = conn.copy()
+    conn[3] = conn[3][perm]
+    m.groups.append(ElementGroup(kind, 1, eid, conn))
     g = GpuModel(ctx, m)
     with pytest.raises(_lib.ElpError) as e:
         g.assemble()
