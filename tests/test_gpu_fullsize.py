@@ -11,7 +11,10 @@ test_gpu_assembly.py):
   Robin face matrix = face area, sum of the FSI coupling = -2 x interface
   area (both coupling blocks, orientation +1, normal +z; the oracle gives the
   same on small meshes);
-- Rayleigh damping C = 1e-5 K + 2 M entry by entry.
+- Rayleigh damping C = 1e-5 K + 2 M entry by entry;
+- node results: exact pressure gradients of a quadratic field on the Hex20
+  cavity and constant stresses of a linear displacement on the jittered
+  Brick8 block, at full size.
 """
 import numpy as np
 import pytest
@@ -112,3 +115,56 @@ def test_config2_sweep_full_independent_residual(ctx):
         A = sp.csr_matrix((a, ci, rp), shape=(g.n, g.n))
         r = np.linalg.norm(F - A @ X[k]) / np.linalg.norm(F)
         assert r <= 1.5e-8, (f, r, rel[k])
+
+
+def test_config2_node_results_full(ctx):
+    """Pressure gradients at the 1.98M nodes of the Hex20 cavity for a
+    quadratic pressure field, which the straight-sided 20-node elements
+    reproduce exactly: every element's gradient at a node equals the field's,
+    so each node holds (elements at the node) x grad p (elPaSo sums, it does
+    not average, element/fluid/elementfluid3d.cpp:395-404)."""
+    m = MG.acoustic_cavity(100, 80, 60, quadratic=True)
+    g = GpuModel(ctx, m)
+    ids, dofs = g.dof_map()
+    assert (dofs == 16).all() and np.array_equal(ids, np.sort(m.node_ids))
+    xyz = m.xyz[np.argsort(m.node_ids)]
+    x, y, z = xyz.T
+    X = (x * x - 2 * y * z + 3 * z + 0.5j * y * y)[None, :]
+    tm = {}
+    _, nodes = g.node_results(X, [100.0], cells=False, timing=tm)
+    print(f"[fullsize] node results of {g.n} nodes: {tm['kernel_ms']:.2f} ms of kernels", flush=True)
+    cnt = np.zeros(len(ids), np.int64)
+    np.add.at(cnt, np.searchsorted(ids, m.groups[0].conn.ravel()), 1)
+    ref = cnt[:, None] * np.stack([2 * x, -2 * z + 1j * y, -2 * y + 3], axis=1)
+    assert np.abs(nodes[0, :, 6:] - ref).max() <= 1e-10 * np.abs(ref).max()
+    assert not nodes[0, :, :6].any()
+
+
+def test_config1_brick_stresses_full(ctx):
+    """Centroid stresses of the 160,000 jittered Brick8 elements of the steel
+    block for a linear displacement field (reproduced exactly by trilinear
+    elements, so the strain is the same everywhere): every cell and every
+    node holds D eps, components xx, yy, zz, xy, yz, xz."""
+    m = MG.steel_block()
+    g = GpuModel(ctx, m)
+    ids, dofs = g.dof_map()
+    order = np.argsort(m.node_ids)
+    xyz = m.xyz[order]
+    x, y, z = xyz.T
+    u = np.stack([1e-3 * x + 4e-4 * y, -2e-3 * y - 6e-4 * z, 0.5e-3 * z + 9e-4 * x], axis=1)
+    rank = np.searchsorted(np.sort(m.node_ids), ids)
+    X = (u[rank, dofs] * (1 - 0.25j))[None, :]
+    cells, nodes = g.node_results(X, [200.0])
+    E, nu = 210e9, 0.3
+    f = E / ((1 + nu) * (1 - 2 * nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = f * nu
+    np.fill_diagonal(D[:3, :3], f * (1 - nu))
+    D[3, 3] = D[4, 4] = D[5, 5] = f * (1 - 2 * nu) / 2
+    sig = D @ np.array([1e-3, -2e-3, 0.5e-3, 4e-4, -6e-4, 9e-4]) * (1 - 0.25j)
+    scale = np.abs(sig).max()
+    print(f"[fullsize] brick stresses of {cells.shape[1]} elements", flush=True)
+    assert cells.shape == (1, 160000, 6)
+    assert np.abs(cells[0] - sig[None, :]).max() <= 1e-9 * scale
+    assert np.abs(nodes[0, :, :6] - sig[None, :]).max() <= 1e-9 * scale
+    assert not nodes[0, :, 6:].any()
