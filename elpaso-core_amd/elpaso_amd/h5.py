@@ -23,6 +23,7 @@ import numpy as np
 from ._lib import Context, ElpError, GpuModel, _PKG, _ptr, load_library
 
 _LIB = None
+HDF5_LIB = "/opt/conda/lib/libhdf5.so.103"  # the libhdf5 1.10 the library is built against
 
 EXPORTED_SYMBOLS = ["elp_h5_last_error", "elp_h5_read_analysis", "elp_h5_read_model", "elp_h5_write_solution",
                     "elp_h5_write_csr", "elp_h5_read_complex_vector"]
@@ -68,7 +69,16 @@ def load_library_h5():
     path = library_path()
     if not os.path.exists(path):
         raise ElpError(-5, f"{path} is missing: build it with __graft_entry__.build() or make -C elpaso-core_amd")
-    L = C.CDLL(path)
+    try:
+        L = C.CDLL(path)
+    except OSError:
+        # the library finds libhdf5 through the link lib/hdf5/libhdf5.so.103
+        # that the build makes; where a copy of the tree lost it, load
+        # libhdf5 from its install location first, then the library
+        if not os.path.exists(HDF5_LIB):
+            raise
+        C.CDLL(HDF5_LIB, mode=C.RTLD_GLOBAL)
+        L = C.CDLL(path)
     vp, i64, cp = C.c_void_p, C.c_int64, C.c_char_p
     L.elp_h5_last_error.restype = cp
     L.elp_h5_last_error.argtypes = []
