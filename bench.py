@@ -241,7 +241,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         steel = MG.steel_block(*(int(v) for v in args.assembly_nel.split(","))) if assembly else None
-        cpu = cpu_baseline(model, freqs, steel=steel, config0=config0 is not None, procs=args.cpu_procs)
+        cpu = cpu_baseline(model, freqs, steel=steel, config0=config0 is not None, procs=args.cpu_procs,
+                           post_model=MG.acoustic_cavity() if node_res else None)
+        if node_res:
+            # the GPU leg's element-node gradients: 20 per Fluid20 element of the cavity
+            gpu_us = node_res["kernel_ms_per_freq"] * 1e3 / (model.n_elements * 20)
+            node_res["gpu_us_per_element_node"] = round(gpu_us, 6)
+            node_res["gpu_vs_cpu_1core_per_element_node"] = round(cpu["node_results_1core"]["us_per_element_node"]
+                                                                  / gpu_us)
         if assembly:
             assembly["gpu_vs_cpu_1core"] = round(cpu["brick8_assembly_1core_s"]["assembly"] / (assembly["ms"] * 1e-3))
         if config0:
@@ -483,7 +490,7 @@ def _cpu_config0(procs):
     return out
 
 
-def cpu_baseline(model, freqs, steel=None, config0=False, procs=16):
+def cpu_baseline(model, freqs, steel=None, config0=False, procs=16, post_model=None):
     """elPaSo's per-frequency flow restated on the CPU (oracle), one core:
     re-integrate every element at w, scatter into CSR (SetupKM).  The direct
     LU solve (MUMPS in elPaSo, SuperLU here) of the 1.98M-DOF Hex20 system
@@ -519,6 +526,19 @@ def cpu_baseline(model, freqs, steel=None, config0=False, procs=16):
     if config0:
         out["config0"] = _cpu_config0(procs)
         log(f"[bench] cpu oracle config 0: {out['config0']}")
+    if post_model is not None:
+        # element and node results of one solution (pressure gradients at the
+        # nodes), the oracle's pure-Python restatement of elPaSo's loops
+        pnum = O.Numbering(post_model)
+        Xp = np.random.default_rng(0).standard_normal((1, pnum.n)) + 0j
+        t = time.perf_counter()
+        O.node_results(post_model, Xp, [100.0], pnum)
+        tp = time.perf_counter() - t
+        n_en = sum(len(gr.elem_ids) * gr.conn.shape[1] for gr in post_model.groups)
+        out["node_results_1core"] = {"sample": f"1 frequency of the Fluid8 20x16x12 cavity ({n_en} element-node "
+                                               f"gradients), pure-Python restatement",
+                                     "s_per_freq": round(tp, 2), "us_per_element_node": round(1e6 * tp / n_en, 1)}
+        log(f"[bench] cpu oracle node results: {out['node_results_1core']}")
     return out
 
 if __name__ == "__main__":
