@@ -278,14 +278,14 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
   d_Tc.upload(c, hex_derivs_at(8, centroid));
   // frequency chunk: solutions, node results and brick stresses within
   // about 4 GiB (the fluid gradient operators are reused across a chunk)
-  int64_t per_f = n * 16 + N * 9 * 16;
+  int64_t per_f = n * 16 + (nodes ? N * 9 * 16 : 0);
   for (auto &kb : m.kinds)
     if (!is_fluid(kb.kind)) per_f += kb.n_elem * 6 * 16;
   const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nf, (int64_t)(4LL << 30) / per_f, 65535}));
   DArr<double2> dX, dN, dScale;
   std::vector<DArr<double2>> dres(m.kinds.size());
   dX.alloc(c, chunk * n);
-  dN.alloc(c, chunk * N * 9);
+  if (nodes) dN.alloc(c, chunk * N * 9);
   dScale.alloc(c, chunk);
   for (size_t k = 0; k < m.kinds.size(); k++)
     if (!is_fluid(m.kinds[k].kind)) dres[k].alloc(c, chunk * m.kinds[k].n_elem * 6);

@@ -80,3 +80,5 @@ def test_many_frequencies_and_repeat_bit_identical(ctx):
     g, X, cells, nodes = _check(ctx, m, list(np.linspace(10.0, 1000.0, 40)), seed=2)
     c2, n2 = g.node_results(X, np.linspace(10.0, 1000.0, 40))
     assert np.array_equal(cells, c2) and np.array_equal(nodes, n2)
+    c3, n3 = g.node_results(X, np.linspace(10.0, 1000.0, 40), nodes=False)  # cells only
+    assert n3 is None and np.array_equal(cells, c3)
