@@ -274,26 +274,34 @@ __global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const 
 }
 
 // A padded row in batches of NB entries: lane u loads entry u's x offset,
-// K and M with three coalesced vector loads; every entry's offset is
-// broadcast with v_readlane and all NB loads of x are issued before the
-// first FMA.  Entries past the row read an offset past the end of x (0).
+// K and M for 2·NB entries with three coalesced vector loads (one memory
+// round trip for both batches of a row of up to 64 entries); every entry's
+// offset is broadcast with v_readlane and all NB loads of x of a batch are
+// issued before its first FMA.  Entries past the row read an offset past the
+// end of x (0).  Entries are summed in row order.
 template <int NB>
 __device__ inline void row_batches(const PadCsr &pc, int32_t base, int32_t p1, uint32_t oob,
                                    __amdgpu_buffer_rsrc_t xr, uint32_t vo, int lane, float &aK, float &aM) {
-  for (; base < p1; base += NB) {
-    const bool in = lane < NB && base + lane < p1;
+  static_assert(2 * NB == 64, "two batches per wavefront-wide index load");
+  for (; base < p1; base += 2 * NB) {
+    const bool in = base + lane < p1;
     const uint32_t myoff = in ? pc.off[base + lane] : oob;
     const float myk = in ? pc.K[base + lane] : 0.f, mym = in ? pc.M[base + lane] : 0.f;
-    float xv[NB];
 #pragma unroll
-    for (int u = 0; u < NB; u++) {
-      const uint32_t so = __builtin_amdgcn_readlane(myoff, u);
-      xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, so, 0));
-    }
+    for (int h = 0; h < 2; h++) {
+      if (h == 1 && base + NB >= p1) break;  // uniform: the row ends in the first batch
+      float xv[NB];
 #pragma unroll
-    for (int u = 0; u < NB; u++) {
-      aK = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, myk), u)), xv[u], aK);
-      aM = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mym), u)), xv[u], aM);
+      for (int u = 0; u < NB; u++) {
+        const uint32_t so = __builtin_amdgcn_readlane(myoff, h * NB + u);
+        xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, so, 0));
+      }
+#pragma unroll
+      for (int u = 0; u < NB; u++) {
+        const int e = h * NB + u;
+        aK = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, myk), e)), xv[u], aK);
+        aM = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mym), e)), xv[u], aM);
+      }
     }
   }
 }
