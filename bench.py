@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # on the default workloads, scaled by the factors a 1 GiB copy gives (x2, x1);
 # the commands and counter values are in the profile below.
 PMC_PROFILE = "profiles/pmc_traffic_row_op_kernel_and_hex8_node_kernel.md"
-PMC_TRAFFIC = {("row_op_kernel<0>", "100,80,60"): 2.792e9, ("hex8_node_kernel<3, false, true>", "200,200,4"): 1.119e9}
+PMC_TRAFFIC = {("row_op_kernel<0>", "100,80,60"): 2.790e9, ("hex8_node_kernel<3, false, true>", "200,200,4"): 1.116e9}
 
 
 def log(*a):

@@ -5,6 +5,10 @@
 #  - acousticmedium_fluid8_2elem_AF_LIN_UAF_ISO_DIR.hdf5: the reference's own
 #    2-element Fluid8 input fixture (test/resources/unitHDF5/material/fluid/
 #    linear/elastic/), used as is
+#  - acousticmedium_fluid8_2elem_AF_LIN_EQF_ISO_DIR.hdf5: the same mesh with an
+#    equivalent-fluid material (test/resources/unitHDF5/material/fluid/linear/
+#    viscoelastic/), a material type elPaSo-Core's factory does not create;
+#    used as is
 #  - egenoutput_dsg4_excerpt.hdf5: /Solution/Maps/mtxDofMap and
 #    /Solution/State/vecFemStep1..2 of the solution file elPaSo wrote for the
 #    DSG4 plate tutorial (doc/tutorials/FEM_tutorials/Simulations/
@@ -17,6 +21,8 @@ H=/opt/conda/bin
 OUT=$(cd "$(dirname "$0")" && pwd)
 cp "$REF/elpasoCore/test/resources/unitHDF5/material/fluid/linear/elastic/acousticmedium_fluid8_2elem_AF_LIN_UAF_ISO_DIR.hdf5" "$OUT/"
 chmod u+w "$OUT/acousticmedium_fluid8_2elem_AF_LIN_UAF_ISO_DIR.hdf5"
+cp "$REF/elpasoCore/test/resources/unitHDF5/material/fluid/linear/viscoelastic/acousticmedium_fluid8_2elem_AF_LIN_EQF_ISO_DIR.hdf5" "$OUT/"
+chmod u+w "$OUT/acousticmedium_fluid8_2elem_AF_LIN_EQF_ISO_DIR.hdf5"
 SOL="$REF/doc/tutorials/FEM_tutorials/Simulations/Ex_1_Analysis_of_a_rectangular_plate_DSG4/eGenOutput_Analysis_of_a_rectangular_plate.hdf5"
 EX="$OUT/egenoutput_dsg4_excerpt.hdf5"
 rm -f "$EX"

@@ -27,6 +27,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FLUID8_2ELEM = os.path.join(ROOT, "tests", "golden", "acousticmedium_fluid8_2elem_AF_LIN_UAF_ISO_DIR.hdf5")
+FLUID8_EQF = os.path.join(ROOT, "tests", "golden", "acousticmedium_fluid8_2elem_AF_LIN_EQF_ISO_DIR.hdf5")
 H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
 
 
@@ -182,6 +183,17 @@ def test_unsupported_content_is_rejected(ctx, tmp_path, case, match):
     write_input(path, m, **kw)
     with pytest.raises(_lib.ElpError, match=match):
         h5.read_model(ctx, path)
+
+
+def test_reference_equivalent_fluid_fixture_is_rejected(ctx):
+    """The reference's Fluid8 fixture with an equivalent-fluid material
+    (AF_LIN_EQF_ISO_DIR): elPaSo-Core's material factory creates no material
+    for that type and its parser stops (femparserinterface.cpp:299-306,
+    materialfactory.cpp:41-100); here the element group that uses it is an
+    error naming the type."""
+    print("[h5] reference equivalent-fluid fixture", flush=True)
+    with pytest.raises(_lib.ElpError, match="material type AF_LIN_EQF_ISO_DIR is not supported"):
+        h5.read_model(ctx, FLUID8_EQF)
 
 
 def test_sweep_written_and_read_back(ctx, tmp_path):

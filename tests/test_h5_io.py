@@ -54,6 +54,19 @@ def test_analysis_of_the_reference_fixture():
     np.testing.assert_array_equal(a.frequencies(), 10.0 + 10.0 * np.arange(10))
 
 
+@needs_h5dump
+def test_equivalent_fluid_fixture_differs_only_in_its_material():
+    """The reference's equivalent-fluid fixture: same analysis as the ideal
+    fluid one, material type AF_LIN_EQF_ISO_DIR as the HDF5 tools read it
+    (the GPU test checks that an element group using it is rejected)."""
+    eqf = os.path.join(GOLD, "acousticmedium_fluid8_2elem_AF_LIN_EQF_ISO_DIR.hdf5")
+    assert h5.read_analysis(eqf) == h5.read_analysis(FLUID8_2ELEM)
+    attrs = _dump("-A", "-d", "/Materials/material1", eqf)
+    assert re.search(r'ATTRIBUTE "MaterialType".*?\(0\): "AF_LIN_EQF_ISO_DIR"', attrs, re.S)
+    for name in ("/Nodes/mtxFemNodes", "/Elements/mtxFemElemGroup1"):
+        assert _dump("-d", name, eqf).split("\n", 1)[1] == _dump("-d", name, FLUID8_2ELEM).split("\n", 1)[1]
+
+
 def test_reads_a_solution_elpaso_wrote():
     x1 = h5.read_complex_vector(DSG4_EXCERPT, "/Solution/State/vecFemStep1", 210)
     x2 = h5.read_complex_vector(DSG4_EXCERPT, "/Solution/State/vecFemStep2", 210)
