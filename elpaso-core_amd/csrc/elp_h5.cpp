@@ -515,6 +515,10 @@ int elp_h5_write_csr(const char *path, const char *name, int64_t n, const int32_
     if (std::strchr(name, '/')) fail("matrix name must not contain '/'");
     const int64_t nnz = rowptr[n];
     if (rowptr[0] != 0 || nnz < 0) fail("rowptr must start at 0");
+    for (int64_t r = 0; r < n; r++)
+      if (rowptr[r + 1] < rowptr[r]) fail("rowptr decreases at row " + std::to_string(r));
+    for (int64_t k = 0; k < nnz; k++)
+      if (colind[k] < 0 || colind[k] >= n) fail("column index out of range at entry " + std::to_string(k));
     Hid f = create_or_open(path, false);
     Hid top = ensure_group(f, "/SystemMatrices");
     const std::string gname = std::string("/SystemMatrices/") + name;

@@ -150,6 +150,12 @@ def test_csr_export_layout(tmp_path):
     assert [int(t) for t in re.findall(r"(?<![\w(])\d+(?![\w)])", ja)] == [1, 3, 2, 1, 3]
     with pytest.raises(ElpError, match="exists already"):
         h5.write_csr(path, "K", rp, ci, v)
+    # a malformed pattern is an error, and nothing of it is written
+    with pytest.raises(ElpError, match="column index out of range at entry 4"):
+        h5.write_csr(path, "C1", rp, np.array([0, 2, 1, 0, 3], np.int32), v)
+    with pytest.raises(ElpError, match="rowptr decreases at row 1"):
+        h5.write_csr(path, "C2", np.array([0, 3, 2, 5], np.int32), ci, v)
+    assert "C1" not in _dump("-n", str(path)) and "C2" not in _dump("-n", str(path))
 
 
 def test_errors_are_reported_not_printed(tmp_path):
