@@ -1096,19 +1096,15 @@ float time_volume_assembly(Model &m, int repeats, hipStream_t s) {
   ELP_REQUIRE(m.assembled, ELP_ESTATE, "assemble first");
   DArr<int> bad;
   bad.alloc(nullptr, 1);
-  hipEvent_t a, b;
-  ELP_HIP(hipEventCreate(&a));
-  ELP_HIP(hipEventCreate(&b));
-  ELP_HIP(hipEventRecord(a, s));
+  HipEvent a, b;
+  ELP_HIP(hipEventRecord(a.e, s));
   for (int r = 0; r < repeats; r++) {
     for (auto &kb : m.kinds) launch_volume(m, kb, bad.p, s);
   }
-  ELP_HIP(hipEventRecord(b, s));
-  ELP_HIP(hipEventSynchronize(b));
+  ELP_HIP(hipEventRecord(b.e, s));
+  ELP_HIP(hipEventSynchronize(b.e));
   float ms = 0;
-  ELP_HIP(hipEventElapsedTime(&ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  ELP_HIP(hipEventElapsedTime(&ms, a.e, b.e));
   return ms / repeats;
 }
 

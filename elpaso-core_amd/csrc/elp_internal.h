@@ -48,6 +48,17 @@ struct Ctx {
   }
 };
 
+// RAII HIP event (destroyed also when an error unwinds the caller).
+struct HipEvent {
+  hipEvent_t e = nullptr;
+  HipEvent() { ELP_HIP(hipEventCreate(&e)); }
+  HipEvent(const HipEvent &) = delete;
+  HipEvent &operator=(const HipEvent &) = delete;
+  ~HipEvent() {
+    if (e) (void)hipEventDestroy(e);
+  }
+};
+
 // RAII device allocation, accounted against the owning context.
 template <typename T>
 struct DArr {

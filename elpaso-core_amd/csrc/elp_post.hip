@@ -16,6 +16,7 @@
 #include <array>
 #include <cmath>
 #include <complex>
+#include <memory>
 #include <vector>
 
 #include "elp_fem.h"
@@ -295,11 +296,11 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
   ELP_REQUIRE(!cells || brick_kinds <= 1, ELP_EINVAL,
               "element stresses of a model mixing Brick8 and Brick20 elements are not supported");
   std::vector<double2> scale(chunk);
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::unique_ptr<HipEvent> ev0, ev1;
   if (kernel_ms) {
     *kernel_ms = 0.0;
-    ELP_HIP(hipEventCreate(&ev0));
-    ELP_HIP(hipEventCreate(&ev1));
+    ev0.reset(new HipEvent);
+    ev1.reset(new HipEvent);
   }
   for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
     const int nc = (int)std::min(chunk, nf - f0);
@@ -312,7 +313,7 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
       scale[f] = double2{1.0, im};
     }
     ELP_HIP(hipMemcpyAsync(dScale.p, scale.data(), sizeof(double2) * nc, hipMemcpyHostToDevice, s));
-    if (kernel_ms) ELP_HIP(hipEventRecord(ev0, s));
+    if (kernel_ms) ELP_HIP(hipEventRecord(ev0->e, s));
     PostKinds K;
     for (size_t k = 0; k < m.kinds.size(); k++) {
       KindBlock &kb = m.kinds[k];
@@ -343,7 +344,7 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
         ELP_HIP(hipGetLastError());
       }
     }
-    if (kernel_ms) ELP_HIP(hipEventRecord(ev1, s));
+    if (kernel_ms) ELP_HIP(hipEventRecord(ev1->e, s));
     if (nodes)
       ELP_HIP(hipMemcpyAsync(nodes + 2 * f0 * N * 9, dN.p, sizeof(double2) * nc * N * 9, hipMemcpyDeviceToHost, s));
     if (cells && brick_cols) {
@@ -362,12 +363,10 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
     ELP_HIP(hipStreamSynchronize(s));
     if (kernel_ms) {
       float ms = 0.f;
-      ELP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+      ELP_HIP(hipEventElapsedTime(&ms, ev0->e, ev1->e));
       *kernel_ms += ms;
     }
   }
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
 }
 
 int64_t brick_element_count(const Model &m) {
