@@ -59,6 +59,24 @@ struct HipEvent {
   }
 };
 
+// RAII pinned host array (hipHostMalloc).
+template <typename T>
+struct PinnedArr {
+  T *p = nullptr;
+  PinnedArr() = default;
+  PinnedArr(const PinnedArr &) = delete;
+  PinnedArr &operator=(const PinnedArr &) = delete;
+  void alloc(size_t n) {
+    ELP_REQUIRE(!p, ELP_ESTATE, "pinned array allocated twice");
+    ELP_HIP(hipHostMalloc((void **)&p, sizeof(T) * n, hipHostMallocDefault));
+  }
+  T &operator[](size_t i) { return p[i]; }
+  const T &operator[](size_t i) const { return p[i]; }
+  ~PinnedArr() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 // RAII device allocation, accounted against the owning context.
 template <typename T>
 struct DArr {

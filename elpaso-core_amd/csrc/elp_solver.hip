@@ -1466,9 +1466,9 @@ struct Solver {
   DArr<V> xi, ri, zi, pi, qi;    // inner Krylov vectors
   DArr<V> fb, fe, fs1, fs2;      // exact box solve: rhs, solution, scratch (lattice size)
   DArr<FdmCoef> coefF;
-  double2 *h_red = nullptr;     // pinned
+  PinnedArr<double2> h_red;     // per-frequency reductions, pinned host memory
   // statistics
-  hipEvent_t ev0, ev1;
+  HipEvent ev0, ev1;
   double spmv_ms = 0;
   int64_t spmv_calls = 0;
   bool timing = true;
@@ -1500,14 +1500,7 @@ struct Solver {
       fb.alloc(ctx, nf); fe.alloc(ctx, nf); fs1.alloc(ctx, nf); fs2.alloc(ctx, nf);
       coefF.alloc(ctx, B);
     }
-    ELP_HIP(hipHostMalloc((void **)&h_red, sizeof(double2) * B, hipHostMallocDefault));
-    ELP_HIP(hipEventCreate(&ev0));
-    ELP_HIP(hipEventCreate(&ev1));
-  }
-  ~Solver() {
-    if (h_red) (void)hipHostFree(h_red);
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    h_red.alloc(B);
   }
 
   template <int MODE, typename TT = T>
@@ -1576,7 +1569,7 @@ struct Solver {
            st.levels[0]->ap_rp.p;
   }
   void fetch(const double2 *dev) {
-    ELP_HIP(hipMemcpyAsync(h_red, dev, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
+    ELP_HIP(hipMemcpyAsync(h_red.p, dev, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
     ELP_HIP(hipStreamSynchronize(s));
   }
 
@@ -1680,9 +1673,9 @@ struct Solver {
     reduce(m.n, ri.p, zi.p, 0, rho.p);
     const int tpb = 256;
     for (int it = 0; it < maxit; it++) {
-      if (timing) ELP_HIP(hipEventRecord(ev0, s));
+      if (timing) ELP_HIP(hipEventRecord(ev0.e, s));
       op<0>(0, coefA.p, pi.p, nullptr, qi.p);
-      if (timing) ELP_HIP(hipEventRecord(ev1, s));
+      if (timing) ELP_HIP(hipEventRecord(ev1.e, s));
       spmv_calls++;
       reduce(m.n, pi.p, qi.p, 0, mu.p);
       hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, rho.p, mu.p, active.p, alpha.p);
@@ -1690,7 +1683,7 @@ struct Solver {
       fetch(red_out.p);
       if (timing) {
         float ms = 0;
-        ELP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        ELP_HIP(hipEventElapsedTime(&ms, ev0.e, ev1.e));
         spmv_ms += ms;
       }
       bool any = false, changed = false;
@@ -2112,9 +2105,9 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       }
     if (restarted) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     // q = A p, alpha = rho / p.q, x += alpha p, r -= alpha q
-    if (S.timing) ELP_HIP(hipEventRecord(S.ev0, s));
+    if (S.timing) ELP_HIP(hipEventRecord(S.ev0.e, s));
     S.template op<0>(0, S.coefA.p, S.pi.p, nullptr, S.qi.p);
-    if (S.timing) ELP_HIP(hipEventRecord(S.ev1, s));
+    if (S.timing) ELP_HIP(hipEventRecord(S.ev1.e, s));
     S.spmv_calls++;
     S.reduce(m.n, S.pi.p, S.qi.p, 0, S.mu.p);
     hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
@@ -2122,7 +2115,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     S.fetch(S.red_out.p);
     if (S.timing) {
       float ms = 0;
-      ELP_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+      ELP_HIP(hipEventElapsedTime(&ms, S.ev0.e, S.ev1.e));
       S.spmv_ms += ms;
     }
     int n_frozen = 0;
