@@ -57,6 +57,7 @@ struct Server {
   void run_job(Job &j, hipStream_t s) {
     Model &m = *j.model->m;
     try {
+      ELP_REQUIRE(s, ELP_EHIP, "the server's worker could not create its HIP stream");
       ELP_HIP(hipSetDevice(ctx->device));
       std::lock_guard<std::mutex> lk(m.mu);
       if (!m.assembled) assemble_model(m, s);
@@ -86,7 +87,7 @@ struct Server {
   void worker_loop() {
     hipStream_t s = nullptr;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;  // jobs then fail
     while (true) {
       std::shared_ptr<Job> j;
       {
