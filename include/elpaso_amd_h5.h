@@ -95,7 +95,8 @@ int elp_h5_write_solution(const char *path, int64_t n, int64_t n_freq, const dou
  * {real, imag}).  rowptr, colind: 0-based CSR as elp_dynamic_matrix gives;
  * vals: nnz interleaved complex.  The file is opened for writing when it
  * exists (other groups are kept; an existing group of this name is an
- * error), else created. */
+ * error), else created.  A rowptr that does not start at 0 or decreases, or
+ * a column index outside [0, n), is ELP_EINVAL and nothing is written. */
 int elp_h5_write_csr(const char *path, const char *name, int64_t n, const int32_t *rowptr,
                      const int32_t *colind, const double *vals);
 
