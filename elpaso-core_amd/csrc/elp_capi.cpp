@@ -114,14 +114,15 @@ int elp_model_add_structure_material(elp_model *mm, int64_t id, double E, double
   ELP_TRY
   auto &m = M(mm);
   ELP_ARG(E > 0 && rho > 0 && nu > -1.0 && nu < 0.5, "invalid structural material");
-  ELP_ARG(eta >= 0 && (visco_type == 1 || visco_type == 2), "invalid loss factor / visco type");
+  ELP_ARG(eta >= 0 && (visco_type == 1 || visco_type == 2 || (visco_type == 0 && eta == 0)),
+          "invalid loss factor / visco type");
   elp::Material mat;
   mat.fluid = false;
   mat.E = E;
   mat.nu = nu;
   mat.rho = rho;
   mat.eta = eta;
-  mat.visco_type = visco_type;
+  mat.visco_type = visco_type == 0 ? 1 : visco_type;  // 0: elastic (eta = 0)
   m.mats[id] = mat;
   ELP_CATCH
 }

@@ -203,6 +203,7 @@ void read_model(elp_model *m, hid_t f, elp_h5_counts *cnt) {
     Hid d(H5Dopen2(f, "/Nodes/mtxFemNodes", H5P_DEFAULT), H5Dclose);
     if (d < 0) fail("cannot open /Nodes/mtxFemNodes");
     Hid sp(H5Dget_space(d), H5Sclose);
+    if (H5Sget_simple_extent_ndims(sp) != 1) fail("/Nodes/mtxFemNodes: expected a 1-D dataset of node records");
     hsize_t n = 0;
     H5Sget_simple_extent_dims(sp, &n, nullptr);
     struct Rec {
@@ -241,8 +242,9 @@ void read_model(elp_model *m, hid_t f, elp_h5_counts *cnt) {
       if (type == "AF_LIN_UAF_ISO_DIR") {
         check(elp_model_add_fluid_material(m, id, attr_number(d, "rho"), attr_number(d, "c")), where);
       } else if (type == "STR_LIN_ELA_ISO_DIR") {
+        // elastic: no loss factor, so the visco type is irrelevant
         check(elp_model_add_structure_material(m, id, attr_number(d, "E"), attr_number(d, "nu"),
-                                               attr_number(d, "rho"), 0.0, 0),
+                                               attr_number(d, "rho"), 0.0, 1),
               where);
       } else if (type == "STR_LIN_VIS_ISO_DIR") {
         check(elp_model_add_structure_material(m, id, attr_number(d, "E"), attr_number(d, "nu"),
@@ -302,7 +304,9 @@ void read_model(elp_model *m, hid_t f, elp_h5_counts *cnt) {
       const std::string type = attr_string(d, "ConstraintType");
       const int64_t node = attr_int(d, "NodeId");
       if (type == "acoustic") {
-        check(elp_model_add_dirichlet(m, node, kFluidDof, attr_number(d, "valp"), 0.0), where);
+        // the prescribed pressure is attribute "p" (sValP of rulesh5.h:332,
+        // read by femparserhdf5.cpp:442), as elPaSo reads it
+        check(elp_model_add_dirichlet(m, node, kFluidDof, attr_number(d, "p"), 0.0), where);
       } else if (type == "structure") {
         for (int k = 0; k < 21; k++) {
           if (!has_attr(d, kConFlags[k]) || attr_int(d, kConFlags[k]) == 0) continue;

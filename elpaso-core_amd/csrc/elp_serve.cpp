@@ -48,6 +48,7 @@ struct Server {
   std::unordered_map<int64_t, std::shared_ptr<Job>> jobs;
   int64_t next_id = 1, in_use = 0, admitted = 0;
   int running = 0;
+  int waiters = 0;  // callers inside elp_server_wait
   bool stop = false;
   std::thread scheduler;
   std::vector<std::thread> workers;
@@ -172,14 +173,20 @@ void elp_server_destroy(elp_server *sv) {
   if (!sv) return;
   {
     std::unique_lock<std::mutex> lk(sv->s.mu);
-    // drop what never started, let admitted work finish
+    // drop what never started (its waiters get ELP_ESTATE), let admitted
+    // work finish, and wait until no caller is inside elp_server_wait
+    sv->s.stop = true;
     for (auto &j : sv->s.pending) {
       delete j->model;
       j->model = nullptr;
+      j->status = ELP_ESTATE;
+      j->error = "server stopped before the request started";
+      j->t_done = std::chrono::steady_clock::now();
+      j->done = true;
     }
     sv->s.pending.clear();
-    sv->s.cv_done.wait(lk, [&] { return sv->s.running == 0; });
-    sv->s.stop = true;
+    sv->s.cv_done.notify_all();
+    sv->s.cv_done.wait(lk, [&] { return sv->s.running == 0 && sv->s.waiters == 0; });
   }
   sv->s.cv_sched.notify_all();
   sv->s.cv_ready.notify_all();
@@ -232,9 +239,12 @@ int elp_server_wait(elp_server *sv, int64_t job_id, double *X, double *relres, i
     auto it = sv->s.jobs.find(job_id);
     ELP_REQUIRE(it != sv->s.jobs.end(), ELP_EINVAL, "unknown job");
     j = it->second;
+    sv->s.waiters++;
     sv->s.cv_done.wait(lk, [&] { return j->done; });
+    sv->s.waiters--;
     sv->s.jobs.erase(job_id);
   }
+  sv->s.cv_done.notify_all();  // a destroy may be waiting for the waiters
   if (latency_s) *latency_s = std::chrono::duration<double>(j->t_done - j->t_submit).count();
   if (j->status != 0) throw Error(j->status, j->error);
   if (X) std::memcpy(X, j->X.data(), j->X.size() * sizeof(double));

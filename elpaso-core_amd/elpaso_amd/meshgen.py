@@ -171,6 +171,43 @@ def coupled_plate_cavity(n_xy=100, n_plate_z=2, n_cav_z=120, plate_t=0.005,
     return m
 
 
+def impedance_tube(nz, length=1.0, width=0.1, quadratic=False, Z=None, p0=1.0,
+                   material=AIR, lumped=None):
+    """Air column of 1 x 1 x nz elements along z: the pressure is prescribed
+    (Dirichlet, value p0) on the face z = 0 and the face z = length carries a
+    Robin impedance Z (default rho c, the matched termination); the side
+    walls are rigid.  A plane wave travels along z, so the solution is the
+    1-D impedance-tube field p(z) = A e^{-ikz} + B e^{ikz}."""
+    nid, xyz, eid, conn, lat = box(1, 1, nz, (width, width, length), quadratic=quadratic)
+    kind = FLUID20 if quadratic else FLUID8
+    m = Model(node_ids=nid, xyz=xyz, name=f"tube_{'hex20' if quadratic else 'hex8'}_{nz}")
+    m.materials[1] = material
+    m.groups.append(ElementGroup(kind, 1, eid, conn))
+    top = lat[:, 2] == nz - 1
+    m.imp_elem_ids = eid[top]
+    m.imp_faces = np.full(int(top.sum()), FACE_PLUS_Z + 1, dtype=np.int32)
+    rho, c = material.rho, material.c
+    m.impedance = complex(rho * c) if Z is None else complex(Z)
+    m.imp_lumped = (not quadratic) if lumped is None else bool(lumped)
+    for nd in nid[np.isclose(xyz[:, 2], 0.0)]:
+        m.dirichlet.append((int(nd), 16, complex(p0)))
+    return m
+
+
+def impedance_tube_field(z, freq, Z, rho=AIR.rho, c=AIR.c, length=1.0, p0=1.0):
+    """Closed-form pressure of the impedance tube at heights z (time factor
+    e^{+iwt}, the convention of elPaSo's loss factor E(1 + i eta)):
+    p = A e^{-ikz} + B e^{ikz}, p(0) = p0, and the Robin condition
+    dp/dz = -i w rho p / Z at z = length gives the reflection coefficient
+    R = B e^{ikL} / (A e^{-ikL}) = (Z - rho c) / (Z + rho c)."""
+    k = 2 * math.pi * freq / c
+    R = (Z - rho * c) / (Z + rho * c)
+    A = p0 / (1 + R * np.exp(-2j * k * length))
+    B = A * R * np.exp(-2j * k * length)
+    z = np.asarray(z, dtype=np.float64)
+    return A * np.exp(-1j * k * z) + B * np.exp(1j * k * z)
+
+
 def linear_frequencies(start, stop, steps):
     """elPaSo's list f_k = start + delta k, k < steps (analysisfrequencybasic.cpp:50-55),
     deduplicated and sorted like its std::set."""

@@ -221,7 +221,11 @@ int elp_node_results(elp_model *m, int64_t n_freq, const double *freqs_hz, const
 /* ---- serving: continuous batching of sweep requests ----------------------------- */
 int elp_server_create(elp_ctx *ctx, int64_t hbm_budget_bytes, int max_slots, elp_server **out);
 void elp_server_destroy(elp_server *s);
-/* the server takes ownership of m (finalized, not yet assembled or assembled) */
+/* On success the server takes ownership of m (finalized, assembled or not)
+ * and destroys it when the request ends; on failure the caller keeps it.
+ * Every elp_server_wait must have returned before elp_server_destroy is
+ * called from another thread; requests that have not started when the
+ * server is destroyed end with ELP_ESTATE ("server stopped"). */
 int elp_server_submit(elp_server *s, elp_model *m, int64_t n_freq, const double *freqs_hz,
                       double z_re, double z_im, const elp_solve_opts *opts, int64_t *job_id);
 /* 1 if finished, 0 if pending */

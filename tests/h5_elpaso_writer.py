@@ -121,12 +121,17 @@ _KIND_NAMES = {1: "Fluid8", 2: "Brick8", 3: "Fluid20", 4: "Brick20"}
 
 
 def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6, element_type=None,
-                material_type=None, elem_load=False, interface_type="InterfaceMindlin"):
+                material_type=None, elem_load=False, interface_type="InterfaceMindlin", valp=None,
+                nodes_rank2=False):
     """Writes model (elpaso_amd.Model) as an elPaSo input file.  element_type
     / material_type replace the first group's ElementType / the first
     material's MaterialType, elem_load adds a "vn" element load, and
     interface_type names the interface elements' type (content the library
-    must reject)."""
+    must reject).  An acoustic constraint's value goes to attribute "p"
+    (u8 when it is a small whole number, as in the reference's fixture, f64
+    otherwise) and to "valp", unless valp gives another "valp" value.
+    nodes_rank2 writes the node records as an n x 1 dataset (malformed: the
+    reader expects a 1-D dataset)."""
     F = _File(path)
     g = F.group("/Analysis")
     F.attr(g, "type", "frequency-basic")
@@ -147,7 +152,7 @@ def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6, eleme
     L.H5Tinsert(ct, b"Ids", 0, _t("H5T_STD_I64LE_g"))
     for k, nm in enumerate((b"xCoords", b"yCoords", b"zCoords")):
         L.H5Tinsert(ct, nm, 8 + 8 * k, _t("H5T_IEEE_F64LE_g"))
-    F.dataset(g, "mtxFemNodes", ct, ct, rec)
+    F.dataset(g, "mtxFemNodes", ct, ct, rec, dims=(len(rec), 1) if nodes_rank2 else None)
     L.H5Tclose(ct)
     # materials: parameters as strings
     g = F.group("/Materials")
@@ -201,8 +206,9 @@ def write_input(path, model, start=10.0, delta=10.0, steps=10, revision=6, eleme
         F.attr(d, "NodeId", ("u64", nid))
         if dof == 16:
             F.attr(d, "ConstraintType", "acoustic")
-            F.attr(d, "p", ("u8", 1))
-            F.attr(d, "valp", ("f64", complex(v).real))
+            pv = complex(v).real
+            F.attr(d, "p", ("u8", int(pv)) if pv == int(pv) and 0 <= pv < 256 else ("f64", pv))
+            F.attr(d, "valp", ("f64", pv if valp is None else float(valp)))
         else:
             F.attr(d, "ConstraintType", "structure")
             for j, fl in enumerate(flags):

@@ -133,6 +133,45 @@ def test_coupled_model_written_in_elpaso_layout(ctx, tmp_path):
         assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
 
 
+def test_elastic_structure_round_trip(ctx, tmp_path):
+    """A structure of elPaSo's plain elastic material (STR_LIN_ELA_ISO_DIR,
+    eta = 0) is read and assembles exactly like the same model built through
+    the C API."""
+    print("[h5] elastic Brick8 block 4x3x2 round trip", flush=True)
+    m = MG.steel_block(4, 3, 2, rayleigh=(0.0, 0.0))
+    m.forces.append((int(m.node_ids[-1]), 0.0, 0.0, 1.0))
+    path = tmp_path / "elastic.hdf5"
+    write_input(path, m)
+    g, _, counts = h5.read_model(ctx, path)
+    assert counts["materials"] == 1
+    ref = GpuModel(ctx, m)
+    for x in (g, ref):
+        x.assemble()
+    _same_csr(g, ref, (_lib.MAT_K, _lib.MAT_M))
+    _oracle_close(m, g, (_lib.MAT_K, _lib.MAT_M))
+    for f in (50.0, 900.0):
+        a1, f1 = g.dynamic_matrix(2 * np.pi * f)
+        a2, f2 = ref.dynamic_matrix(2 * np.pi * f)
+        assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("p,valp,expect", [(1.0, 0.5, 1.0), (0.5, 0.25, 0.5)])
+def test_acoustic_constraint_value_is_attribute_p(ctx, tmp_path, p, valp, expect):
+    """elPaSo takes an acoustic constraint's value from attribute "p"
+    (rulesh5.h:332 sValP = "p", femparserhdf5.cpp:442), not from "valp"."""
+    print(f"[h5] acoustic constraint p={p} valp={valp}", flush=True)
+    m = MG.impedance_tube(4, p0=p)
+    path = tmp_path / "tube.hdf5"
+    write_input(path, m, valp=valp)
+    g, _, counts = h5.read_model(ctx, path)
+    assert counts["constraints"] == 4
+    g.assemble()
+    _, f = g.dynamic_matrix(2 * np.pi * 100.0)
+    ids, _ = g.dof_map()
+    fixed = np.isin(ids, [nid for nid, _, _ in m.dirichlet])
+    np.testing.assert_array_equal(f[fixed], expect)
+
+
 @pytest.mark.parametrize("kind", ["fluid20", "brick20"])
 def test_twenty_node_models_round_trip(ctx, tmp_path, kind):
     print(f"[h5] {kind} model written and read back", flush=True)
@@ -168,6 +207,7 @@ def test_twenty_node_models_round_trip(ctx, tmp_path, kind):
     ("material", "material type pointmass is not supported"),
     ("elemload", "element loads are not supported"),
     ("interface", "interface type InterfaceKirch is not supported"),
+    ("nodes_rank2", "mtxFemNodes: expected a 1-D dataset"),
 ])
 def test_unsupported_content_is_rejected(ctx, tmp_path, case, match):
     """Content the library does not assemble is an error naming the item,
@@ -179,7 +219,7 @@ def test_unsupported_content_is_rejected(ctx, tmp_path, case, match):
     path = tmp_path / f"bad_{case}.hdf5"
     kw = {"revision": {"revision": 5}, "element": {"element_type": "DSG4"},
           "material": {"material_type": "pointmass"}, "elemload": {"elem_load": True},
-          "interface": {"interface_type": "InterfaceKirch"}}[case]
+          "interface": {"interface_type": "InterfaceKirch"}, "nodes_rank2": {"nodes_rank2": True}}[case]
     write_input(path, m, **kw)
     with pytest.raises(_lib.ElpError, match=match):
         h5.read_model(ctx, path)

@@ -344,3 +344,57 @@ def test_direct_solve_residual_config0():
     num = O.Numbering(m)
     rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * 250.0, num)
     assert np.linalg.norm(F - O.csr(rp, ci, A, num.n) @ X[0]) < 1e-12 * np.linalg.norm(F)
+
+
+# ---------------------------------------------------------------- Robin impedance term
+def _tube_error(nz, quadratic, freq, Z, Z_oracle=None, lumped=None):
+    """Max error of the oracle's impedance-tube solution (elPaSo's dynamic
+    matrix with p = 1 prescribed at z = 0, SuperLU) against the closed form,
+    relative to the largest |p|."""
+    m = MG.impedance_tube(nz, quadratic=quadratic, Z=Z, lumped=lumped)
+    num = O.Numbering(m)
+    pat = O.pattern(m, num)
+    rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * freq, num, pat, Z=Z_oracle)
+    x = sla.spsolve(O.csr(rp, ci, A, num.n).tocsc(), F)
+    ids = np.sort(m.node_ids)
+    z = num.xyz_by_rank(m)[:, 2]
+    exact = MG.impedance_tube_field(z, freq, Z)
+    return np.abs(x[num.row(ids, 16)] - exact).max() / np.abs(exact).max()
+
+
+RHO_C = 1.21 * 343.0
+
+
+@pytest.mark.parametrize("Z", [RHO_C * (1 + 0.5j), RHO_C, RHO_C * (0.3 + 1.5j)])
+@pytest.mark.parametrize("quadratic,sizes,rate", [(False, (20, 40, 80), 4.0), (True, (4, 8, 16), 16.0)])
+def test_impedance_tube_closed_form(Z, quadratic, sizes, rate):
+    """The Robin term (i w rho / Z) C_Gamma of elementfluid3d.cpp:324-369 in a
+    1-D impedance tube: the FE field converges to p = A e^{-ikz} + B e^{ikz}
+    with reflection coefficient (Z - rho c)/(Z + rho c) at the rate of the
+    element order (h^2 for Fluid8, h^4 for Fluid20)."""
+    e = [_tube_error(n, quadratic, 300.0, Z) for n in sizes]
+    for a, b in zip(e, e[1:]):
+        assert 0.85 * rate < a / b < 1.2 * rate, e
+    assert e[-1] < (2.5e-2 if not quadratic else 3e-4), e
+
+
+def test_impedance_tube_matched_termination_has_no_standing_wave():
+    """Z = rho c absorbs the wave completely: |p| = 1 along the tube."""
+    m = MG.impedance_tube(64, quadratic=True)
+    num = O.Numbering(m)
+    rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * 400.0, num)
+    x = sla.spsolve(O.csr(rp, ci, A, num.n).tocsc(), F)
+    assert np.abs(np.abs(x) - 1.0).max() < 1e-5
+
+
+@pytest.mark.parametrize("mistake", ["sign of i", "power of omega", "missing rho"])
+def test_impedance_tube_catches_robin_mistakes(mistake):
+    """A Robin coefficient with the sign of i flipped, omega^2 instead of
+    omega, or without rho (each planted by handing the oracle a changed
+    impedance) misses the closed form by far more than the discretization
+    error of test_impedance_tube_closed_form."""
+    Z, w = RHO_C * (1 + 0.5j), 2 * np.pi * 300.0
+    Zm = {"sign of i": -Z, "power of omega": Z / w, "missing rho": Z * 1.21}[mistake]
+    for quadratic, nz in ((False, 40), (True, 16)):
+        assert _tube_error(nz, quadratic, 300.0, Z) < 8e-3
+        assert _tube_error(nz, quadratic, 300.0, Z, Z_oracle=Zm) > 0.15
