@@ -50,6 +50,7 @@ int elp_ctx_create(elp_ctx **out, int device) {
   c->device = device;
   ELP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   elp::upload_tables(c.get());
+  elp::post_scratch_init(device);
   auto *h = new elp_ctx();
   h->c = c;
   *out = h;
@@ -355,6 +356,18 @@ int elp_node_results(elp_model *mm, int64_t n_freq, const double *freqs_hz, cons
   ELP_HIP(hipSetDevice(mm->m->ctx->device));
   std::lock_guard<std::mutex> lk(m.mu);
   elp::node_results(m, n_freq, freqs_hz, X, cells, nodes, kernel_ms);
+  ELP_CATCH
+}
+
+int elp_node_results_noalloc(elp_model *mm, int64_t n_freq, const double *freqs_hz, const double *X_dev,
+                             double *nodes_dev) {
+  ELP_TRY
+  auto &m = M(mm);
+  ELP_ARG(n_freq > 0 && freqs_hz && X_dev && nodes_dev, "invalid arguments");
+  ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
+  ELP_HIP(hipSetDevice(mm->m->ctx->device));
+  std::lock_guard<std::mutex> lk(m.mu);
+  elp::node_results_noalloc(m, n_freq, freqs_hz, X_dev, nodes_dev);
   ELP_CATCH
 }
 

@@ -240,6 +240,11 @@ struct Model {
   DArr<int32_t> d_x_rowptr, d_x_col, d_x_fullpos, d_xc_ptr, d_xc_idx, d_face_nodes, d_if_fnodes;
   DArr<uint8_t> d_x_type, d_row_field;
   DArr<double> d_x_val, d_fbuf, d_if_factor;
+  // node results: node -> (kind slot, entry of the kind's node list) and the
+  // shape-derivative tables (fluid kinds at their nodes, bricks at the centroid)
+  DArr<int8_t> d_post_slot;
+  DArr<int32_t> d_post_pos;
+  DArr<double> d_T8, d_T20, d_Tc;
   // solver-side state (built lazily)
   struct SolverState;
   std::shared_ptr<SolverState> solver;
@@ -259,6 +264,12 @@ void dynamic_matrix(Model &m, double omega, std::complex<double> Z, double *a, d
 // element stresses (bricks) and node results (sigma, grad p) of solutions X
 void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes,
                   double *kernel_ms);
+// node results of device-resident solutions into device memory, with the
+// library's static per-device scratch instead of allocations
+void node_results_noalloc(Model &m, int64_t nf, const double *freqs, const double *X_dev, double *nodes_dev);
+void post_prepare(Model &m);          // per-model tables of the node results (at finalize)
+void post_scratch_init(int device);   // address of the static scratch on `device` (current)
+int64_t post_scratch_bytes();
 int64_t brick_element_count(const Model &m);
 
 }  // namespace elp

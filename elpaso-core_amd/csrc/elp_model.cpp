@@ -502,6 +502,7 @@ void finalize_model(Model &m) {
   m.d_face_nodes.upload(c, m.face_nodes);
   m.d_if_fnodes.upload(c, m.if_fnodes);
   m.d_if_factor.upload(c, m.if_factor);
+  post_prepare(m);
   m.finalized = true;
 }
 

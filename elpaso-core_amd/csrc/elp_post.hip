@@ -17,6 +17,8 @@
 #include <cmath>
 #include <complex>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "elp_fem.h"
@@ -251,6 +253,116 @@ __global__ __launch_bounds__(kPostThreads) void node_results_kernel(int64_t n_no
   for (int i = 0; i < 9; i++) o[i] = acc[i];
 }
 
+// visco scale of D per frequency: E_vis = E (1 + i eta) (type 1) or
+// E (1 + i omega eta) (type 2) (materialstructurevisco.cpp:283-285); D is
+// linear in E.  Up to kScaleBatch frequencies per launch, passed by value.
+constexpr int kScaleBatch = 64;
+struct ScaleArgs {
+  double w[kScaleBatch];
+};
+__global__ void visco_scale_kernel(ScaleArgs a, int nc, int has_visco, int visco_type, double eta,
+                                   double2 *__restrict__ out) {
+  const int f = threadIdx.x;
+  if (f >= nc) return;
+  const double im = has_visco ? (visco_type == 2 ? a.w[f] * eta : eta) : 0.0;
+  out[f] = double2{1.0, im};
+}
+
+// Scratch of elp_node_results_noalloc: the scale factors of a chunk of
+// frequencies and the brick centroid stresses of the chunk, which
+// node_results_kernel averages per node.  Each device holds its own copy for
+// the life of the process; host code uses the address hipGetSymbolAddress
+// returns for the current device (kept in g_scratch_ptr), never the symbol.
+constexpr int64_t kScratchBytes = 64ll << 20;
+constexpr int64_t kScratchScaleBytes = kScaleBatch * (int64_t)sizeof(double2);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+}  // namespace
+
+void post_scratch_init(int device) {
+  ELP_REQUIRE(device >= 0 && device < kMaxDevices, ELP_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  ELP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the caller made `device` current
+  g_scratch_ptr[device] = p;
+}
+
+int64_t post_scratch_bytes() { return kScratchBytes; }
+
+// node -> (kind slot, entry of the kind's node list) and the derivative
+// tables (fluid kinds at their nodes, bricks at the centroid), once per model
+void post_prepare(Model &m) {
+  Ctx *c = m.ctx;
+  const int64_t N = m.n_nodes;
+  std::vector<int8_t> slot(N, -1);
+  std::vector<int32_t> pos(N, 0);
+  for (size_t k = 0; k < m.kinds.size(); k++)
+    for (size_t i = 0; i < m.kinds[k].nodes.size(); i++) {
+      slot[m.kinds[k].nodes[i]] = (int8_t)k;
+      pos[m.kinds[k].nodes[i]] = (int32_t)i;
+    }
+  m.d_post_slot.upload(c, slot);
+  m.d_post_pos.upload(c, pos);
+  m.d_T8.upload(c, hex_nodal_derivs(8));
+  m.d_T20.upload(c, hex_nodal_derivs(20));
+  const std::vector<std::array<double, 3>> centroid(1, std::array<double, 3>{0.0, 0.0, 0.0});
+  m.d_Tc.upload(c, hex_derivs_at(8, centroid));
+}
+
+namespace {
+
+// The kernels of one chunk of nc frequencies: solutions dX [f][row], node
+// results into dN [f][node][9] (may be null), brick stresses into res[k]
+// [f][element][6] of each brick kind k.
+void launch_post(Model &m, int nc, const double *w, const double2 *dX, double2 *dN, double2 *const *res,
+                 double2 *dScale, hipStream_t s) {
+  const int64_t N = m.n_nodes, n = m.n;
+  ScaleArgs sa{};
+  for (int f = 0; f < nc; f++) sa.w[f] = w[f];
+  hipLaunchKernelGGL(visco_scale_kernel, dim3(1), dim3(kScaleBatch), 0, s, sa, nc, (int)m.has_visco, m.visco_type,
+                     m.eta, dScale);
+  ELP_HIP(hipGetLastError());
+  PostKinds K;
+  for (size_t k = 0; k < m.kinds.size(); k++) {
+    KindBlock &kb = m.kinds[k];
+    K.k[k] = PostKind{kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, res[k], kb.n_elem, kb.npe,
+                      (int)is_fluid(kb.kind)};
+    if (kb.n_elem == 0 || is_fluid(kb.kind)) continue;
+    dim3 grid((unsigned)((kb.n_elem + kPostThreads - 1) / kPostThreads), (unsigned)nc);
+    hipLaunchKernelGGL(brick_centroid_kernel, grid, dim3(kPostThreads), 0, s, kb.n_elem, kb.npe, kb.d_conn.p,
+                       m.d_xyz.p, m.d_node_row.p, kb.d_params.p, m.d_Tc.p, dScale, n, dX, res[k]);
+    ELP_HIP(hipGetLastError());
+  }
+  if (!dN) return;
+  dim3 grid((unsigned)((N + kPostThreads - 1) / kPostThreads), (unsigned)nc);
+  hipLaunchKernelGGL(node_results_kernel, grid, dim3(kPostThreads), 0, s, N, m.d_post_slot.p, m.d_post_pos.p, K, dN);
+  ELP_HIP(hipGetLastError());
+  for (auto &kb : m.kinds) {
+    const int64_t nk = (int64_t)kb.nodes.size();
+    if (!is_fluid(kb.kind) || nk == 0) continue;
+    dim3 g1((unsigned)((nk + kPostThreads - 1) / kPostThreads));
+    if (kb.kind == ELP_FLUID8)
+      hipLaunchKernelGGL(fluid_node_gradp_kernel<8>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p, kb.d_adj_ptr.p,
+                         kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p, m.d_node_row.p, m.d_T8.p, nc, n,
+                         N, dX, dN);
+    else
+      hipLaunchKernelGGL(fluid_node_gradp_kernel<20>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p,
+                         kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p, m.d_node_row.p,
+                         m.d_T20.p, nc, n, N, dX, dN);
+    ELP_HIP(hipGetLastError());
+  }
+}
+
+int64_t brick_kinds_of(const Model &m) {
+  int64_t b = 0;
+  for (auto &kb : m.kinds) b += !is_fluid(kb.kind);
+  return b;
+}
+
 }  // namespace
 
 void node_results(Model &m, int64_t nf, const double *freqs, const double *X, double *cells, double *nodes,
@@ -259,91 +371,39 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
   Ctx *c = m.ctx;
   hipStream_t s = c->stream;
   const int64_t N = m.n_nodes, n = m.n;
-  // node -> (kind slot, entry of the kind's node list)
-  std::vector<int8_t> slot(N, -1);
-  std::vector<int32_t> pos(N, 0);
-  for (size_t k = 0; k < m.kinds.size(); k++)
-    for (size_t i = 0; i < m.kinds[k].nodes.size(); i++) {
-      slot[m.kinds[k].nodes[i]] = (int8_t)k;
-      pos[m.kinds[k].nodes[i]] = (int32_t)i;
-    }
-  DArr<int8_t> d_slot;
-  DArr<int32_t> d_pos;
-  d_slot.upload(c, slot);
-  d_pos.upload(c, pos);
-  // derivative tables: fluid kinds at their nodes, bricks at the centroid
-  DArr<double> d_T8, d_T20, d_Tc;
-  d_T8.upload(c, hex_nodal_derivs(8));
-  d_T20.upload(c, hex_nodal_derivs(20));
-  const std::vector<std::array<double, 3>> centroid(1, std::array<double, 3>{0.0, 0.0, 0.0});
-  d_Tc.upload(c, hex_derivs_at(8, centroid));
   // frequency chunk: solutions, node results and brick stresses within
   // about 4 GiB (the fluid gradient operators are reused across a chunk)
   int64_t per_f = n * 16 + (nodes ? N * 9 * 16 : 0);
   for (auto &kb : m.kinds)
     if (!is_fluid(kb.kind)) per_f += kb.n_elem * 6 * 16;
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nf, (int64_t)(4LL << 30) / per_f, 65535}));
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nf, (int64_t)(4LL << 30) / per_f, kScaleBatch}));
   DArr<double2> dX, dN, dScale;
   std::vector<DArr<double2>> dres(m.kinds.size());
   dX.alloc(c, chunk * n);
   if (nodes) dN.alloc(c, chunk * N * 9);
-  dScale.alloc(c, chunk);
+  dScale.alloc(c, kScaleBatch);
+  std::vector<double2 *> res(m.kinds.size(), nullptr);
   for (size_t k = 0; k < m.kinds.size(); k++)
-    if (!is_fluid(m.kinds[k].kind)) dres[k].alloc(c, chunk * m.kinds[k].n_elem * 6);
-  int64_t brick_cols = 0, brick_kinds = 0;  // brick elements, ascending id
-  for (auto &kb : m.kinds)
-    if (!is_fluid(kb.kind)) brick_cols += kb.n_elem, brick_kinds++;
-  ELP_REQUIRE(!cells || brick_kinds <= 1, ELP_EINVAL,
+    if (!is_fluid(m.kinds[k].kind)) {
+      dres[k].alloc(c, chunk * m.kinds[k].n_elem * 6);
+      res[k] = dres[k].p;
+    }
+  const int64_t brick_cols = brick_element_count(m);
+  ELP_REQUIRE(!cells || brick_kinds_of(m) <= 1, ELP_EINVAL,
               "element stresses of a model mixing Brick8 and Brick20 elements are not supported");
-  std::vector<double2> scale(chunk);
   std::unique_ptr<HipEvent> ev0, ev1;
   if (kernel_ms) {
     *kernel_ms = 0.0;
     ev0.reset(new HipEvent);
     ev1.reset(new HipEvent);
   }
+  std::vector<double> w(kScaleBatch);
   for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
     const int nc = (int)std::min(chunk, nf - f0);
     ELP_HIP(hipMemcpyAsync(dX.p, X + 2 * f0 * n, sizeof(double2) * nc * n, hipMemcpyHostToDevice, s));
-    for (int f = 0; f < nc; f++) {
-      // E_vis = E (1 + i eta) (type 1) or E (1 + i omega eta) (type 2)
-      // (materialstructurevisco.cpp:283-285); D is linear in E
-      const double w = 2.0 * M_PI * freqs[f0 + f];
-      const double im = m.has_visco ? (m.visco_type == 2 ? w * m.eta : m.eta) : 0.0;
-      scale[f] = double2{1.0, im};
-    }
-    ELP_HIP(hipMemcpyAsync(dScale.p, scale.data(), sizeof(double2) * nc, hipMemcpyHostToDevice, s));
+    for (int f = 0; f < nc; f++) w[f] = 2.0 * M_PI * freqs[f0 + f];
     if (kernel_ms) ELP_HIP(hipEventRecord(ev0->e, s));
-    PostKinds K;
-    for (size_t k = 0; k < m.kinds.size(); k++) {
-      KindBlock &kb = m.kinds[k];
-      K.k[k] = PostKind{kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, dres[k].p, kb.n_elem, kb.npe,
-                        (int)is_fluid(kb.kind)};
-      if (kb.n_elem == 0 || is_fluid(kb.kind)) continue;
-      dim3 grid((unsigned)((kb.n_elem + kPostThreads - 1) / kPostThreads), (unsigned)nc);
-      hipLaunchKernelGGL(brick_centroid_kernel, grid, dim3(kPostThreads), 0, s, kb.n_elem, kb.npe, kb.d_conn.p,
-                         m.d_xyz.p, m.d_node_row.p, kb.d_params.p, d_Tc.p, dScale.p, n, dX.p, dres[k].p);
-      ELP_HIP(hipGetLastError());
-    }
-    if (nodes) {
-      dim3 grid((unsigned)((N + kPostThreads - 1) / kPostThreads), (unsigned)nc);
-      hipLaunchKernelGGL(node_results_kernel, grid, dim3(kPostThreads), 0, s, N, d_slot.p, d_pos.p, K, dN.p);
-      ELP_HIP(hipGetLastError());
-      for (auto &kb : m.kinds) {
-        const int64_t nk = (int64_t)kb.nodes.size();
-        if (!is_fluid(kb.kind) || nk == 0) continue;
-        dim3 g1((unsigned)((nk + kPostThreads - 1) / kPostThreads));
-        if (kb.kind == ELP_FLUID8)
-          hipLaunchKernelGGL(fluid_node_gradp_kernel<8>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p,
-                             kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p,
-                             m.d_node_row.p, d_T8.p, nc, n, N, dX.p, dN.p);
-        else
-          hipLaunchKernelGGL(fluid_node_gradp_kernel<20>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p,
-                             kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p,
-                             m.d_node_row.p, d_T20.p, nc, n, N, dX.p, dN.p);
-        ELP_HIP(hipGetLastError());
-      }
-    }
+    launch_post(m, nc, w.data(), dX.p, nodes ? dN.p : nullptr, res.data(), dScale.p, s);
     if (kernel_ms) ELP_HIP(hipEventRecord(ev1->e, s));
     if (nodes)
       ELP_HIP(hipMemcpyAsync(nodes + 2 * f0 * N * 9, dN.p, sizeof(double2) * nc * N * 9, hipMemcpyDeviceToHost, s));
@@ -367,6 +427,47 @@ void node_results(Model &m, int64_t nf, const double *freqs, const double *X, do
       *kernel_ms += ms;
     }
   }
+}
+
+void node_results_noalloc(Model &m, int64_t nf, const double *freqs, const double *X_dev, double *nodes_dev) {
+  ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
+  Ctx *c = m.ctx;
+  hipStream_t s = c->stream;
+  const int64_t N = m.n_nodes, n = m.n;
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = static_cast<unsigned char *>(g_scratch_ptr[c->device]);
+  }
+  ELP_REQUIRE(base, ELP_ESTATE, "scratch of the device not set up (create the context first)");
+  // per frequency: the centroid stresses of every brick, each kind's block
+  // starting on a 256-byte boundary
+  int64_t per_f = 0;
+  for (auto &kb : m.kinds)
+    if (!is_fluid(kb.kind)) per_f += (kb.n_elem * 6 * 16 + 255) / 256 * 256;
+  const int64_t room = kScratchBytes - kScratchScaleBytes;
+  ELP_REQUIRE(per_f <= room, ELP_EINVAL,
+              "the model's " + std::to_string(brick_element_count(m)) + " brick elements need " +
+                  std::to_string(per_f) + " bytes of scratch per frequency; the allocation-free node results hold " +
+                  std::to_string(room) + " (at most " + std::to_string(room / 96) + " bricks)");
+  const int64_t chunk = std::min<int64_t>({nf, per_f ? room / per_f : (int64_t)kScaleBatch, kScaleBatch});
+  double2 *dScale = reinterpret_cast<double2 *>(base);
+  std::vector<double2 *> res(m.kinds.size(), nullptr);
+  double w[kScaleBatch];
+  for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
+    const int nc = (int)std::min(chunk, nf - f0);
+    unsigned char *q = base + kScratchScaleBytes;
+    for (size_t k = 0; k < m.kinds.size(); k++) {
+      const KindBlock &kb = m.kinds[k];
+      if (is_fluid(kb.kind)) continue;
+      res[k] = reinterpret_cast<double2 *>(q);
+      q += (nc * kb.n_elem * 6 * 16 + 255) / 256 * 256;
+    }
+    for (int f = 0; f < nc; f++) w[f] = 2.0 * M_PI * freqs[f0 + f];
+    launch_post(m, nc, w, reinterpret_cast<const double2 *>(X_dev) + f0 * n,
+                reinterpret_cast<double2 *>(nodes_dev) + f0 * N * 9, res.data(), dScale, s);
+  }
+  ELP_HIP(hipStreamSynchronize(s));
 }
 
 int64_t brick_element_count(const Model &m) {

@@ -21,7 +21,7 @@ EXPORTED_SYMBOLS = [
     "elp_model_add_dirichlet", "elp_model_set_rayleigh", "elp_model_set_box", "elp_model_finalize",
     "elp_model_info", "elp_model_dof_map", "elp_assemble", "elp_assemble_timed", "elp_get_pattern",
     "elp_get_values", "elp_get_device_csr", "elp_dynamic_matrix", "elp_element_matrices", "elp_sweep",
-    "elp_results_sizes", "elp_node_results",
+    "elp_results_sizes", "elp_node_results", "elp_node_results_noalloc",
     "elp_server_create", "elp_server_destroy", "elp_server_submit", "elp_server_poll", "elp_server_wait",
     "elp_server_stats",
 ]
@@ -97,6 +97,7 @@ def load_library():
     L.elp_sweep.argtypes = [vp, i64, vp, dbl, dbl, P(SolveOpts), vp, vp, vp, P(SweepStats)]
     L.elp_results_sizes.argtypes = [vp, P(i64), P(i64)]
     L.elp_node_results.argtypes = [vp, i64, vp, vp, vp, vp, P(dbl)]
+    L.elp_node_results_noalloc.argtypes = [vp, i64, vp, vp, vp]
     L.elp_server_create.argtypes = [vp, i64, C.c_int, P(vp)]
     L.elp_server_destroy.argtypes = [vp]
     L.elp_server_destroy.restype = None
@@ -321,6 +322,19 @@ class GpuModel:
         if timing is not None:
             timing["kernel_ms"] = kms.value
         return c, nd
+
+
+    def node_results_noalloc(self, X_dev, freqs, nodes_dev):
+        """elp_node_results_noalloc: X_dev and nodes_dev are device addresses
+        (ints) of n_freq x n and n_freq x nodes x 9 complex arrays."""
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        _check(load_library().elp_node_results_noalloc(self._h, len(freqs), _ptr(freqs), C.c_void_p(X_dev),
+                                                       C.c_void_p(nodes_dev)))
+
+    def results_sizes(self):
+        nn, nb = C.c_int64(), C.c_int64()
+        _check(load_library().elp_results_sizes(self._h, C.byref(nn), C.byref(nb)))
+        return nn.value, nb.value
 
 
 class Server:

@@ -218,6 +218,22 @@ int elp_results_sizes(const elp_model *m, int64_t *n_nodes, int64_t *n_brick_ele
 int elp_node_results(elp_model *m, int64_t n_freq, const double *freqs_hz, const double *X,
                      double *cells, double *nodes, double *kernel_ms);
 
+/* Allocation-free variant of elp_node_results, nodes only, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It has no counterpart in the
+ * reference.  X_dev: device pointer, n_freq x n interleaved complex
+ * solutions; nodes_dev: device pointer, n_freq x n_nodes x 9 complex, the
+ * layout of elp_node_results' nodes.  Same kernels and bits as
+ * elp_node_results.  Partial results (the brick centroid stresses) go to a
+ * static scratch array of 64 MiB per GPU, held for the life of the process:
+ * ELP_EINVAL when one frequency of the model's bricks does not fit
+ * (about 699,000 brick elements); frequencies are processed in chunks that
+ * fit.  The call allocates nothing, runs on the context's stream and
+ * returns when the results are written.  Calls on one GPU share its scratch:
+ * calls that overlap on one GPU (from several threads) race. */
+int elp_node_results_noalloc(elp_model *m, int64_t n_freq, const double *freqs_hz, const double *X_dev,
+                             double *nodes_dev);
+
 /* ---- serving: continuous batching of sweep requests ----------------------------- */
 int elp_server_create(elp_ctx *ctx, int64_t hbm_budget_bytes, int max_slots, elp_server **out);
 void elp_server_destroy(elp_server *s);
