@@ -393,7 +393,13 @@ int elp_sweep(elp_model *mm, int64_t n_freq, const double *freqs_hz, double z_re
   if (z_re == 0 && z_im == 0) Z = m.Z;
   std::lock_guard<std::mutex> lk(m.mu);
   if (!m.assembled) elp::assemble_model(m, m.ctx->stream);
-  elp::sweep(m, n_freq, freqs_hz, Z, o, X, relres, iters, stats, m.ctx->stream);
+  const int64_t bad = elp::sweep(m, n_freq, freqs_hz, Z, o, X, relres, iters, stats, m.ctx->stream);
+  if (bad) {
+    elp::set_last_error(std::to_string(bad) + " of " + std::to_string(n_freq) +
+                        " frequencies did not reach the relative residual " + std::to_string(o.rtol) +
+                        " (their best solutions are returned)");
+    return ELP_ENOCONV;
+  }
   ELP_CATCH
 }
 

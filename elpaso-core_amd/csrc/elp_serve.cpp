@@ -67,8 +67,9 @@ struct Server {
       j.relres.assign(nf, 0.0);
       j.iters.assign(nf, 0);
       elp_sweep_stats st{};
-      sweep(m, nf, j.freqs.data(), j.Z, j.opts, j.X.data(), j.relres.data(), j.iters.data(), &st, s);
-      j.status = 0;
+      const int64_t bad = sweep(m, nf, j.freqs.data(), j.Z, j.opts, j.X.data(), j.relres.data(), j.iters.data(), &st, s);
+      j.status = bad ? ELP_ENOCONV : 0;
+      if (bad) j.error = std::to_string(bad) + " of " + std::to_string(nf) + " frequencies did not reach the tolerance";
     } catch (const Error &e) {
       j.status = e.code;
       j.error = e.what();
@@ -246,10 +247,11 @@ int elp_server_wait(elp_server *sv, int64_t job_id, double *X, double *relres, i
   }
   sv->s.cv_done.notify_all();  // a destroy may be waiting for the waiters
   if (latency_s) *latency_s = std::chrono::duration<double>(j->t_done - j->t_submit).count();
-  if (j->status != 0) throw Error(j->status, j->error);
+  if (j->status != 0 && j->status != ELP_ENOCONV) throw Error(j->status, j->error);
   if (X) std::memcpy(X, j->X.data(), j->X.size() * sizeof(double));
   if (relres) std::memcpy(relres, j->relres.data(), j->relres.size() * sizeof(double));
   if (iters) std::memcpy(iters, j->iters.data(), j->iters.size() * sizeof(int32_t));
+  if (j->status == ELP_ENOCONV) throw Error(j->status, j->error);  // results are filled
   ELP_CATCH
 }
 

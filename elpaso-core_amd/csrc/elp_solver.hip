@@ -234,7 +234,10 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
 }
 
 constexpr int kRowPad = 8;  // padded rows hold a multiple of this many entries
-constexpr int64_t kRowOpMaxRows = (1LL << 31) / 256;  // x of B = 32 within 31-bit byte offsets
+// x of B = 32 frequencies (256 bytes per row) within 31-bit byte offsets, so
+// that n * 256 fits the int and uint32 offsets of the padded operator
+constexpr int64_t kRowOpMaxRows = ((1LL << 31) - 1) / 256;
+static_assert(kRowOpMaxRows * 256 <= INT32_MAX, "padded operator offsets overflow");
 
 // Row-padded copy of the FP32 operator for row_op_kernel: each row holds a
 // multiple of kRowPad entries; entries store the byte offset of their column's
@@ -771,26 +774,78 @@ __global__ void gather_cols_kernel(int64_t n, int B, int CB, const int *__restri
   xc[i] = x[(i / CB) * B + map[i % CB]];
 }
 __global__ void scatter_residual_kernel(int64_t n, int B, int CB, int nfz, const int *__restrict__ map,
-                                        const double2 *__restrict__ F, const double2 *__restrict__ y,
+                                        const double2 *__restrict__ b64, const double2 *__restrict__ y,
                                         double2 *__restrict__ r64) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * CB) return;
   const int f = (int)(i % CB);
   if (f >= nfz) return;
   const int64_t r = i / CB;
-  const double2 fv = F[r], yv = y[i];
-  r64[r * B + map[f]] = {fv.x - yv.x, fv.y - yv.y};
+  const double2 bv = b64[r * B + map[f]], yv = y[i];
+  r64[r * B + map[f]] = {bv.x - yv.x, bv.y - yv.y};
 }
 
-// columns taking a new frequency: x64 = 0, r64 = b
-__global__ void slot_fresh_kernel(int64_t n, int B, const int *__restrict__ fresh, const double2 *__restrict__ F,
+// columns taking a new frequency: x64 = g (the prescribed values on
+// Dirichlet rows, 0 elsewhere; g = nullptr: all 0) and r64 = b - A x64,
+// which is b with the Dirichlet rows zeroed (identity rows, zeroed columns)
+__global__ void slot_fresh_kernel(int64_t n, int B, const int *__restrict__ fresh, const double2 *__restrict__ b64,
+                                  const double2 *__restrict__ g, const uint8_t *__restrict__ field,
                                   double2 *__restrict__ x64, double2 *__restrict__ r64) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * B) return;
   const int f = (int)(i % B);
   if (!fresh[f]) return;
-  x64[i] = {0, 0};
-  r64[i] = F[i / B];
+  const int64_t r = i / B;
+  if (g) {
+    x64[i] = g[r];
+    r64[i] = field[r] == 2 ? double2{0, 0} : b64[i];
+  } else {
+    x64[i] = {0, 0};
+    r64[i] = b64[i];
+  }
+}
+
+// Prescribed nonzero values (MatZeroRows + F[row] = value,
+// analysis.cpp:419-454) lifted into the right-hand side of a fresh column f:
+// b[r][f] = F[r] - sum over Dirichlet columns c of A(w_f)[r][c] g[c] on
+// the free rows r that touch a Dirichlet column (lift_rows); the solver's
+// operator has those columns zeroed, so it stays complex symmetric.  A uses
+// the model's unzeroed K, M and extra entries with the frequency's
+// coefficients, as the operator does.
+__global__ void lift_rhs_kernel(int64_t nl, const int32_t *__restrict__ lift_rows, int B,
+                                const int *__restrict__ fresh, const int32_t *__restrict__ rowptr,
+                                const int32_t *__restrict__ colind, const uint8_t *__restrict__ field,
+                                const double *__restrict__ K, const double *__restrict__ M,
+                                const int32_t *__restrict__ xrp, const int32_t *__restrict__ xcol,
+                                const uint8_t *__restrict__ xtype, const double *__restrict__ xval,
+                                const double2 *__restrict__ g, const OpCoef *__restrict__ coef,
+                                const double2 *__restrict__ F, double2 *__restrict__ b64) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl * B) return;
+  const int f = (int)(i % B);
+  if (!fresh[f]) return;
+  const int32_t r = lift_rows[i / B];
+  const OpCoef &cf = coef[f];
+  const bool fl = field[r] == 1;
+  const double2 ck = fl ? cf.kf : cf.ks, cm = fl ? cf.mf : cf.ms;
+  double2 acc = {0, 0};
+  for (int32_t p = rowptr[r]; p < rowptr[r + 1]; p++) {
+    const int32_t c = colind[p];
+    if (field[c] != 2 || c == r) continue;
+    const double2 a = {ck.x * K[p] + cm.x * M[p], ck.y * K[p] + cm.y * M[p]};
+    const double2 t = cmul(a, g[c]);
+    acc.x += t.x;
+    acc.y += t.y;
+  }
+  for (int32_t k = xrp[r]; k < xrp[r + 1]; k++) {
+    const int32_t c = xcol[k];
+    if (field[c] != 2) continue;
+    const double2 a = xtype[k] == 0 ? double2{cf.g.x * xval[k], cf.g.y * xval[k]} : double2{xval[k], 0.0};
+    const double2 t = cmul(a, g[c]);
+    acc.x += t.x;
+    acc.y += t.y;
+  }
+  b64[(int64_t)r * B + f] = {F[r].x - acc.x, F[r].y - acc.y};
 }
 
 // columns (re)starting the inner solve: x_in = 0, r_in = r64 * s_f
@@ -941,7 +996,6 @@ void default_opts(elp_solve_opts &o) {
   o.rtol = 1e-8;
   o.max_iters = 4000;
   o.inner_rtol = 3e-3;
-  o.restart = 50;
   o.shift = -1.0;  // automatic: 0 with the exact box coarse solve, 0.5 otherwise
   o.smooth = 1;
   o.coarse_sweeps = 20;
@@ -1759,6 +1813,47 @@ static void weight_l1(OpCoef &c, double theta) {
   c.ag *= theta;
 }
 
+// Nonzero prescribed values: g (the values on the Dirichlet rows, 0
+// elsewhere) and the free rows coupled to a Dirichlet column, whose
+// right-hand side is lifted per frequency (lift_rhs_kernel).  Off when every
+// prescribed value is 0: then b = F and x starts at 0, as without Dirichlet
+// rows.
+struct Lift {
+  bool on = false;
+  DArr<double2> g;
+  DArr<int32_t> rows;
+  int64_t nrows = 0;
+};
+
+static void build_lift(const Model &m, Lift &L) {
+  bool any = false;
+  for (auto &v : m.dir_vals) any |= v != std::complex<double>(0, 0);
+  if (!any) return;
+  std::vector<double2> g(m.n, double2{0, 0});
+  for (size_t k = 0; k < m.dir_rows.size(); k++) g[m.dir_rows[k]] = {m.dir_vals[k].real(), m.dir_vals[k].imag()};
+  std::vector<int32_t> rows;
+  for (int64_t r = 0; r < m.n; r++) {
+    if (m.row_field[r] == 2) continue;
+    bool touch = false;
+    for (int32_t p = m.rowptr[r]; p < m.rowptr[r + 1] && !touch; p++) touch = m.row_field[m.colind[p]] == 2;
+    for (int32_t k = m.x_rowptr[r]; k < m.x_rowptr[r + 1] && !touch; k++) touch = m.row_field[m.x_col[k]] == 2;
+    if (touch) rows.push_back((int32_t)r);
+  }
+  L.on = true;
+  L.g.upload(m.ctx, g);
+  L.rows.upload(m.ctx, rows);
+  L.nrows = (int64_t)rows.size();
+}
+
+static void launch_lift(const Model &m, const Lift &L, int B, const int *fresh, const OpCoef *coefA,
+                        const double2 *F, double2 *b64, hipStream_t s) {
+  if (!L.on || L.nrows == 0) return;
+  hipLaunchKernelGGL(lift_rhs_kernel, dim3(nblk(L.nrows * B, 256)), dim3(256), 0, s, L.nrows, L.rows.p, B, fresh,
+                     m.d_rowptr.p, m.d_colind.p, m.d_row_field.p, m.d_K.p, m.d_M.p, m.d_x_rowptr.p, m.d_x_col.p,
+                     m.d_x_type.p, m.d_x_val.p, L.g.p, coefA, F, b64);
+  ELP_HIP(hipGetLastError());
+}
+
 // One block of up to B frequencies: FP64 refinement around the inner solver.
 template <typename T, int B>
 static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
@@ -1813,10 +1908,38 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
   // the FP64 operator lives in the FP64 solver when T = float; otherwise S itself
   auto op64 = [&](const double2 *x, const double2 *b, double2 *y) { S.template op<1, double>(0, S.coefA.p, x, b, y); };
+  // residuals relative to ||F|| (= ||b|| without prescribed values)
   S.reduce(m.n, b64.p, b64.p, 1, S.red_out.p);
   S.fetch(S.red_out.p);
   std::vector<double> nb2(B);
   for (int f = 0; f < B; f++) nb2[f] = std::sqrt(S.h_red[f].x);
+  std::vector<double> nref(nb2), nrep(nb2);
+  Lift lift;
+  build_lift(m, lift);
+  double gnorm2 = 0;  // ||g||^2 of the prescribed values
+  for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
+  if (lift.on) {
+    std::vector<int> all(B, 1);
+    DArr<int> d_all;
+    d_all.upload(m.ctx, all);
+    launch_lift(m, lift, B, d_all.p, S.coefA.p, F.p, b64.p, s);
+    DArr<double2> r0;
+    r0.alloc(m.ctx, nt);
+    hipLaunchKernelGGL(slot_fresh_kernel, dim3(nblk(nt, 256)), dim3(256), 0, s, m.n, B, d_all.p, b64.p, lift.g.p,
+                       m.d_row_field.p, x64.p, r0.p);
+    ELP_HIP(hipGetLastError());
+    // stop against the free rows' lifted right-hand side, report against the
+    // whole lifted right-hand side (see Slot::ref in solve_slots)
+    S.reduce(m.n, r0.p, r0.p, 1, S.red_out.p);
+    S.fetch(S.red_out.p);
+    double gnorm2 = 0;
+    for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
+    for (int f = 0; f < B; f++) {
+      const double w0 = std::sqrt(S.h_red[f].x);
+      if (w0 > 0) nref[f] = w0;
+      nrep[f] = std::sqrt(w0 * w0 + gnorm2);
+    }
+  }
   std::vector<int> it(B, 0);
   std::vector<double> rel(B, 1.0);
   DArr<double> scl;
@@ -1832,7 +1955,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     op64(x64.p, b64.p, r64.p);
     S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
     S.fetch(S.red_out.p);
-    for (int f = 0; f < B; f++) rel[f] = nb2[f] > 0 ? std::sqrt(S.h_red[f].x) / nb2[f] : 0.0;
+    for (int f = 0; f < B; f++) rel[f] = nref[f] > 0 ? std::sqrt(S.h_red[f].x) / nref[f] : 0.0;
     // a refinement step that did not reduce the residual is taken back and
     // the frequency is not refined further
     std::vector<double> undo(B, 0.0);
@@ -1854,13 +1977,13 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
       op64(x64.p, b64.p, r64.p);
       S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
       S.fetch(S.red_out.p);
-      for (int f = 0; f < B; f++) rel[f] = nb2[f] > 0 ? std::sqrt(S.h_red[f].x) / nb2[f] : 0.0;
+      for (int f = 0; f < B; f++) rel[f] = nref[f] > 0 ? std::sqrt(S.h_red[f].x) / nref[f] : 0.0;
     }
     std::vector<int> act(B, 0);
     std::vector<double> inv(B, 0.0), wgt(B, 0.0);
     bool any = false;
     for (int f = 0; f < B; f++) {
-      const double nr = rel[f] * nb2[f];
+      const double nr = rel[f] * nref[f];
       if (f < nb && !stalled[f] && rel[f] > o.rtol && std::isfinite(rel[f])) {
         act[f] = 1;
         any = true;
@@ -1897,7 +2020,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     }
   }
   for (int f = 0; f < nb; f++) {
-    if (relres) relres[f] = rel[f];
+    if (relres) relres[f] = nrep[f] > 0 ? rel[f] * nref[f] / nrep[f] : rel[f];
     if (iters) iters[f] = it[f];
   }
   if (X) {
@@ -1952,6 +2075,10 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   x64.alloc(m.ctx, nt);
   r64.alloc(m.ctx, nt);
   if (X) col.alloc(m.ctx, m.n);
+  Lift lift;
+  build_lift(m, lift);
+  double gnorm2 = 0;  // ||g||^2 of the prescribed values
+  for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
   hipLaunchKernelGGL(bcast_rhs_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, F.p, b64.p);
   ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
   ELP_HIP(hipMemsetAsync(r64.p, 0, nt * sizeof(double2), s));
@@ -2015,6 +2142,15 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     double prev = 1;  // FP64 relative residual before the current inner solve
     double w = 0;     // ||r64|| the inner right-hand side was scaled by
     double tol = 0;   // relative tolerance of the current inner solve
+    // The stopping test is ||r64|| <= rtol ref and the reported residual is
+    // ||r64|| / rep.  Without prescribed values ref = rep = ||F||.  With
+    // lifted values the system solved is A' x = b (b = F - A[:,D] g on the
+    // free rows, g on the Dirichlet rows), x64 = g holds the Dirichlet rows
+    // exactly, so r64 lives on the free rows: ref = ||b|| of the free rows,
+    // rep = ||b||.  The values' identity rows (scale 1) and the free rows
+    // (fluid rows divided by rho w^2, structure rows of order E h) can differ
+    // by many orders, so neither part may set the scale of the other.
+    double ref = 0, rep = 0;
   };
   std::vector<Slot> sl(B);
   int64_t next = 0;
@@ -2045,6 +2181,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     sl[k].state = 1;
     sl[k].prev = 1.0;
     sl[k].w = nbn;
+    sl[k].ref = sl[k].rep = nbn;
     sl[k].tol = inner_tol(1.0);
     fresh[k] = 1;
     act[k] = 2;
@@ -2066,11 +2203,28 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       ELP_HIP(hipMemcpyAsync(X + 2 * fi * m.n, col.p, sizeof(double2) * m.n, hipMemcpyDeviceToHost, s));
     }
   };
-  // start: fill the slots, x64 = 0, r64 = b, inner rhs = b / ||b||
+  // start: fill the slots, x64 = g (0 without prescribed values), r64 = b -
+  // A x64, inner rhs = r64 / ||r64|| (= b / ||b|| without prescribed values)
   auto start_columns = [&]() {
     ELP_HIP(hipMemcpyAsync(d_fresh.p, fresh.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(slot_fresh_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, d_fresh.p, F.p, x64.p,
-                       r64.p);
+    launch_lift(m, lift, B, d_fresh.p, S.coefA.p, F.p, b64.p, s);
+    hipLaunchKernelGGL(slot_fresh_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, d_fresh.p, b64.p,
+                       lift.on ? lift.g.p : nullptr, m.d_row_field.p, x64.p, r64.p);
+    if (lift.on) {
+      // the lifted residual of a fresh column is not ||F||: scale by its norm
+      S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+      S.fetch(S.red_out.p);
+      for (int k = 0; k < B; k++) {
+        if (!fresh[k]) continue;
+        const double w0 = std::sqrt(S.h_red[k].x);
+        sl[k].w = w0;
+        sl[k].ref = w0 > 0 ? w0 : nbn;
+        sl[k].rep = std::sqrt(w0 * w0 + gnorm2);
+        sl[k].prev = w0 / sl[k].ref;
+        sl[k].tol = inner_tol(std::max(sl[k].prev, 1e-300));
+        scale[k] = w0 > 0 ? 1.0 / w0 : 0.0;
+      }
+    }
     ELP_HIP(hipMemcpyAsync(d_scale.p, scale.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     // restarted columns also get the preconditioner's first sweep (u) when
@@ -2166,7 +2320,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       if (CB == 8) S.template op_fp64_narrow<8>(d_cAn.p, xn.p, yn.p);
       else S.template op_fp64_narrow<16>(d_cAn.p, xn.p, yn.p);
       hipLaunchKernelGGL(scatter_residual_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, nfz,
-                         d_fmap.p, F.p, yn.p, r64.p);
+                         d_fmap.p, b64.p, yn.p, r64.p);
       ELP_HIP(hipGetLastError());
     } else {
       S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
@@ -2174,7 +2328,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
     S.fetch(S.red_out.p);
     std::vector<double> rel(B);
-    for (int k = 0; k < B; k++) rel[k] = std::sqrt(S.h_red[k].x) / nbn;
+    for (int k = 0; k < B; k++) rel[k] = std::sqrt(S.h_red[k].x) / (sl[k].ref > 0 ? sl[k].ref : nbn);
     // a step that did not reduce the residual is taken back
     bool any_undo = false;
     std::vector<int> undo(B, 0);
@@ -2200,12 +2354,12 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       const bool kept_back = q.broken || undo[k];
       const bool done = kept_back || rel[k] <= o.rtol || rel[k] > 0.5 * q.prev || it_f[q.f] >= o.max_iters;
       if (done) {
-        finish(k, kept_back ? q.prev : rel[k]);
+        finish(k, (kept_back ? q.prev : rel[k]) * q.ref / q.rep);
         any_new |= assign(k);
       } else {
         q.prev = rel[k];
         q.tol = inner_tol(rel[k]);
-        q.w = rel[k] * nbn;
+        q.w = rel[k] * q.ref;
         q.state = 1;
         act[k] = 2;
         scale[k] = 1.0 / q.w;
@@ -2255,13 +2409,13 @@ int64_t solver_bytes_estimate(const Model &m, int block, int precision) {
   return b + b / 4;                                    // coarse levels (about 1/8 each) and slack
 }
 
-void sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z, const elp_solve_opts &o, double *X,
-           double *relres, int32_t *iters, elp_sweep_stats *stats, hipStream_t s) {
-  ELP_REQUIRE(o.method == ELP_METHOD_COCG, ELP_EINVAL, "only the COCG method is available");
+int64_t sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z, const elp_solve_opts &o,
+              double *X, double *relres_out, int32_t *iters, elp_sweep_stats *stats, hipStream_t s) {
+  ELP_REQUIRE(o.method == ELP_METHOD_COCG, ELP_EINVAL, "method must be ELP_METHOD_COCG");
+  std::vector<double> own_rel(relres_out ? 0 : nf);
+  double *relres = relres_out ? relres_out : own_rel.data();
   ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
   for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
-  for (auto &d : m.dir_vals)
-    ELP_REQUIRE(d == std::complex<double>(0, 0), ELP_EINVAL, "the sweep solver supports zero prescribed values only");
   auto t0 = std::chrono::steady_clock::now();
   Model::SolverState &st = solver_state(m, o, Z, s);
   auto t1 = std::chrono::steady_clock::now();
@@ -2279,10 +2433,14 @@ void sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z, co
     }
   }
   auto t2 = std::chrono::steady_clock::now();
+  int64_t unconverged = 0;
+  for (int64_t k = 0; k < nf; k++) unconverged += !(relres[k] <= o.rtol);
   if (stats) {
     stats->setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     stats->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    stats->unconverged = (int32_t)std::min<int64_t>(unconverged, INT32_MAX);
   }
+  return unconverged;
 }
 
 }  // namespace elp

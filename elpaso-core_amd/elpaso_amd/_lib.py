@@ -1,8 +1,10 @@
 """ctypes bindings of the C-ABI declared in include/elpaso_amd.h."""
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -27,8 +29,33 @@ EXPORTED_SYMBOLS = [
 ]
 
 MAT_K, MAT_M, MAT_C, MAT_G, MAT_S = 0, 1, 2, 3, 4
+ENOCONV = -6
 PREC_FP64, PREC_MIXED = 0, 1
 PC_JACOBI, PC_MG = 0, 1
+
+
+# Handles still open, closed in a fixed order (servers, then models, then
+# contexts) by an exit handler that runs at the start of interpreter
+# shutdown, while every object is intact.  Objects that outlive it (held by a
+# stored traceback, a reference cycle or a module global) are then released
+# by the interpreter in no particular order; their __del__ must not call into
+# the library any more.
+_LIVE = {"server": weakref.WeakSet(), "model": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+_SHUT_DOWN = False
+
+
+def _close_all():
+    global _SHUT_DOWN
+    for kind in ("server", "model", "ctx"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _SHUT_DOWN = True
+
+
+atexit.register(_close_all)
 
 
 class ElpError(RuntimeError):
@@ -40,13 +67,14 @@ class ElpError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("method", C.c_int), ("precond", C.c_int), ("block", C.c_int),
                 ("rtol", C.c_double), ("max_iters", C.c_int), ("inner_rtol", C.c_double),
-                ("restart", C.c_int), ("shift", C.c_double), ("smooth", C.c_int),
+                ("shift", C.c_double), ("smooth", C.c_int),
                 ("coarse_sweeps", C.c_int), ("mg_levels", C.c_int), ("mg_min_nodes", C.c_int)]
 
 
 class SweepStats(C.Structure):
     _fields_ = [("setup_ms", C.c_double), ("solve_ms", C.c_double), ("spmv_calls", C.c_int64),
-                ("spmv_ms", C.c_double), ("spmv_bytes", C.c_int64), ("refinements", C.c_int)]
+                ("spmv_ms", C.c_double), ("spmv_bytes", C.c_int64), ("refinements", C.c_int),
+                ("unconverged", C.c_int)]
 
 
 def library_path():
@@ -132,6 +160,7 @@ class Context:
         L = load_library()
         self._h = C.c_void_p()
         _check(L.elp_ctx_create(C.byref(self._h), device))
+        _LIVE["ctx"].add(self)
 
     @property
     def handle(self):
@@ -143,6 +172,8 @@ class Context:
             self._h = C.c_void_p()
 
     def __del__(self):
+        if _SHUT_DOWN:
+            return
         try:
             self.close()
         except Exception:
@@ -168,6 +199,7 @@ class GpuModel:
         self.model = model
         self._h = C.c_void_p()
         _check(L.elp_model_create(ctx.handle, C.byref(self._h)))
+        _LIVE["model"].add(self)
         ids = np.ascontiguousarray(model.node_ids, dtype=np.int64)
         xyz = np.ascontiguousarray(model.xyz, dtype=np.float64)
         _check(L.elp_model_set_nodes(self._h, len(ids), _ptr(ids), _ptr(xyz)))
@@ -217,6 +249,7 @@ class GpuModel:
         self.model = None
         self._h = C.c_void_p()
         _check(load_library().elp_model_create(ctx.handle, C.byref(self._h)))
+        _LIVE["model"].add(self)
         return self
 
     def set_box(self, box):
@@ -249,6 +282,8 @@ class GpuModel:
             self._h = C.c_void_p()
 
     def __del__(self):
+        if _SHUT_DOWN:
+            return
         try:
             self.close()
         except Exception:
@@ -297,8 +332,10 @@ class GpuModel:
         it = np.zeros(nf, dtype=np.int32)
         st = SweepStats()
         Z = complex(Z)
-        _check(load_library().elp_sweep(self._h, nf, _ptr(freqs), Z.real, Z.imag, C.byref(o),
-                                        _ptr(X) if X is not None else None, _ptr(rel), _ptr(it), C.byref(st)))
+        rc = load_library().elp_sweep(self._h, nf, _ptr(freqs), Z.real, Z.imag, C.byref(o),
+                                      _ptr(X) if X is not None else None, _ptr(rel), _ptr(it), C.byref(st))
+        if rc != ENOCONV:  # ELP_ENOCONV: results are filled, st.unconverged counts the misses
+            _check(rc)
         return X, rel, it, st
 
 
@@ -348,6 +385,7 @@ class Server:
         self._n = {}
         _check(load_library().elp_server_create(ctx.handle, int(hbm_budget_bytes), int(max_slots),
                                                 C.byref(self._h)))
+        _LIVE["server"].add(self)
 
     def submit(self, gmodel: "GpuModel", freqs, Z=0j, opts=None, **kw):
         """Queue a sweep of ``gmodel`` (the server takes the model over)."""
@@ -370,8 +408,10 @@ class Server:
         rel = np.zeros(nf)
         it = np.zeros(nf, dtype=np.int32)
         lat = C.c_double()
-        _check(load_library().elp_server_wait(self._h, int(job), _ptr(X) if X is not None else None, _ptr(rel),
-                                              _ptr(it), C.byref(lat)))
+        rc = load_library().elp_server_wait(self._h, int(job), _ptr(X) if X is not None else None, _ptr(rel),
+                                            _ptr(it), C.byref(lat))
+        if rc != ENOCONV:  # ELP_ENOCONV: results are filled, relres shows the misses
+            _check(rc)
         return X, rel, it, lat.value
 
     def stats(self):
@@ -385,6 +425,8 @@ class Server:
             self._h = C.c_void_p()
 
     def __del__(self):
+        if _SHUT_DOWN:
+            return
         try:
             self.close()
         except Exception:

@@ -72,8 +72,7 @@ extern "C" {
 /* solver precision / method */
 #define ELP_PREC_FP64 0   /* complex FP64 Krylov + FP64 preconditioner, one frequency at a time */
 #define ELP_PREC_MIXED 1  /* complex FP32 Krylov/preconditioner, FP64 refinement, frequency blocks */
-#define ELP_METHOD_COCG 0
-#define ELP_METHOD_GMRES 1
+#define ELP_METHOD_COCG 0 /* the only method: A(w) is complex symmetric */
 #define ELP_PC_JACOBI 0   /* l1-Jacobi */
 #define ELP_PC_MG 1       /* shifted-Laplacian multigrid (box hierarchy) */
 
@@ -92,7 +91,6 @@ typedef struct elp_solve_opts {
   int max_iters;     /* Krylov iterations per frequency                   */
   double inner_rtol; /* mixed: FP32 inner reduction per refinement step   */
                      /* (the last step aims just past rtol when it can)   */
-  int restart;       /* GMRES restart length                              */
   double shift;      /* shifted-Laplacian damping beta (MG)               */
   int smooth;        /* MG pre/post smoothing sweeps                      */
   int coarse_sweeps; /* MG coarsest-level smoothing sweeps                */
@@ -107,6 +105,7 @@ typedef struct elp_sweep_stats {
   double spmv_ms;      /* time in the fine-level fused SpMV (HIP events)  */
   int64_t spmv_bytes;  /* algorithmic bytes of one fine-level SpMV launch */
   int refinements;     /* FP64 refinement steps (mixed)                   */
+  int unconverged;     /* frequencies above rtol (then ELP_ENOCONV)       */
 } elp_sweep_stats;
 
 /* ---- context -------------------------------------------------------------- */
@@ -183,8 +182,16 @@ int elp_element_matrices(elp_ctx *ctx, int kind, int64_t n, const double *xyz, d
                          double p1, double p2, double *Ke, double *Me);
 
 /* ---- harmonic sweep ---------------------------------------------------------- */
-/* Solves A(w_k) x_k = f_k for every frequency.  X: n_freq x n complex (may be
- * NULL), relres: FP64 relative residual per frequency, iters per frequency. */
+/* Solves A(w_k) x_k = f_k for every frequency, A and F as elp_dynamic_matrix
+ * returns them (MatZeroRows semantics: prescribed values honoured, exactly,
+ * on their rows).  X: n_freq x n complex (may be NULL); iters per frequency;
+ * relres: FP64 relative residual ||F - A x|| / ||b|| per frequency, where
+ * b = F when every prescribed value is 0, and otherwise the right-hand side of
+ * the lifted system the solver works on (F - A[:,D] g on the free rows, the
+ * values g on the Dirichlet rows D); the iteration stops when the residual is
+ * below rtol times the free rows' part of b.  Returns ELP_ENOCONV, with X,
+ * relres, iters and stats filled, when any frequency stays above
+ * opts->rtol. */
 int elp_sweep(elp_model *m, int64_t n_freq, const double *freqs_hz, double z_re,
               double z_im, const elp_solve_opts *opts, double *X, double *relres,
               int32_t *iters, elp_sweep_stats *stats);
@@ -246,6 +253,8 @@ int elp_server_submit(elp_server *s, elp_model *m, int64_t n_freq, const double 
                       double z_re, double z_im, const elp_solve_opts *opts, int64_t *job_id);
 /* 1 if finished, 0 if pending */
 int elp_server_poll(elp_server *s, int64_t job_id);
+/* ELP_ENOCONV: the request ran, X / relres / iters are filled, and some
+ * frequency stayed above its rtol */
 int elp_server_wait(elp_server *s, int64_t job_id, double *X, double *relres, int32_t *iters,
                     double *latency_s);
 int elp_server_stats(elp_server *s, int64_t *admitted, int64_t *queued, int64_t *bytes_in_use);

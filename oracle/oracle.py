@@ -433,6 +433,33 @@ def apply_dirichlet(model, num, rp, ci, A, F):
         F[r] = val
 
 
+def lifted_rhs(model, num, rp, ci, A, F):
+    """Right-hand side of the system with the prescribed values lifted out:
+    b = F - A[:, D] g on the free rows, g on the Dirichlet rows D (A, F as
+    dynamic_matrix returns them: MatZeroRows keeps the columns, so A[:, D]
+    is still in the free rows, analysis.cpp:419-454).  b = F when every
+    prescribed value is 0."""
+    D = np.zeros(num.n, dtype=bool)
+    g = np.zeros(num.n, dtype=np.complex128)
+    for nid, dof, val in model.dirichlet:
+        r = num.row([nid], dof)[0]
+        if r >= 0:
+            D[r] = True
+            g[r] = val
+    if not g.any():
+        return F.copy()
+    Am = csr(rp, ci, A, num.n)
+    b = F - Am @ g
+    b[D] = g[D]
+    return b
+
+
+def relres(model, num, rp, ci, A, F, x):
+    """||F - A x|| / ||b|| with b = lifted_rhs (the measure the sweep reports)."""
+    r = F - csr(rp, ci, A, num.n) @ x
+    return np.linalg.norm(r) / np.linalg.norm(lifted_rhs(model, num, rp, ci, A, F))
+
+
 def csr(rp, ci, vals, n):
     return sp.csr_matrix((vals, ci, rp), shape=(n, n))
 

@@ -77,13 +77,9 @@ def _oracle_close(model, g, which):
 
 
 @pytest.mark.skipif(H5DUMP is None, reason="h5dump (HDF5 tools) not installed")
-def test_reference_fluid8_fixture(ctx):
-    print("[h5] reference Fluid8 2-element fixture", flush=True)
-    g, an, counts = h5.read_model(ctx, FLUID8_2ELEM)
-    assert counts == {"nodes": 12, "materials": 1, "element_groups": 1, "elements": 2, "constraints": 1,
-                      "node_loads": 0, "interfaces": 0}
-    assert an.type == "frequency-basic" and an.steps == 10
-    # the same model from the file's contents as the HDF5 tools print them
+def _fixture_model():
+    """The reference's 2-element Fluid8 input built from the file's contents
+    as the HDF5 tools print them (independently of the library's reader)."""
     nodes = np.array(_numbers(_dump("-d", "/Nodes/mtxFemNodes", FLUID8_2ELEM))).reshape(-1, 4)
     rows = np.array(_numbers(_dump("-d", "/Elements/mtxFemElemGroup1", FLUID8_2ELEM)), np.int64).reshape(-1, 9)
     attrs = _dump("-A", "-d", "/Materials/material1", FLUID8_2ELEM)
@@ -93,7 +89,17 @@ def test_reference_fluid8_fixture(ctx):
     m = Model(node_ids=nodes[:, 0].astype(np.int64), xyz=np.ascontiguousarray(nodes[:, 1:]), name="fluid8_2elem")
     m.materials[1] = Material("fluid", rho=rho, c=c)
     m.groups.append(ElementGroup(FLUID8, 1, rows[:, 0], rows[:, 1:]))
-    m.dirichlet.append((7, 16, 1.0))  # nodeConstraint1_7: acoustic, NodeId 7, valp 1
+    m.dirichlet.append((7, 16, 1.0))  # nodeConstraint1_7: acoustic, NodeId 7, p = 1
+    return m, nodes
+
+
+def test_reference_fluid8_fixture(ctx):
+    print("[h5] reference Fluid8 2-element fixture", flush=True)
+    g, an, counts = h5.read_model(ctx, FLUID8_2ELEM)
+    assert counts == {"nodes": 12, "materials": 1, "element_groups": 1, "elements": 2, "constraints": 1,
+                      "node_loads": 0, "interfaces": 0}
+    assert an.type == "frequency-basic" and an.steps == 10
+    m, nodes = _fixture_model()
     ref = GpuModel(ctx, m)
     for x in (g, ref):
         x.assemble()
@@ -237,24 +243,33 @@ def test_reference_equivalent_fluid_fixture_is_rejected(ctx):
 
 
 def test_sweep_written_and_read_back(ctx, tmp_path):
+    """The reference's own input end to end: read, sweep on the GPU (the
+    prescribed p = 1 lifted into the right-hand side), write elPaSo's solution
+    file, read it back; the solutions match the oracle's SuperLU solve of the
+    same model built from the file's contents."""
     print("[h5] sweep of the reference fixture -> solution file", flush=True)
     g, an, _ = h5.read_model(ctx, FLUID8_2ELEM)
     g.assemble()
     freqs = an.frequencies()
-    # the sweep solver takes zero prescribed values; the dynamic matrix keeps
-    # the fixture's value 1: solve a few frequencies with numpy instead
-    rp, ci = g.pattern()
-    X = np.zeros((len(freqs), g.n), complex)
+    X, rel, it, st = g.sweep(freqs, precond=_lib.PC_JACOBI, block=8)
+    assert (rel <= 1e-8).all() and st.unconverged == 0, (rel, it)
+    m, _ = _fixture_model()
+    Xr = O.solve_direct(m, freqs)
+    num = O.Numbering(m)
     for k, f in enumerate(freqs):
-        a, fv = g.dynamic_matrix(2 * np.pi * f)
-        A = np.zeros((g.n, g.n), complex)
-        for r in range(g.n):
-            A[r, ci[rp[r]:rp[r + 1]]] = a[rp[r]:rp[r + 1]]
-        X[k] = np.linalg.solve(A, fv)
+        # elPaSo's system (MatZeroRows, F = 1 at node 7): residual at the bar,
+        # and the forward-error bound cond(A) relres against SuperLU (the
+        # 12-DOF system has cond up to 4e7 between 10 and 100 Hz)
+        rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * f, num)
+        D = O.csr(rp, ci, A, num.n).toarray()
+        r = O.relres(m, num, rp, ci, A, F, X[k])
+        assert r <= 1.5e-8, (f, r)
+        bound = 2 * np.linalg.cond(D) * max(r, 1e-16)
+        assert np.linalg.norm(X[k] - Xr[k]) <= bound * np.linalg.norm(Xr[k]), f
     ids, dofs = g.dof_map()
     out = tmp_path / "eGenOutput_fixture.hdf5"
     h5.write_solution(out, freqs, X, ids, dofs)
     for k in range(len(freqs)):
         np.testing.assert_array_equal(h5.read_complex_vector(out, f"/Solution/State/vecFemStep{k + 1}", g.n), X[k])
     r = int(np.nonzero(ids == 7)[0][0])
-    assert np.allclose(X[:, r], 1.0)  # the prescribed pressure
+    assert np.array_equal(X[:, r], np.ones(len(freqs)))  # exactly the prescribed pressure

@@ -145,3 +145,95 @@ def test_sweep_32_slots(ctx, quadratic):
     X, rel, it, st = g.sweep(freqs, block=32)
     assert (rel <= 1e-8).all(), rel
     _check_solution(m, freqs[::7], X[::7], rel[::7])
+
+
+def _check_forward_error(m, freqs, X):
+    """Small models: FP64 residual of elPaSo's system (relative to the lifted
+    right-hand side) at the bar, and the distance to SuperLU within the
+    forward-error bound cond(A) relres."""
+    num = O.Numbering(m)
+    pat = O.pattern(m, num)
+    for k, f in enumerate(freqs):
+        rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * f, num, pat)
+        Am = O.csr(rp, ci, A, num.n)
+        r = O.relres(m, num, rp, ci, A, F, X[k])
+        assert r <= 1.5e-8, (f, r)
+        x = sla.splu(Am.tocsc()).solve(F)
+        bound = 2 * np.linalg.cond(Am.toarray()) * max(r, 1e-16)
+        assert np.linalg.norm(X[k] - x) <= bound * np.linalg.norm(x), (f, bound)
+
+
+@pytest.mark.parametrize("quadratic,nz", [(False, 40), (True, 12)])
+def test_impedance_tube_sweep_prescribed_pressure(ctx, quadratic, nz):
+    """elp_sweep with a nonzero prescribed pressure (p = 1 on z = 0, lifted
+    into the right-hand side): the FP64 residual of elPaSo's system
+    (MatZeroRows, F = value) at 1e-8, agreement with SuperLU, and the
+    impedance-tube closed form within the element's discretization error."""
+    Z = 1.21 * 343.0 * (1 + 0.5j)
+    m = MG.impedance_tube(nz, quadratic=quadratic, Z=Z)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = np.array([80.0, 300.0, 650.0])
+    print(f"[sweep] impedance tube quadratic={quadratic} nz={nz}", flush=True)
+    X, rel, it, st = g.sweep(freqs, precond=_lib.PC_JACOBI, block=4, max_iters=20000)
+    assert (rel <= 1e-8).all() and st.unconverged == 0, (rel, it)
+    _check_forward_error(m, freqs, X)
+    num = O.Numbering(m)
+    ids = np.sort(m.node_ids)
+    z = num.xyz_by_rank(m)[:, 2]
+    rows = num.row(ids, 16)
+    fixed = np.isclose(z, 0.0)
+    for k, f in enumerate(freqs):
+        assert np.array_equal(X[k][rows[fixed]], np.ones(int(fixed.sum())))  # exactly the prescribed value
+        exact = MG.impedance_tube_field(z, f, Z)
+        err = np.abs(X[k][rows] - exact).max() / np.abs(exact).max()
+        # about 1.5x the discretization error of the oracle's SuperLU solution
+        # of the same mesh (1.1e-4 / 7.5e-3 / 7.0e-2 for Fluid8, 2.0e-7 /
+        # 2.2e-4 / 1.1e-2 for Fluid20 at 80 / 300 / 650 Hz)
+        lim = {False: {80.0: 2e-4, 300.0: 1.2e-2, 650.0: 0.1}, True: {80.0: 1e-6, 300.0: 4e-4, 650.0: 1.6e-2}}
+        assert err < lim[quadratic][f], (f, err)
+
+
+def test_prescribed_complex_displacements_steel(ctx):
+    """Structure rows with complex prescribed displacements on one face, a
+    clamped opposite face and a loss factor (no Rayleigh damping: elPaSo's
+    system, which the oracle restates, has none)."""
+    m = MG.steel_block(6, 5, 4, L=(0.3, 0.25, 0.2), jitter=0.1, rayleigh=(0.0, 0.0))
+    from elpaso_amd import Material
+    m.materials[1] = Material("structure", rho=7850.0, E=210e9, nu=0.3, eta=0.02)
+    xyz = m.xyz
+    for nid, p in zip(m.node_ids, xyz):
+        if np.isclose(p[0], 0.0):
+            for d in (0, 1, 2):
+                m.dirichlet.append((int(nid), d, 0.0))
+        elif np.isclose(p[0], 0.3):
+            m.dirichlet.append((int(nid), 2, 1e-4 * (1 + 0.3j) * (1 + p[1])))
+            m.dirichlet.append((int(nid), 0, -2e-5j))
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = np.array([100.0, 2500.0])
+    print("[sweep] steel block with complex prescribed displacements", flush=True)
+    X, rel, it, st = g.sweep(freqs, precond=_lib.PC_JACOBI, block=4, max_iters=40000)
+    assert (rel <= 1e-8).all(), (rel, it)
+    _check_forward_error(m, freqs, X)
+
+
+def test_unconverged_sweep_reports_enoconv(ctx):
+    """A sweep that cannot reach rtol (one Krylov iteration) returns
+    ELP_ENOCONV with its best solutions and residuals filled."""
+    m = MG.acoustic_cavity(8, 6, 5)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = np.array([120.0, 480.0])
+    L = _lib.load_library()
+    o = _lib.default_opts(block=4, max_iters=1)
+    X = np.zeros((2, g.n), complex)
+    rel = np.zeros(2)
+    it = np.zeros(2, np.int32)
+    st = _lib.SweepStats()
+    rc = L.elp_sweep(g.handle, 2, _lib._ptr(freqs), 0.0, 0.0, _lib.C.byref(o), _lib._ptr(X), _lib._ptr(rel),
+                     _lib._ptr(it), _lib.C.byref(st))
+    assert rc == _lib.ENOCONV
+    assert st.unconverged == 2 and (rel > 1e-8).all() and (rel <= 1.0).all()
+    assert np.abs(X).max() > 0
+    assert b"did not reach" in L.elp_last_error(None)

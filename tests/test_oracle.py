@@ -398,3 +398,21 @@ def test_impedance_tube_catches_robin_mistakes(mistake):
     for quadratic, nz in ((False, 40), (True, 16)):
         assert _tube_error(nz, quadratic, 300.0, Z) < 8e-3
         assert _tube_error(nz, quadratic, 300.0, Z, Z_oracle=Zm) > 0.15
+
+
+def test_lifted_rhs_matches_elimination():
+    """b = F - A[:, D] g on free rows: solving the free block against it
+    reproduces the solution of elPaSo's full system (MatZeroRows)."""
+    m = MG.impedance_tube(6, Z=RHO_C * (0.7 + 0.2j), p0=0.4 - 0.3j)
+    num = O.Numbering(m)
+    rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * 210.0, num)
+    Am = O.csr(rp, ci, A, num.n).tocsc()
+    x = sla.spsolve(Am, F)
+    b = O.lifted_rhs(m, num, rp, ci, A, F)
+    D = np.zeros(num.n, bool)
+    D[[num.row([nid], 16)[0] for nid, _, _ in m.dirichlet]] = True
+    free = ~D
+    y = sla.spsolve(Am[free][:, free].tocsc(), b[free])
+    assert np.allclose(y, x[free], rtol=1e-10, atol=0)
+    assert np.array_equal(b[D], x[D])
+    assert O.relres(m, num, rp, ci, A, F, x) < 1e-13
