@@ -25,8 +25,10 @@ struct FdmCoef {
 
 struct FdmBox;
 
-// nullptr when the model is not a uniform fluid box with Robin faces on z sides only
-std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m);
+// nullptr when the model is not a uniform fluid box with Robin faces on z
+// sides only; with fluid_part, the box describes the fluid rows of a model
+// that has other rows too (a Hex8 cavity coupled to a structure)
+std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid_part = false);
 int64_t fdm_nodes(const FdmBox &f);
 const int *fdm_dims(const FdmBox &f);
 double fdm_rho(const FdmBox &f);

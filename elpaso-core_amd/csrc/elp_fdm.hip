@@ -49,12 +49,32 @@ static void modes_1d(int n, double h, std::vector<double> &V, std::vector<double
   }
 }
 
-std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m) {
+std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid_part) {
   std::unique_ptr<FdmBox, void (*)(FdmBox *)> none(nullptr, fdm_free);
-  if (!m.box.set || m.mats.size() != 1 || !m.mats.begin()->second.fluid) return none;
-  if (!m.dir_rows.empty()) return none;
-  for (auto v : m.row_field)
-    if (v != 1) return none;
+  if (!m.box.set) return none;
+  // the box's fluid: the model's one fluid material
+  const Material *fluid = nullptr;
+  int n_fluid = 0;
+  for (auto &kv : m.mats)
+    if (kv.second.fluid) {
+      fluid = &kv.second;
+      n_fluid++;
+    }
+  if (n_fluid != 1 || (!fluid_part && m.mats.size() != 1)) return none;
+  if (!fluid_part) {
+    if (!m.dir_rows.empty()) return none;
+    for (auto v : m.row_field)
+      if (v != 1) return none;
+  } else {
+    // the box lattice covers exactly the fluid rows (a constrained fluid row
+    // is not a fluid row: the count then differs)
+    int64_t nf = 0;
+    for (auto v : m.row_field) nf += v == 1;
+    const int64_t lattice = (int64_t)(m.box.nel[0] + 1) * (m.box.nel[1] + 1) * (m.box.nel[2] + 1);
+    if (m.box.order != 1 || nf != lattice) return none;
+    for (auto &kb : m.kinds)
+      if (is_fluid(kb.kind) && kb.kind != ELP_FLUID8) return none;
+  }
   if (m.box.robin_sides & ~((1 << 2) | (1 << 4))) return none;  // Robin only on z sides
   if (m.box.nel[0] + 1 > kFdmMaxAxis || m.box.nel[1] + 1 > kFdmMaxAxis) return none;
   // the model's corner nodes must form the uniform lattice of the box
@@ -63,8 +83,8 @@ std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m) {
     f->dims[d] = m.box.nel[d] + 1;
     f->h[d] = m.box.length[d] / m.box.nel[d];
   }
-  f->rho = m.mats.begin()->second.rho;
-  f->c = m.mats.begin()->second.c;
+  f->rho = fluid->rho;
+  f->c = fluid->c;
   f->robin_lo = (m.box.robin_sides >> 2) & 1;
   f->robin_hi = (m.box.robin_sides >> 4) & 1;
   Ctx *c = m.ctx;

@@ -24,8 +24,11 @@
 // of the Hex8 box operator (elp_fdm), used directly for Hex8 models and as the
 // coarse level of a symmetric two-level cycle (l1-Jacobi pre- and post-sweep)
 // for Hex20, with the shifted operator P(w) = K/w^2 - (1 - i beta) M + ...;
-// otherwise a rediscretized Hex8 multigrid hierarchy (trilinear transfer,
-// l1-Jacobi smoothing) or l1-Jacobi alone.  Every cycle is a fixed linear
+// for a Hex8 fluid box coupled to a structure the block-diagonal
+// preconditioner of the exact box solve (fluid rows) and the block cyclic
+// reduction of the structure block (elp_bcr, factored per window of
+// frequencies); otherwise a rediscretized Hex8 multigrid hierarchy
+// (trilinear transfer, l1-Jacobi smoothing) or l1-Jacobi alone.  Every cycle is a fixed linear
 // operator and complex symmetric (R = P^T, symmetric smoothing), as COCG
 // requires.
 #include <hip/hip_runtime.h>
@@ -35,6 +38,7 @@
 #include <cmath>
 #include <thread>
 
+#include "elp_bcr.h"
 #include "elp_fdm.h"
 #include "elp_internal.h"
 #include "elp_solver.h"
@@ -984,6 +988,10 @@ struct elp::Model::SolverState {
   // rediscretized hierarchy when available)
   std::unique_ptr<elp::FdmBox, void (*)(elp::FdmBox *)> fdm{nullptr, nullptr};
   int order = 1;
+  // coupled models: the box solve on the fluid rows and the direct solve of
+  // the structure block
+  bool coupled = false, cpl_requested = false;
+  std::unique_ptr<elp::BcrPlan, void (*)(elp::BcrPlan *)> bcr{nullptr, nullptr};
 };
 
 namespace elp {
@@ -1368,7 +1376,7 @@ static void permute_level_float(LevelData &L, hipStream_t s) {
 }
 
 // trilinear interpolation from a coarse box lattice to the rows of `fine`
-static void build_interp(LevelData &F, const Model &coarse_like_box, const int nelc[3]) {
+static void build_interp(LevelData &F, const Model &coarse_like_box, const int nelc[3], bool fluid_only = false) {
   const Model &fm = *F.model;
   const BoxInfo &bx = coarse_like_box.box;
   std::vector<int32_t> rp(1, 0), ci;
@@ -1377,6 +1385,7 @@ static void build_interp(LevelData &F, const Model &coarse_like_box, const int n
   const int px = nelc[0] + 1, py = nelc[1] + 1;
   for (int64_t r = 0; r < fm.n_nodes; r++) {
     if (fm.node_row[r] < 0) continue;
+    if (fluid_only && fm.row_field[fm.node_row[r]] != 1) continue;
     const int64_t in = fm.rank_to_input[r];
     double t[3];
     int cell[3];
@@ -1440,14 +1449,27 @@ static bool mg_possible(const Model &m) {
   return m.dir_rows.empty();
 }
 
+// a Hex8 fluid box (the box hint) coupled to structure rows
+static bool coupled_possible(const Model &m) {
+  if (!m.box.set || m.box.order != 1) return false;
+  bool has_s = false, has_f = false;
+  for (auto v : m.row_field) {
+    has_s |= v == 0;
+    has_f |= v == 1;
+  }
+  return has_s && has_f;
+}
+
 static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::complex<double> Z, hipStream_t s) {
   const bool want_mg = o.precond == ELP_PC_MG && mg_possible(m);
+  const bool want_cpl = o.precond == ELP_PC_MG && !want_mg && coupled_possible(m);
   if (m.solver && m.solver->mg == want_mg && m.solver->mg_levels == o.mg_levels &&
-      m.solver->mg_min_nodes == o.mg_min_nodes)
+      m.solver->mg_min_nodes == o.mg_min_nodes && m.solver->cpl_requested == want_cpl)
     return *m.solver;
   m.solver.reset();
   auto st = std::make_shared<Model::SolverState>();
   st->mg = want_mg;
+  st->cpl_requested = want_cpl;
   st->mg_levels = o.mg_levels;
   st->mg_min_nodes = o.mg_min_nodes;
   auto L0 = std::make_unique<LevelData>();
@@ -1456,7 +1478,23 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   st->levels.push_back(std::move(L0));
   if (want_mg) st->fdm = fdm_build(m);
   st->order = m.box.order;
-  if (st->fdm) {
+  if (want_cpl) {
+    LevelData &L = *st->levels[0];
+    st->fdm = fdm_build(m, true);
+    if (st->fdm) {
+      Model lattice;
+      lattice.box = m.box;
+      build_interp(L, lattice, m.box.nel, true);
+    }
+    permute_level_float(L, s);
+    std::vector<int32_t> ip;
+    if (L.permuted) L.iperm.download(ip);
+    if (st->fdm) {
+      st->bcr = bcr_plan(m, L.permuted ? &ip : nullptr);
+      st->coupled = st->bcr != nullptr;
+    }
+    if (!st->coupled) st->fdm.reset();  // l1-Jacobi
+  } else if (st->fdm) {
     // prolongation from the Hex8 lattice of the box to the model's rows
     Model lattice;
     lattice.box = m.box;
@@ -1520,6 +1558,14 @@ struct Solver {
   DArr<V> xi, ri, zi, pi, qi;    // inner Krylov vectors
   DArr<V> fb, fe, fs1, fs2;      // exact box solve: rhs, solution, scratch (lattice size)
   DArr<FdmCoef> coefF;
+  // coupled models: the structure block's factorization in use (at
+  // frequency fac_hz) for every slot, in FP64 also for the FP32 inner solve:
+  // the stiffness of a thin plate has cond ~1e7-1e9, beyond an FP32
+  // factorization
+  std::unique_ptr<BcrFactor<double>, void (*)(BcrFactor<double> *)> fac{nullptr, nullptr};
+  double fac_hz = -1;
+  int refactors = 0;
+  double refactor_ms = 0;
   PinnedArr<double2> h_red;     // per-frequency reductions, pinned host memory
   // statistics
   HipEvent ev0, ev1;
@@ -1704,7 +1750,25 @@ struct Solver {
     smooth(0, b, W.x.p, nu, z);
   }
 
+  // factor the structure block at f_hz: A_ss = ks K + ms M
+  void refactor(double f_hz, std::complex<double> ks, std::complex<double> ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!fac) fac = bcr_new_factor<double>();
+    bcr_factor<double>(m, *st.bcr, ks, ms, *fac, s);
+    fac_hz = f_hz;
+    refactors++;
+    refactor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
   void precond(const V *r, V *z) {
+    if (st.coupled) {
+      // block diagonal: the box solve on the fluid rows (0 elsewhere), then
+      // the direct solve of the structure block on its rows; Dirichlet rows
+      // stay 0 (their residual is 0)
+      box_cycle(r, z);
+      bcr_apply<double, V>(*st.bcr, *fac, B, r, z, s, std::is_same<T, double>::value);
+      return;
+    }
     if (st.fdm) box_cycle(r, z);
     else if (st.levels.size() == 1) smooth(0, r, nullptr, std::max(2, o.smooth), z);  // l1-Jacobi: 2 sweeps
     else vcycle(0, r, z);
@@ -1854,6 +1918,14 @@ static void launch_lift(const Model &m, const Lift &L, int B, const int *fresh, 
   ELP_HIP(hipGetLastError());
 }
 
+// Structure-block factorization of the coupled preconditioner at f_hz:
+// A_ss = ks K + ms M with the model's loss factor and Rayleigh terms.
+static void plate_coefs(const Model &m, double f_hz, std::complex<double> &ks, std::complex<double> &ms) {
+  const OpCoef c = make_coef(m, 2.0 * M_PI * f_hz, std::complex<double>(0, 0), 0.0);
+  ks = {c.ks.x, c.ks.y};
+  ms = {c.ms.x, c.ms.y};
+}
+
 // One block of up to B frequencies: FP64 refinement around the inner solver.
 template <typename T, int B>
 static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
@@ -1894,6 +1966,15 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   ELP_HIP(hipMemcpyAsync(S.coefA.p, cA.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
   ELP_HIP(hipMemcpyAsync(S.coefP.p, cP.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
   if (S.st.fdm) ELP_HIP(hipMemcpyAsync(S.coefF.p, cF.data(), sizeof(FdmCoef) * B, hipMemcpyHostToDevice, s));
+  if (S.st.coupled) {
+    // the structure block factored at the block's middle frequency (the
+    // frequency itself for one-frequency blocks)
+    std::vector<double> fs(freqs, freqs + nb);
+    std::sort(fs.begin(), fs.end());
+    std::complex<double> ks, ms;
+    plate_coefs(m, fs[nb / 2], ks, ms);
+    S.refactor(fs[nb / 2], ks, ms);
+  }
   // FP64 outer vectors
   const int64_t nt = m.n * B;
   DArr<double2> F, b64, x64, r64;
@@ -1916,8 +1997,6 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   std::vector<double> nref(nb2), nrep(nb2);
   Lift lift;
   build_lift(m, lift);
-  double gnorm2 = 0;  // ||g||^2 of the prescribed values
-  for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
   if (lift.on) {
     std::vector<int> all(B, 1);
     DArr<int> d_all;
@@ -2161,9 +2240,31 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   for (int64_t k = 0; k < nf; k++) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return freqs[a] > freqs[b]; });
 
+  // Coupled models: the structure block is factored at one frequency for a
+  // window of 2 half frequencies (the sweep's range over its number of
+  // slot-sized groups); a slot taking a frequency outside the window makes
+  // the next factorization, at the top of the next window (frequencies are
+  // taken in descending order).
+  const bool cpl = S.st.coupled;
+  double half = 0;
+  if (cpl) {
+    const double fmax = freqs[order[0]], fmin = freqs[order[nf - 1]];
+    const int64_t nwin = std::max<int64_t>(1, (nf + B - 1) / B);
+    half = (fmax - fmin) / (2.0 * (double)nwin);
+  }
+  bool need_refactor = false;
+  double top_new = 0;  // highest frequency assigned since the last factorization
+  auto refactor_at = [&](double ftop) {
+    const double fc = std::max(freqs[order[nf - 1]], ftop - half);
+    std::complex<double> ks, ms;
+    plate_coefs(m, fc, ks, ms);
+    S.refactor(fc, ks, ms);
+  };
   // inner tolerance for a slot at FP64 relative residual rel: inner_rtol, or
   // just past the target when that is within reach of one FP32 solve
   auto inner_tol = [&](double rel) {
+    // FP64 inner solves aim straight below the target
+    if (std::is_same<T, double>::value) return std::max(0.3 * o.rtol / rel, 1e-15);
     const double finish = 0.5 * o.rtol / rel;
     return finish >= 1e-4 ? std::min(o.inner_rtol, finish) : o.inner_rtol;
   };
@@ -2176,6 +2277,11 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     }
     const int64_t fi = order[next++];
     set_coefs(k, fi);
+    if (cpl && S.fac_hz >= 0 && std::fabs(freqs[fi] - S.fac_hz) > half * (1 + 1e-9) + 1e-12) {
+      if (!need_refactor) top_new = freqs[fi];
+      need_refactor = true;
+      top_new = std::max(top_new, freqs[fi]);
+    }
     sl[k] = Slot();
     sl[k].f = fi;
     sl[k].state = 1;
@@ -2237,6 +2343,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   };
   for (int k = 0; k < B; k++) assign(k);
   upload_coefs();
+  if (cpl) refactor_at(freqs[order[0]]);
   start_columns();
 
   const int freeze_batch = std::max(1, B / 3);
@@ -2296,6 +2403,8 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
 
     // FP64 refinement of the frozen columns
     refinements++;
+    std::vector<int> was_iter(B, 0);
+    for (int k = 0; k < B; k++) was_iter[k] = sl[k].state == 1;
     for (int k = 0; k < B; k++) wgt[k] = (sl[k].state == 2 && !sl[k].broken) ? sl[k].w : 0.0;
     ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
@@ -2365,6 +2474,36 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
         scale[k] = 1.0 / q.w;
       }
     }
+    if (need_refactor) {
+      // slots in the middle of an inner solve keep their progress (x64 +=
+      // w d) and restart on the new residual under the new factorization
+      bool anyf = false;
+      for (int k = 0; k < B; k++) {
+        wgt[k] = was_iter[k] ? sl[k].w : 0.0;
+        anyf |= was_iter[k] != 0;
+      }
+      if (anyf) {
+        ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p,
+                           x64.p, iperm);
+        ELP_HIP(hipGetLastError());
+        S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+        S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+        S.fetch(S.red_out.p);
+        for (int k = 0; k < B; k++) {
+          if (!was_iter[k]) continue;
+          Slot &q = sl[k];
+          const double rl = std::sqrt(S.h_red[k].x) / q.ref;
+          q.prev = rl;
+          q.tol = inner_tol(rl);
+          q.w = rl * q.ref;
+          act[k] = 2;
+          scale[k] = q.w > 0 ? 1.0 / q.w : 0.0;
+        }
+      }
+      refactor_at(top_new);
+      need_refactor = false;
+    }
     // the stream is idle here only after a sync: the host mirrors are reused
     ELP_HIP(hipStreamSynchronize(s));
     if (any_new) upload_coefs();
@@ -2379,7 +2518,7 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
                       hipStream_t s) {
   Solver<T, B> S(m, st, o, s);
   int refinements = 0;
-  if (std::is_same<T, float>::value) {
+  if (std::is_same<T, float>::value || B > 1) {
     solve_slots<T, B>(S, nf, freqs, Z, X, relres, iters, refinements);
   } else {
     for (int64_t f0 = 0; f0 < nf; f0 += B) {
@@ -2396,6 +2535,8 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
     stats->spmv_bytes = m.nnz * (int64_t)(4 + 2 * sizeof(T)) + (m.n + 1) * 4 * 2 + m.n +
                         nx * (int64_t)(4 + 1 + sizeof(T)) + 2 * m.n * B * (int64_t)sizeof(V);
     stats->refinements = refinements;
+    stats->refactors = S.refactors;
+    stats->refactor_ms = S.refactor_ms;
   }
 }
 
@@ -2422,6 +2563,19 @@ int64_t sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z,
   const int B = o.precision == ELP_PREC_FP64 ? 1 : o.block;
   if (o.precision == ELP_PREC_FP64) {
     run_sweep<double, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s);
+  } else if (st.coupled) {
+    // Coupled models batch their frequencies like the mixed solver but keep
+    // the inner Krylov in FP64: a thin plate's stiffness has cond ~1e8, so
+    // the FP32 operator's own rounding (~6e-8 relative) leaves FP32 inner
+    // solves that no FP64 refinement step can improve on
+    switch (B) {
+      case 1: run_sweep<double, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 4: run_sweep<double, 4>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 8: run_sweep<double, 8>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 16: run_sweep<double, 16>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      case 32: run_sweep<double, 32>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
+      default: throw Error(ELP_EINVAL, "block must be 1, 4, 8, 16 or 32");
+    }
   } else {
     switch (B) {
       case 1: run_sweep<float, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;

@@ -74,7 +74,7 @@ class SolveOpts(C.Structure):
 class SweepStats(C.Structure):
     _fields_ = [("setup_ms", C.c_double), ("solve_ms", C.c_double), ("spmv_calls", C.c_int64),
                 ("spmv_ms", C.c_double), ("spmv_bytes", C.c_int64), ("refinements", C.c_int),
-                ("unconverged", C.c_int)]
+                ("unconverged", C.c_int), ("refactors", C.c_int), ("refactor_ms", C.c_double)]
 
 
 def library_path():

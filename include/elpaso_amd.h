@@ -106,6 +106,8 @@ typedef struct elp_sweep_stats {
   int64_t spmv_bytes;  /* algorithmic bytes of one fine-level SpMV launch */
   int refinements;     /* FP64 refinement steps (mixed)                   */
   int unconverged;     /* frequencies above rtol (then ELP_ENOCONV)       */
+  int refactors;       /* coupled: factorizations of the structure block  */
+  double refactor_ms;  /* coupled: time spent in them                     */
 } elp_sweep_stats;
 
 /* ---- context -------------------------------------------------------------- */
@@ -144,7 +146,10 @@ int elp_model_set_rayleigh(elp_model *m, double alpha, double beta);
 /* Optional: the model is a structured box (origin, edge lengths, elements per
  * direction, order 1 = Hex8 / 2 = Hex20, Robin faces on the box sides in
  * robin_sides: bit k = 0-based Hex8 face numbering -y,+y,-z,+x,+z,-x).
- * Enables the multigrid preconditioner. */
+ * Enables the multigrid preconditioner.  For a model with structure rows the
+ * box describes its Hex8 fluid cavity (order 1): the sweep then uses the
+ * block-diagonal coupled preconditioner (exact box solve of the fluid rows,
+ * direct block cyclic reduction of the structure block). */
 int elp_model_set_box(elp_model *m, const double *origin, const double *length,
                       const int32_t *nel, int order, int robin_sides);
 /* DOF numbering, symbolic phase (sparsity pattern, adjacency), upload.

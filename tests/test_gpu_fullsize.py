@@ -210,3 +210,25 @@ def test_config1_brick_stresses_full(ctx):
     assert np.abs(cells[0] - sig[None, :]).max() <= 1e-9 * scale
     assert np.abs(nodes[0, :, :6] - sig[None, :]).max() <= 1e-9 * scale
     assert not nodes[0, :, 6:].any()
+
+
+def test_config3_coupled_sweep_full(ctx):
+    """The coupled plate/cavity at full size (1,326,130 DOF): 32 consecutive
+    frequencies of the 512-frequency sweep (20-1000 Hz) at the top of the
+    range, on 32 slots with the coupled preconditioner; the FP64 residual of
+    elPaSo's system is recomputed on the host from elp_dynamic_matrix for
+    three of them."""
+    m = MG.coupled_plate_cavity()
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = MG.linear_frequencies(20, 1000, 512)[480:]
+    X, rel, it, st = g.sweep(freqs, block=32)
+    print(f"[fullsize] coupled sweep relres max {rel.max():.2e} iters {it.min()}-{it.max()} "
+          f"refactors {st.refactors}", flush=True)
+    assert (rel <= 1e-8).all() and st.unconverged == 0
+    rp, ci = g.pattern()
+    for k in (0, 17, 31):
+        a, F = g.dynamic_matrix(2 * np.pi * freqs[k])
+        A = sp.csr_matrix((a, ci, rp), shape=(g.n, g.n))
+        r = np.linalg.norm(F - A @ X[k]) / np.linalg.norm(F)
+        assert r <= 1.5e-8, (freqs[k], r, rel[k])

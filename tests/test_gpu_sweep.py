@@ -237,3 +237,21 @@ def test_unconverged_sweep_reports_enoconv(ctx):
     assert st.unconverged == 2 and (rel > 1e-8).all() and (rel <= 1.0).all()
     assert np.abs(X).max() > 0
     assert b"did not reach" in L.elp_last_error(None)
+
+
+@pytest.mark.parametrize("block", [4, 32])
+def test_coupled_sweep_block_preconditioner(ctx, block):
+    """The coupled plate/cavity with its default preconditioner: the exact
+    box solve on the cavity's fluid rows and the direct (block cyclic
+    reduction) solve of the clamped plate, factored per window of
+    frequencies; FP64 residuals of elPaSo's system at the bar and the
+    forward-error bound against SuperLU."""
+    m = MG.coupled_plate_cavity(n_xy=10, n_plate_z=2, n_cav_z=12)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = MG.linear_frequencies(20, 1000, 40)[::3]
+    print(f"[sweep] coupled block preconditioner, {len(freqs)} freqs on {block} slots", flush=True)
+    X, rel, it, st = g.sweep(freqs, block=block)
+    assert (rel <= 1e-8).all() and st.unconverged == 0, (rel, it)
+    assert st.refactors >= 1
+    _check_forward_error(m, freqs, X)
