@@ -39,10 +39,18 @@ struct BcrPlan {
   int64_t ns = 0;  // free structure rows
   int m = 0, N = 0, axis = 0;
   DArr<int32_t> rows_model, rows_solver, pos;
-  rocblas_handle h = nullptr;
+  // separate handles for solves (h) and factorizations (hf): a factorization
+  // may run on another thread and stream while solves go on
+  rocblas_handle h = nullptr, hf = nullptr;
   DArr<unsigned char> X, Tw;  // solve work: N blocks and ceil(N/2) blocks of m x B
+  // factorization work (one factorization at a time), kept between
+  // factorizations so that none frees device memory (hipFree waits for the
+  // whole device): D and C of a level, ping-pong, pivots
+  DArr<unsigned char> fD[2], fC[2];
+  DArr<int> ipiv, info, bad;
   ~BcrPlan() {
     if (h) (void)rocblas_destroy_handle(h);
+    if (hf) (void)rocblas_destroy_handle(hf);
   }
 };
 
@@ -150,6 +158,7 @@ std::unique_ptr<BcrPlan, void (*)(BcrPlan *)> bcr_plan(const Model &m, const std
   P->rows_solver.upload(m.ctx, rs);
   P->pos.upload(m.ctx, best_pos);
   ELP_RB(rocblas_create_handle(&P->h));
+  ELP_RB(rocblas_create_handle(&P->hf));
   return P;
 }
 
@@ -257,6 +266,14 @@ static void gemm_sb(rocblas_handle h, bool ta, bool tb, int M, int N, int K, dou
                                        (const rocblas_double_complex *)B, ldb, sb, &be, (rocblas_double_complex *)C,
                                        ldc, sc, batch));
 }
+static void ensure_bytes(DArr<unsigned char> &a, Ctx *c, int64_t bytes) {
+  if ((int64_t)a.n < bytes) a.alloc(c, bytes);
+}
+template <typename V>
+static void ensure(DArr<V> &a, Ctx *c, int64_t count) {
+  if ((int64_t)a.n < count) a.alloc(c, count);
+}
+
 static void inverse_sb(rocblas_handle h, int m, float2 *A, int batch, rocblas_int *ipiv, rocblas_int *info) {
   const int64_t m2 = (int64_t)m * m;
   ELP_RB(rocsolver_cgetrf_strided_batched(h, m, m, (rocblas_float_complex *)A, m, m2, ipiv, m, info, batch));
@@ -286,68 +303,69 @@ void bcr_factor(const Model &m, BcrPlan &p, std::complex<double> ks, std::comple
   using V = typename CT<T>::v;
   const int mb = p.m;
   const int64_t m2 = (int64_t)mb * mb;
-  ELP_RB(rocblas_set_stream(p.h, s));
-  f.lv.clear();
-  DArr<V> D, C;
-  D.alloc(m.ctx, (size_t)p.N * m2);
-  C.alloc(m.ctx, (size_t)p.N * m2);
-  D.zero(s);
-  C.zero(s);
-  DArr<int> bad;
-  bad.alloc(m.ctx, 1);
-  bad.zero(s);
+  Ctx *cx = m.ctx;
+  ELP_RB(rocblas_set_stream(p.hf, s));
+  for (int k = 0; k < 2; k++) {
+    ensure_bytes(p.fD[k], cx, (int64_t)p.N * m2 * (int64_t)sizeof(V));
+    ensure_bytes(p.fC[k], cx, (int64_t)p.N * m2 * (int64_t)sizeof(V));
+  }
+  const int E0 = (p.N + 1) / 2;
+  ensure(p.ipiv, cx, (int64_t)E0 * mb);
+  ensure(p.info, cx, E0);
+  ensure(p.bad, cx, 1);
+  V *D = reinterpret_cast<V *>(p.fD[0].p), *C = reinterpret_cast<V *>(p.fC[0].p);
+  V *Dn = reinterpret_cast<V *>(p.fD[1].p), *Cn = reinterpret_cast<V *>(p.fC[1].p);
+  ELP_HIP(hipMemsetAsync(D, 0, sizeof(V) * p.N * m2, s));
+  ELP_HIP(hipMemsetAsync(C, 0, sizeof(V) * p.N * m2, s));
+  ELP_HIP(hipMemsetAsync(p.bad.p, 0, sizeof(int), s));
   const int64_t total = (int64_t)p.N * mb;
   hipLaunchKernelGGL(bcr_extract_kernel<V>, dim3(nb256(total)), dim3(256), 0, s, total, mb, p.rows_model.p, p.pos.p,
                      m.d_rowptr.p, m.d_colind.p, m.d_K.p, m.d_M.p, double2{ks.real(), ks.imag()},
-                     double2{ms.real(), ms.imag()}, D.p, C.p, bad.p);
+                     double2{ms.real(), ms.imag()}, D, C, p.bad.p);
   ELP_HIP(hipGetLastError());
   int hbad = 0;
-  ELP_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  ELP_HIP(hipMemcpyAsync(&hbad, p.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   ELP_HIP(hipStreamSynchronize(s));
   ELP_REQUIRE(!hbad, ELP_EINVAL, "structure block is not block tridiagonal in its ordering");
-  DArr<rocblas_int> ipiv, info;
-  const int E0 = (p.N + 1) / 2;
-  ipiv.alloc(m.ctx, (size_t)E0 * mb);
-  info.alloc(m.ctx, E0);
+  // the level structure depends on N only: a refactorization reuses the
+  // level arrays
   int N = p.N;
+  size_t l = 0;
   while (true) {
-    typename BcrFactor<T>::Level L;
+    if (f.lv.size() <= l) f.lv.emplace_back();
+    typename BcrFactor<T>::Level &L = f.lv[l];
     L.N = N;
     L.E = (N + 1) / 2;
     L.K = N / 2;
     L.K1 = (N % 2) ? L.K : L.K - 1;
+    const int K = L.K, K1 = L.K1;
     // inverses of the eliminated (even-position) blocks
-    L.Dinv.alloc(m.ctx, (size_t)L.E * m2);
-    copy_blocks<V>(L.E, m2, D.p, 2 * m2, L.Dinv.p, m2, s);
-    inverse_sb(p.h, mb, L.Dinv.p, L.E, ipiv.p, info.p);
-    if (L.K == 0) {
-      f.lv.push_back(std::move(L));
+    ensure(L.Dinv, cx, (int64_t)L.E * m2);
+    copy_blocks<V>(L.E, m2, D, 2 * m2, L.Dinv.p, m2, s);
+    inverse_sb(p.hf, mb, L.Dinv.p, L.E, p.ipiv.p, p.info.p);
+    if (K == 0) {
+      f.lv.resize(l + 1);
       break;
     }
-    const int K = L.K, K1 = L.K1;
-    L.W.alloc(m.ctx, (size_t)K * m2);
-    L.Vv.alloc(m.ctx, (size_t)K * m2);
+    ensure(L.W, cx, (int64_t)K * m2);
+    ensure(L.Vv, cx, (int64_t)K * m2);
     // W_k = C_k D_{k-1}^-1 ; V_k = C_{k+1}^T D_{k+1}^-1
-    gemm_sb(p.h, false, false, mb, mb, mb, (T)1, C.p + m2, 2 * m2, L.Dinv.p, mb, m2, (T)0, L.W.p, mb, m2, K);
-    gemm_sb(p.h, true, false, mb, mb, mb, (T)1, C.p + 2 * m2, 2 * m2, L.Dinv.p + m2, mb, m2, (T)0, L.Vv.p, mb, m2,
+    gemm_sb(p.hf, false, false, mb, mb, mb, (T)1, C + m2, 2 * m2, L.Dinv.p, mb, m2, (T)0, L.W.p, mb, m2, K);
+    gemm_sb(p.hf, true, false, mb, mb, mb, (T)1, C + 2 * m2, 2 * m2, L.Dinv.p + m2, mb, m2, (T)0, L.Vv.p, mb, m2,
             K1);
     if (K1 < K) ELP_HIP(hipMemsetAsync(L.Vv.p + (size_t)(K - 1) * m2, 0, sizeof(V) * m2, s));
     // D'_k = D_k - W_k C_k^T - V_k C_{k+1}
-    DArr<V> Dn, Cn;
-    Dn.alloc(m.ctx, (size_t)K * m2);
-    copy_blocks<V>(K, m2, D.p + m2, 2 * m2, Dn.p, m2, s);
-    gemm_sb(p.h, false, true, mb, mb, mb, (T)-1, L.W.p, m2, C.p + m2, mb, 2 * m2, (T)1, Dn.p, mb, m2, K);
-    gemm_sb(p.h, false, false, mb, mb, mb, (T)-1, L.Vv.p, m2, C.p + 2 * m2, mb, 2 * m2, (T)1, Dn.p, mb, m2, K1);
+    copy_blocks<V>(K, m2, D + m2, 2 * m2, Dn, m2, s);
+    gemm_sb(p.hf, false, true, mb, mb, mb, (T)-1, L.W.p, m2, C + m2, mb, 2 * m2, (T)1, Dn, mb, m2, K);
+    gemm_sb(p.hf, false, false, mb, mb, mb, (T)-1, L.Vv.p, m2, C + 2 * m2, mb, 2 * m2, (T)1, Dn, mb, m2, K1);
     // C'_k = -W_k C_{k-1} (coupling of kept k to the kept block before it)
-    Cn.alloc(m.ctx, (size_t)K * m2);
-    ELP_HIP(hipMemsetAsync(Cn.p, 0, sizeof(V) * m2, s));
-    gemm_sb(p.h, false, false, mb, mb, mb, (T)-1, L.W.p + m2, m2, C.p + 2 * m2, mb, 2 * m2, (T)0, Cn.p + m2, mb, m2,
+    ELP_HIP(hipMemsetAsync(Cn, 0, sizeof(V) * m2, s));
+    gemm_sb(p.hf, false, false, mb, mb, mb, (T)-1, L.W.p + m2, m2, C + 2 * m2, mb, 2 * m2, (T)0, Cn + m2, mb, m2,
             K - 1);
-    f.lv.push_back(std::move(L));
-    ELP_HIP(hipStreamSynchronize(s));  // D, C of this level are released next
-    D = std::move(Dn);
-    C = std::move(Cn);
+    std::swap(D, Dn);
+    std::swap(C, Cn);
     N = K;
+    l++;
   }
   ELP_HIP(hipStreamSynchronize(s));
 }

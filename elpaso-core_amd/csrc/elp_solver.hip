@@ -1566,6 +1566,13 @@ struct Solver {
   double fac_hz = -1;
   int refactors = 0;
   double refactor_ms = 0;
+  // the next window's factorization, prepared on a thread and stream of its
+  // own while the current window iterates
+  std::unique_ptr<BcrFactor<double>, void (*)(BcrFactor<double> *)> next{nullptr, nullptr};
+  double next_hz = -1;
+  std::thread pre;
+  std::exception_ptr pre_err;
+  hipStream_t s2 = nullptr;
   PinnedArr<double2> h_red;     // per-frequency reductions, pinned host memory
   // statistics
   HipEvent ev0, ev1;
@@ -1752,12 +1759,49 @@ struct Solver {
 
   // factor the structure block at f_hz: A_ss = ks K + ms M
   void refactor(double f_hz, std::complex<double> ks, std::complex<double> ms) {
+    join_prefetch();
     const auto t0 = std::chrono::steady_clock::now();
     if (!fac) fac = bcr_new_factor<double>();
     bcr_factor<double>(m, *st.bcr, ks, ms, *fac, s);
     fac_hz = f_hz;
     refactors++;
     refactor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  // start factoring at f_hz in the background (one at a time)
+  void prefetch(double f_hz, std::complex<double> ks, std::complex<double> ms) {
+    join_prefetch();
+    if (!next) next = bcr_new_factor<double>();
+    if (!s2) ELP_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    next_hz = f_hz;
+    pre_err = nullptr;
+    const int dev = ctx->device;
+    pre = std::thread([this, ks, ms, dev]() {
+      try {
+        ELP_HIP(hipSetDevice(dev));
+        bcr_factor<double>(m, *st.bcr, ks, ms, *next, s2);
+      } catch (...) {
+        pre_err = std::current_exception();
+      }
+    });
+  }
+  void join_prefetch() {
+    if (pre.joinable()) pre.join();
+  }
+  // use the prefetched factorization if it was made at f_hz
+  bool take_prefetch(double f_hz) {
+    if (!pre.joinable() || std::fabs(next_hz - f_hz) > 1e-9 * std::max(1.0, f_hz)) return false;
+    const auto t0 = std::chrono::steady_clock::now();
+    pre.join();
+    refactor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (pre_err) std::rethrow_exception(pre_err);
+    std::swap(fac, next);
+    fac_hz = next_hz;
+    refactors++;
+    return true;
+  }
+  ~Solver() {
+    join_prefetch();
+    if (s2) (void)hipStreamDestroy(s2);
   }
 
   void precond(const V *r, V *z) {
@@ -2241,24 +2285,34 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return freqs[a] > freqs[b]; });
 
   // Coupled models: the structure block is factored at one frequency for a
-  // window of 2 half frequencies (the sweep's range over its number of
-  // slot-sized groups); a slot taking a frequency outside the window makes
-  // the next factorization, at the top of the next window (frequencies are
-  // taken in descending order).
+  // window of frequencies: the sweep's range cut into as many equal windows
+  // as it has slot-sized groups of frequencies, each factored at its centre.
+  // Frequencies are taken in descending order, so the windows are used one
+  // after the other and the next one is factored in the background.
   const bool cpl = S.st.coupled;
-  double half = 0;
-  if (cpl) {
-    const double fmax = freqs[order[0]], fmin = freqs[order[nf - 1]];
-    const int64_t nwin = std::max<int64_t>(1, (nf + B - 1) / B);
-    half = (fmax - fmin) / (2.0 * (double)nwin);
-  }
+  const double ftop = freqs[order[0]], fbot = freqs[order[nf - 1]];
+  const int64_t nwin = cpl ? std::max<int64_t>(1, (nf + B - 1) / B) : 1;
+  const double wwidth = nwin > 1 ? (ftop - fbot) / (double)nwin : 0.0;
+  auto window_of = [&](double f) -> int64_t {
+    if (!(wwidth > 0)) return 0;
+    return std::min<int64_t>(nwin - 1, std::max<int64_t>(0, (int64_t)std::floor((ftop - f) / wwidth)));
+  };
+  auto centre_of = [&](int64_t w) { return nwin > 1 ? ftop - wwidth * (w + 0.5) : 0.5 * (ftop + fbot); };
+  int64_t cur_win = -1, want_win = -1;
   bool need_refactor = false;
-  double top_new = 0;  // highest frequency assigned since the last factorization
-  auto refactor_at = [&](double ftop) {
-    const double fc = std::max(freqs[order[nf - 1]], ftop - half);
+  auto use_window = [&](int64_t w) {
     std::complex<double> ks, ms;
-    plate_coefs(m, fc, ks, ms);
-    S.refactor(fc, ks, ms);
+    const double fc = centre_of(w);
+    if (!S.take_prefetch(fc)) {
+      plate_coefs(m, fc, ks, ms);
+      S.refactor(fc, ks, ms);
+    }
+    cur_win = w;
+    if (w + 1 < nwin) {
+      const double fn = centre_of(w + 1);
+      plate_coefs(m, fn, ks, ms);
+      S.prefetch(fn, ks, ms);
+    }
   };
   // inner tolerance for a slot at FP64 relative residual rel: inner_rtol, or
   // just past the target when that is within reach of one FP32 solve
@@ -2277,10 +2331,10 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     }
     const int64_t fi = order[next++];
     set_coefs(k, fi);
-    if (cpl && S.fac_hz >= 0 && std::fabs(freqs[fi] - S.fac_hz) > half * (1 + 1e-9) + 1e-12) {
-      if (!need_refactor) top_new = freqs[fi];
+    if (cpl && cur_win >= 0 && window_of(freqs[fi]) != cur_win) {
+      // the slots move on to the next window
+      want_win = need_refactor ? std::min(want_win, window_of(freqs[fi])) : window_of(freqs[fi]);
       need_refactor = true;
-      top_new = std::max(top_new, freqs[fi]);
     }
     sl[k] = Slot();
     sl[k].f = fi;
@@ -2343,7 +2397,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   };
   for (int k = 0; k < B; k++) assign(k);
   upload_coefs();
-  if (cpl) refactor_at(freqs[order[0]]);
+  if (cpl) use_window(0);
   start_columns();
 
   const int freeze_batch = std::max(1, B / 3);
@@ -2501,7 +2555,7 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
           scale[k] = q.w > 0 ? 1.0 / q.w : 0.0;
         }
       }
-      refactor_at(top_new);
+      use_window(want_win);
       need_refactor = false;
     }
     // the stream is idle here only after a sync: the host mirrors are reused
