@@ -48,13 +48,15 @@ def main():
     ap.add_argument("--nel", type=str, default="100,80,60", help="Hex20 elements per direction")
     ap.add_argument("--nfreq", type=int, default=256)
     ap.add_argument("--block", type=int, default=32)
-    ap.add_argument("--fp64-sample", type=int, default=4, help="frequencies timed with the FP64 solver")
+    ap.add_argument("--fp64-sample", type=int, default=32,
+                    help="frequencies timed with the FP64 solver (the error against FP64 covers all of them)")
+    ap.add_argument("--no-config3", action="store_true", help="skip the coupled plate/cavity sweep")
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-peaks", action="store_true", help="skip re-measuring copy bandwidth and FP64 GEMM")
     ap.add_argument("--no-config0", action="store_true",
                     help="skip the 20x16x12 Fluid8 cavity leg (GPU sweep vs the CPU oracle's direct solves)")
-    ap.add_argument("--cpu-procs", type=int, default=16,
+    ap.add_argument("--cpu-procs", type=int, default=0,
                     help="processes of the CPU oracle's frequency-parallel run (capped by the host's cores)")
     ap.add_argument("--assembly-nel", type=str, default="200,200,4")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving stream (0: skip)")
@@ -120,21 +122,27 @@ def main():
     # result does not depend on the others in the batch)
     fp64 = None
     if args.fp64_sample > 0:
-        idx = np.linspace(0, len(freqs) - 1, args.fp64_sample).round().astype(int)
+        # timed: the plain per-frequency FP64 solve on an evenly spaced sample
+        idx = np.linspace(0, len(freqs) - 1, min(args.fp64_sample, len(freqs))).round().astype(int)
         o64 = _lib.default_opts(precision=_lib.PREC_FP64)
         t = time.perf_counter()
-        _, rel64, it64, _ = g.sweep(freqs[idx], opts=o64, want_x=False)
+        _, rel64s, it64s, _ = g.sweep(freqs[idx], opts=o64, want_x=False)
         dt64 = time.perf_counter() - t
-        X64, _, _, _ = g.sweep(freqs[idx], opts=o64, want_x=True)
-        Xm, relm, itm, _ = g.sweep(freqs[idx], opts=opts, want_x=True)
+        # untimed: every frequency of the sweep solved both ways, the error of
+        # the mixed solution against the FP64 one per frequency
+        X64, rel64, it64, _ = g.sweep(freqs, opts=o64, want_x=True)
+        Xm, relm, itm, _ = g.sweep(freqs, opts=opts, want_x=True)
         err = np.linalg.norm(Xm - X64, axis=1) / np.linalg.norm(X64, axis=1)
         del Xm, X64
-        fp64 = {"sample_freqs_hz": [round(float(f), 3) for f in freqs[idx]],
-                "fp64_solves_per_s": len(idx) / dt64, "fp64_relres_max": float(rel64.max()),
-                "fp64_iters": it64.tolist(), "mixed_iters": itm.tolist(),
-                "err_mixed_vs_fp64": [float(e) for e in err],
-                "speedup_mixed_batched_vs_fp64": value / (len(idx) / dt64)}
-        log(f"[bench] fp64 sample: {fp64}")
+        fp64 = {"timed_sample_freqs": len(idx), "fp64_solves_per_s": len(idx) / dt64,
+                "fp64_relres_max": float(max(rel64.max(), rel64s.max())),
+                "fp64_iters_mean": float(it64.mean()), "fp64_iters_max": int(it64.max()),
+                "speedup_mixed_batched_vs_fp64": value / (len(idx) / dt64),
+                "err_mixed_vs_fp64": {"freqs": len(freqs), "max": float(err.max()),
+                                      "median": float(np.median(err)),
+                                      "per_freq": [float("%.3g" % e) for e in err]}}
+        log(f"[bench] fp64: {len(idx) / dt64:.2f} solves/s, error vs fp64 max {err.max():.2e} "
+            f"median {np.median(err):.2e} over {len(freqs)} frequencies")
 
     # element and node results (pressure gradients at the nodes) of a sample
     # of the sweep's own solutions
@@ -238,6 +246,11 @@ def main():
         config0 = config0_leg(ctx, MG)
         log(f"[bench] config 0: {config0}")
 
+    config3 = None
+    if not args.no_config3:
+        config3 = config3_leg(ctx, MG)
+        log(f"[bench] config 3: {config3}")
+
     cpu = None
     if not args.no_cpu:
         steel = MG.steel_block(*(int(v) for v in args.assembly_nel.split(","))) if assembly else None
@@ -270,7 +283,7 @@ def main():
         "sweep_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "fp64_baseline": fp64, "roofline": roof, "assembly": assembly, "serving": serving,
         "node_results": node_res,
-        "cpu_baseline": cpu, "config0": config0, "measured_peaks": peaks,
+        "cpu_baseline": cpu, "config0": config0, "config3": config3, "measured_peaks": peaks,
     }
     print(json.dumps(line), flush=True)
 
@@ -442,6 +455,53 @@ def config0_leg(ctx, MG):
             "gpu_relres_max": float(rel.max()), "gpu_iters_max": int(it.max())}
 
 
+def config3_leg(ctx, MG, sample=8):
+    """The coupled vibroacoustic case: steel plate 100x100x2 Brick8 (clamped
+    edges, loss factor 0.01) on an air cavity 100x100x120 Fluid8 through
+    10,000 interface Quad4, 1,326,130 DOF, unit normal force at a seeded plate
+    node, 512 frequencies 20-1000 Hz on 32 slots.  The inner Krylov solve is
+    FP64 here (the plate's stiffness is too ill-conditioned for FP32 inner
+    solves); the preconditioner is the exact box solve of the cavity and the
+    block cyclic reduction of the plate, one factorization per window of 32
+    frequencies prepared in the background.  The first sweep keeps its
+    solutions (and warms up); the second is timed with the solutions left on
+    the device.  Beside it: the plain per-frequency FP64 solve (the plate
+    factored at each frequency) on an evenly spaced sample, and the error of
+    the sweep's solutions against it."""
+    from elpaso_amd import GpuModel
+    from elpaso_amd import _lib
+    m = MG.coupled_plate_cavity()
+    freqs = MG.linear_frequencies(20.0, 1000.0, 512)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    opts = _lib.default_opts(block=32)
+    X, rel0, _, _ = g.sweep(freqs, opts=opts, want_x=True)
+    t = time.perf_counter()
+    _, rel, it, st = g.sweep(freqs, opts=opts, want_x=False)
+    dt = time.perf_counter() - t
+    idx = np.linspace(0, len(freqs) - 1, sample).round().astype(int)
+    o64 = _lib.default_opts(precision=_lib.PREC_FP64)
+    t = time.perf_counter()
+    X64, rel64, it64, st64 = g.sweep(freqs[idx], opts=o64, want_x=True)
+    dt64 = time.perf_counter() - t
+    err = np.linalg.norm(X[idx] - X64, axis=1) / np.linalg.norm(X64, axis=1)
+    del X, X64
+    g.close()
+    return {"workload": "coupled plate 1.0x1.0x0.005 m Brick8 100x100x2 (clamped, eta 0.01) + air cavity "
+                        "1.0x1.0x1.2 m Fluid8 100x100x120, 10,000 interface Quad4 (1,326,130 DOF), "
+                        "512 freqs 20-1000 Hz on 32 slots",
+            "solves_per_s": round(len(freqs) / dt, 2), "sweep_s": round(dt, 3),
+            "converged": f"{int((rel <= 1e-8).sum())}/{len(rel)}", "relres_max": float(max(rel.max(), rel0.max())),
+            "iters_mean": float(it.mean()), "iters_max": int(it.max()),
+            "precision": "complex-f64 Krylov (batched on 32 slots) + f64 refinement",
+            "precond": "box solve (cavity) + block cyclic reduction (plate), 16 windows",
+            "structure_factorizations": st.refactors, "factorization_wait_ms": round(st.refactor_ms, 1),
+            "fp64_per_frequency": {"sample_freqs": len(idx), "solves_per_s": round(len(idx) / dt64, 3),
+                                   "iters": it64.tolist(), "relres_max": float(rel64.max())},
+            "speedup_batched_vs_per_frequency_fp64": round((len(freqs) / dt) / (len(idx) / dt64), 2),
+            "err_vs_fp64_per_frequency": [float("%.3g" % e) for e in err]}
+
+
 def _cpu_config0(procs):
     """CPU side of the reference case: the oracle's restatement of elPaSo's
     per-frequency loop (re-integration + scatter + BCs + SuperLU, standing in
@@ -464,7 +524,7 @@ def _cpu_config0(procs):
     out = {"cpu_1core_solves_per_s": round(10 / dt1, 3),
            "cpu_1core_ms_per_freq": {"assembly": round(1e3 * float(np.mean([a for a, _ in tm])), 1),
                                      "lu_solve": round(1e3 * float(np.mean([b for _, b in tm])), 1)}}
-    p = max(1, min(procs, os.cpu_count() or 1))
+    p = max(1, procs)
     import multiprocessing as mp
     shares = [freqs[i * len(freqs) // p:(i + 1) * len(freqs) // p] for i in range(p)]
     # one thread per process (inherited at spawn): BLAS threads of P processes
@@ -490,6 +550,101 @@ def _cpu_config0(procs):
     return out
 
 
+def host_info(requested):
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process
+    may use, the CPU model, and the processes the baseline uses (the box's
+    CPU share for one GPU, 16, unless asked otherwise)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    n = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else n
+    procs = requested if requested > 0 else min(16, aff)
+    return {"nproc": n, "usable_cpus": aff, "cpu_model": model, "processes": procs}
+
+
+def _oracle_assembly_worker(lo, hi, barrier, out):
+    """One process of the parallel oracle assembly: the Brick8 block's
+    symbolic phase (untimed), a barrier, then K + M + C of elements [lo, hi)
+    into the full pattern."""
+    sys.path.insert(0, os.path.join(ROOT, "elpaso-core_amd"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from elpaso_amd import meshgen as MG
+    m = MG.steel_block()
+    num = O.Numbering(m)
+    pat = O.pattern(m, num)
+    g = m.groups[0]
+    g.elem_ids, g.conn = g.elem_ids[lo:hi], g.conn[lo:hi]
+    barrier.wait()
+    t = time.perf_counter()
+    O.assemble_km(m, num, pat)
+    out.put(time.perf_counter() - t)
+
+
+def _cpu_parallel_assembly(procs):
+    """The oracle's K + M + C assembly of the 200x200x4 Brick8 block on P
+    processes, each on a contiguous range of elements (into its own copy of
+    the pattern; the final sum of the P partial matrices is not timed)."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    n_el = 200 * 200 * 4
+    barrier = ctx.Barrier(procs)
+    q = ctx.Queue()
+    ps = [ctx.Process(target=_oracle_assembly_worker,
+                      args=(i * n_el // procs, (i + 1) * n_el // procs, barrier, q)) for i in range(procs)]
+    for p in ps:
+        p.start()
+    times = [q.get() for _ in ps]
+    for p in ps:
+        p.join()
+    return {"processes": procs, "wall_s": round(max(times), 3),
+            "sample": "element integration + scatter of the whole block, P contiguous element ranges after a "
+                      "barrier (symbolic phase and the sum of the partial matrices untimed)"}
+
+
+def _cpu_lu_probe():
+    """The direct LU (SuperLU, COLAMD ordering, complex128, one thread) of the
+    Hex20 cavity's A(w) at 1 kHz on three small meshes: its time and fill
+    grow so fast with n that the 1.98M-DOF system is out of reach; the fitted
+    growth and the extrapolation to n = 1,976,881 are reported as such."""
+    sys.path.insert(0, os.path.join(ROOT, "elpaso-core_amd"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import scipy.sparse.linalg as sla
+    from elpaso_amd import meshgen as MG
+    from threadpoolctl import threadpool_limits
+    pts = []
+    for nel in ((10, 8, 6), (15, 12, 9), (20, 16, 12)):
+        m = MG.acoustic_cavity(*nel, quadratic=True)
+        num = O.Numbering(m)
+        pat = O.pattern(m, num)
+        rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * 1000.0, num, pat)
+        Am = O.csr(rp, ci, A, num.n).tocsc()
+        with threadpool_limits(limits=1):
+            t = time.perf_counter()
+            lu = sla.splu(Am)
+            dt = time.perf_counter() - t
+        pts.append({"mesh": "x".join(map(str, nel)), "n": num.n, "lu_s": round(dt, 3),
+                    "lu_entries": int(lu.L.nnz + lu.U.nnz)})
+    ln = np.log([p["n"] for p in pts])
+    et = float(np.polyfit(ln, np.log([p["lu_s"] for p in pts]), 1)[0])
+    ef = float(np.polyfit(ln, np.log([p["lu_entries"] for p in pts]), 1)[0])
+    ratio = 1976881 / pts[-1]["n"]
+    return {"points": pts, "time_exponent": round(et, 2), "fill_exponent": round(ef, 2),
+            "extrapolated_to_config2": {"n": 1976881, "lu_s": float("%.3g" % (pts[-1]["lu_s"] * ratio ** et)),
+                                        "lu_GB": float("%.3g" % (pts[-1]["lu_entries"] * ratio ** ef * 16 / 1e9)),
+                                        "note": "extrapolation of the three measured points (SuperLU with "
+                                                "COLAMD ordering); a nested-dissection ordering, as MUMPS "
+                                                "uses, fills less"}}
+
+
 def cpu_baseline(model, freqs, steel=None, config0=False, procs=16, post_model=None):
     """elPaSo's per-frequency flow restated on the CPU (oracle), one core:
     re-integrate every element at w, scatter into CSR (SetupKM).  The direct
@@ -508,8 +663,9 @@ def cpu_baseline(model, freqs, steel=None, config0=False, procs=16, post_model=N
     O.dynamic_matrix(model, 2 * np.pi * float(freqs[len(freqs) // 2]), num, pat)
     t_asm = time.perf_counter() - t
     log(f"[bench] cpu oracle: symbolic {t_sym:.1f}s, one-frequency assembly {t_asm:.1f}s")
+    host = host_info(procs)
     out = {"value": round(1.0 / t_asm, 5), "unit": "freq-point solves/s (upper bound: assembly only)",
-           "cores": 1, "kind": "port",
+           "cores": 1, "kind": "port", "host": host,
            "sample": f"1 frequency ({float(freqs[len(freqs) // 2]):.1f} Hz) of the same model: elPaSo's "
                      f"per-frequency element re-integration + CSR scatter + BCs ({t_asm:.1f} s); the "
                      f"direct LU of this system is beyond a bounded sample and is not included"}
@@ -523,8 +679,13 @@ def cpu_baseline(model, freqs, steel=None, config0=False, procs=16, post_model=N
         out["brick8_assembly_1core_s"] = {"assembly": round(time.perf_counter() - t, 2),
                                           "symbolic": round(t_ssym, 2)}
         log(f"[bench] cpu oracle Brick8 assembly: {out['brick8_assembly_1core_s']}")
+        out["brick8_assembly_parallel"] = _cpu_parallel_assembly(host["processes"])
+        log(f"[bench] cpu oracle Brick8 assembly on {host['processes']} processes: "
+            f"{out['brick8_assembly_parallel']}")
+    out["config2_lu_probe"] = _cpu_lu_probe()
+    log(f"[bench] cpu LU probe: {out['config2_lu_probe']}")
     if config0:
-        out["config0"] = _cpu_config0(procs)
+        out["config0"] = _cpu_config0(host["processes"])
         log(f"[bench] cpu oracle config 0: {out['config0']}")
     if post_model is not None:
         # element and node results of one solution (pressure gradients at the
