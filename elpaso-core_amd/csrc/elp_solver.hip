@@ -91,10 +91,14 @@ struct OpArgs {
 // XCD walks one contiguous range of rows: neighbouring rows share most of
 // their columns, so the gathers of x hit in that XCD's L2.
 constexpr int kXcds = 8;
-__device__ inline int xcd_block() {
-  const int per = gridDim.x / kXcds;
-  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+// block `local` of a launch (or of a segment of a grouped launch) of
+// `nblocks` blocks, a multiple of 8, remapped so that each XCD walks one
+// contiguous range
+__device__ inline int xcd_map(int local, int nblocks) {
+  const int per = nblocks / kXcds;
+  return (local % kXcds) * per + local / kXcds;
 }
+__device__ inline int xcd_block() { return xcd_map(blockIdx.x, gridDim.x); }
 static int xcd_grid(int64_t blocks) { return (int)((blocks + kXcds - 1) / kXcds * kXcds); }
 
 template <int B> struct OpShape {
@@ -177,10 +181,9 @@ __device__ inline typename VT<T>::t op_extras(const OpArgs<T> &a, const OpCoef &
 }
 
 template <typename T, int B, int MODE>
-__global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__restrict__ coef,
-                                                 const typename VT<T>::t *__restrict__ x,
-                                                 const typename VT<T>::t *__restrict__ b,
-                                                 typename VT<T>::t *__restrict__ y) {
+__device__ inline void op_body(const OpArgs<T> &a, const OpCoef *__restrict__ coef,
+                               const typename VT<T>::t *__restrict__ x, const typename VT<T>::t *__restrict__ b,
+                               typename VT<T>::t *__restrict__ y, int blk) {
   using V = typename VT<T>::t;
   using SH = OpShape<B>;
   constexpr int G = SH::G, S = SH::S, U = SH::U, CH = SH::CH;
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
   __shared__ T s_m[SH::RPB][CH];
   const int grp = threadIdx.x / G;
   const int lane = threadIdx.x % G;
-  const int64_t r = (int64_t)xcd_block() * SH::RPB + grp;
+  const int64_t r = (int64_t)blk * SH::RPB + grp;
   if (r >= a.n) return;  // the whole group leaves together
   const int f = lane % B, sub = lane / B;
   const uint8_t fld = a.field[r];
@@ -235,6 +238,14 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
   }
   if (sub != 0) return;
   op_finish<T, B, MODE>(a, cf, r, f, fld, accK, accM, accX, x, b, y);
+}
+
+template <typename T, int B, int MODE>
+__global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__restrict__ coef,
+                                                 const typename VT<T>::t *__restrict__ x,
+                                                 const typename VT<T>::t *__restrict__ b,
+                                                 typename VT<T>::t *__restrict__ y) {
+  op_body<T, B, MODE>(a, coef, x, b, y, xcd_block());
 }
 
 constexpr int kRowPad = 8;  // padded rows hold a multiple of this many entries
@@ -562,7 +573,7 @@ __device__ inline void red_term(double2 &acc, V av, V bv, int mode) {
 }
 
 // block tree over the R row-lanes of each frequency; lane rl == 0 writes
-__device__ inline void red_block_store(double2 acc, int B, double2 *__restrict__ partial) {
+__device__ inline void red_block_store(double2 acc, int B, double2 *__restrict__ partial, int blk) {
   __shared__ double2 sh[256];
   const int t = threadIdx.x, rl = t / B, R = 256 / B;
   sh[t] = acc;
@@ -574,15 +585,15 @@ __device__ inline void red_block_store(double2 acc, int B, double2 *__restrict__
     }
     __syncthreads();
   }
-  if (rl == 0) partial[blockIdx.x * B + t] = sh[t];
+  if (rl == 0) partial[blk * B + t] = sh[t];
 }
 
 template <typename V>
-__global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__restrict__ a, const V *__restrict__ b,
-                                                  int mode, double2 *__restrict__ partial) {
+__device__ inline void dot_body(int64_t n, int B, const V *__restrict__ a, const V *__restrict__ b, int mode,
+                                double2 *__restrict__ partial, int blk, int nblocks) {
   const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+  const int64_t chunk = (n + nblocks - 1) / nblocks;
+  const int64_t r0 = blk * chunk, r1 = min(n, r0 + chunk);
   double2 acc = {0, 0};
   int64_t r = r0 + rl;
   for (; r + 3 * R < r1; r += 4 * R) {
@@ -596,25 +607,29 @@ __global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__r
     for (int u = 0; u < 4; u++) red_term(acc, av[u], bv[u], mode);
   }
   for (; r < r1; r += R) red_term(acc, a[r * B + f], mode == 0 ? b[r * B + f] : a[r * B + f], mode);
-  red_block_store(acc, B, partial);
+  red_block_store(acc, B, partial, blk);
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__restrict__ a, const V *__restrict__ b,
+                                                  int mode, double2 *__restrict__ partial) {
+  dot_body<V>(n, B, a, b, mode, partial, blockIdx.x, gridDim.x);
 }
 
 // x += alpha p ; r -= alpha q ; partial sums of |r|^2 (dot_kernel's layout);
 // with u != nullptr also u = D^-1 r, the first l1-Jacobi sweep of the
 // preconditioner (jacobi0_kernel's values)
 template <typename T>
-__global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B, const double2 *__restrict__ alpha,
-                                                               const typename VT<T>::t *__restrict__ p,
-                                                               const typename VT<T>::t *__restrict__ q,
-                                                               typename VT<T>::t *__restrict__ x,
-                                                               typename VT<T>::t *__restrict__ r,
-                                                               double2 *__restrict__ partial, OpArgs<T> la,
-                                                               const OpCoef *__restrict__ coefp,
-                                                               typename VT<T>::t *__restrict__ u) {
+__device__ inline void update_norm_body(int64_t n, int B, const double2 *__restrict__ alpha,
+                                        const typename VT<T>::t *__restrict__ p,
+                                        const typename VT<T>::t *__restrict__ q, typename VT<T>::t *__restrict__ x,
+                                        typename VT<T>::t *__restrict__ r, double2 *__restrict__ partial,
+                                        const OpArgs<T> &la, const OpCoef *__restrict__ coefp,
+                                        typename VT<T>::t *__restrict__ u, int blk, int nblocks) {
   using V = typename VT<T>::t;
   const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+  const int64_t chunk = (n + nblocks - 1) / nblocks;
+  const int64_t r0 = blk * chunk, r1 = min(n, r0 + chunk);
   const double2 a = alpha[f];
   const bool upd = !(a.x == 0 && a.y == 0);  // frozen column: x, r untouched
   double2 acc = {0, 0};
@@ -634,12 +649,23 @@ __global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B,
     if (u) u[i] = jacobi0_value<T>(la, coefp, row, f, rv);
     acc.x += (double)rv.x * rv.x + (double)rv.y * rv.y;
   }
-  red_block_store(acc, B, partial);
+  red_block_store(acc, B, partial, blk);
 }
 
-__global__ __launch_bounds__(kRedFinalThreads) void dot_final_kernel(int nblk, int B,
-                                                                     const double2 *__restrict__ partial,
-                                                                     double2 *__restrict__ out) {
+template <typename T>
+__global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B, const double2 *__restrict__ alpha,
+                                                               const typename VT<T>::t *__restrict__ p,
+                                                               const typename VT<T>::t *__restrict__ q,
+                                                               typename VT<T>::t *__restrict__ x,
+                                                               typename VT<T>::t *__restrict__ r,
+                                                               double2 *__restrict__ partial, OpArgs<T> la,
+                                                               const OpCoef *__restrict__ coefp,
+                                                               typename VT<T>::t *__restrict__ u) {
+  update_norm_body<T>(n, B, alpha, p, q, x, r, partial, la, coefp, u, blockIdx.x, gridDim.x);
+}
+
+__device__ inline void dot_final_body(int nblk, int B, const double2 *__restrict__ partial,
+                                      double2 *__restrict__ out) {
   __shared__ double2 sh[kRedFinalThreads];
   const int t = threadIdx.x, f = t % B, rl = t / B, R = kRedFinalThreads / B;
   double2 acc = {0, 0};
@@ -670,9 +696,15 @@ __global__ __launch_bounds__(kRedFinalThreads) void dot_final_kernel(int nblk, i
   if (rl == 0) out[f] = sh[t];
 }
 
+__global__ __launch_bounds__(kRedFinalThreads) void dot_final_kernel(int nblk, int B,
+                                                                     const double2 *__restrict__ partial,
+                                                                     double2 *__restrict__ out) {
+  dot_final_body(nblk, B, partial, out);
+}
+
 // COCG scalars on the device: alpha = rho / mu (0 if inactive), beta = rho'/rho.
 // active[f]: 0 = frozen column, 1 = iterating, 2 = (re)starting (beta = 0, p = z)
-__global__ void cocg_alpha_kernel(int B, const double2 *rho, const double2 *mu, const int *active, double2 *alpha) {
+__device__ inline void alpha_body(int B, const double2 *rho, const double2 *mu, const int *active, double2 *alpha) {
   const int f = threadIdx.x;
   if (f >= B) return;
   double2 a = {0, 0};
@@ -683,8 +715,11 @@ __global__ void cocg_alpha_kernel(int B, const double2 *rho, const double2 *mu, 
   }
   alpha[f] = a;
 }
+__global__ void cocg_alpha_kernel(int B, const double2 *rho, const double2 *mu, const int *active, double2 *alpha) {
+  alpha_body(B, rho, mu, active, alpha);
+}
 
-__global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, const int *active, double2 *beta) {
+__device__ inline void beta_body(int B, const double2 *rho_new, double2 *rho, const int *active, double2 *beta) {
   const int f = threadIdx.x;
   if (f >= B) return;
   double2 b = {0, 0};
@@ -696,12 +731,14 @@ __global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, co
   beta[f] = b;
   rho[f] = rho_new[f];
 }
+__global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, const int *active, double2 *beta) {
+  beta_body(B, rho_new, rho, active, beta);
+}
 
 // p = z + beta p
 template <typename V>
-__global__ void cocg_dir_kernel(int64_t nt, int B, const double2 *__restrict__ beta, const V *__restrict__ z,
-                                V *__restrict__ p) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void dir_body(int64_t nt, int B, const double2 *__restrict__ beta, const V *__restrict__ z,
+                                V *__restrict__ p, int64_t i) {
   if (i >= nt) return;
   const double2 bt = beta[i % B];
   const V zv = z[i];
@@ -714,6 +751,11 @@ __global__ void cocg_dir_kernel(int64_t nt, int B, const double2 *__restrict__ b
   o.x = zv.x + bt.x * pv.x - bt.y * pv.y;
   o.y = zv.y + bt.x * pv.y + bt.y * pv.x;
   p[i] = o;
+}
+template <typename V>
+__global__ void cocg_dir_kernel(int64_t nt, int B, const double2 *__restrict__ beta, const V *__restrict__ z,
+                                V *__restrict__ p) {
+  dir_body<V>(nt, B, beta, z, p, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // refinement: r_in = r64 * s_f (to the inner precision), and x64 += d * w_f
@@ -2169,58 +2211,132 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
 // residual.  Every column is computed independently of the others, so a
 // frequency's result does not depend on which frequencies share the block.
 template <typename T, int B>
-static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::complex<double> Z, double *X,
-                        double *relres, int32_t *iters, int &refinements) {
-  Model &m = S.m;
-  const elp_solve_opts &o = S.o;
-  hipStream_t s = S.s;
+struct SlotRun {
   using V = typename VT<T>::t;
-  const int64_t nt = m.n * B;
-  const int tpb = 256;
-  // right-hand side (the same for every frequency)
+  static constexpr int tpb = 256;
+  Solver<T, B> &S;
+  Model &m;
+  const elp_solve_opts &o;
+  hipStream_t s;
+  const int64_t nf;
+  const double *freqs;
+  const std::complex<double> Z;
+  double *X, *relres;
+  int32_t *iters;
+  const int64_t nt;
+  int refinements = 0;
+  bool done = false;
+  // right-hand side: F (elPaSo's, with the prescribed values), per column b
+  // (lifted when values are prescribed), FP64 solution and residual
   DArr<double2> F, b64, x64, r64, col;
-  std::vector<double2> hF(m.n);
-  for (int64_t r = 0; r < m.n; r++) hF[r] = {m.F[r].real(), m.F[r].imag()};
-  for (size_t k = 0; k < m.dir_rows.size(); k++) hF[m.dir_rows[k]] = {m.dir_vals[k].real(), m.dir_vals[k].imag()};
-  double nb2 = 0;
-  for (const auto &v : hF) nb2 += v.x * v.x + v.y * v.y;
-  const double nbn = std::sqrt(nb2);
-  if (!(nbn > 0)) {  // b = 0: x = 0 for every frequency
-    if (X) std::fill(X, X + 2 * nf * m.n, 0.0);
-    for (int64_t k = 0; k < nf; k++) {
-      if (relres) relres[k] = 0.0;
-      if (iters) iters[k] = 0;
-    }
-    return;
-  }
-  F.upload(m.ctx, hF);
-  b64.alloc(m.ctx, nt);
-  x64.alloc(m.ctx, nt);
-  r64.alloc(m.ctx, nt);
-  if (X) col.alloc(m.ctx, m.n);
+  double nbn = 0, gnorm2 = 0;
   Lift lift;
-  build_lift(m, lift);
-  double gnorm2 = 0;  // ||g||^2 of the prescribed values
-  for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
-  hipLaunchKernelGGL(bcast_rhs_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, F.p, b64.p);
-  ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
-  ELP_HIP(hipMemsetAsync(r64.p, 0, nt * sizeof(double2), s));
-  ELP_HIP(hipMemsetAsync(S.ri.p, 0, nt * sizeof(V), s));
-  ELP_HIP(hipMemsetAsync(S.xi.p, 0, nt * sizeof(V), s));
-  ELP_HIP(hipMemsetAsync(S.pi.p, 0, nt * sizeof(V), s));
-  const LevelData &L0 = *S.st.levels[0];
-  const bool use_perm = std::is_same<T, float>::value && L0.permuted;
-  const int32_t *perm = use_perm ? L0.perm.p : nullptr;
-  const int32_t *iperm = use_perm ? L0.iperm.p : nullptr;
+  const int32_t *perm = nullptr, *iperm = nullptr;
   double hmax = 0;
-  if (S.st.fdm)
-    for (int d = 0; d < 3; d++) hmax = std::max(hmax, m.box.length[d] / m.box.nel[d]);
-
   // host mirrors (modified only while the stream is idle)
-  std::vector<OpCoef> cA(B), cP(B);
-  std::vector<FdmCoef> cF(B);
+  std::vector<OpCoef> cA, cP;
+  std::vector<FdmCoef> cF;
+  std::vector<int> act, fresh;
+  std::vector<double> scale, wgt;
+  // narrow FP64 residual of the frozen columns (allocated at first use)
+  std::vector<int> fmap;
+  std::vector<OpCoef> cAn;
+  DArr<int> d_fmap, d_fresh;
+  DArr<OpCoef> d_cAn;
+  DArr<double2> xn, yn;
+  DArr<double> d_scale, d_wgt;
+  struct Slot {
+    int64_t f = -1;   // frequency index, -1 = empty
+    int state = 0;    // 0 empty, 1 iterating, 2 frozen (inner solve done)
+    bool broken = false;
+    double prev = 1;  // FP64 relative residual before the current inner solve
+    double w = 0;     // ||r64|| the inner right-hand side was scaled by
+    double tol = 0;   // relative tolerance of the current inner solve
+    // The stopping test is ||r64|| <= rtol ref and the reported residual is
+    // ||r64|| / rep.  Without prescribed values ref = rep = ||F||.  With
+    // lifted values the system solved is A' x = b (b = F - A[:,D] g on the
+    // free rows, g on the Dirichlet rows), x64 = g holds the Dirichlet rows
+    // exactly, so r64 lives on the free rows: ref = ||b|| of the free rows,
+    // rep = ||b||.  The values' identity rows (scale 1) and the free rows
+    // (fluid rows divided by rho w^2, structure rows of order E h) can differ
+    // by many orders, so neither part may set the scale of the other.
+    double ref = 0, rep = 0;
+  };
+  std::vector<Slot> sl;
+  int64_t next = 0;
+  std::vector<int> it_f;
+  // slots take frequencies from the highest down: high frequencies need the
+  // most iterations, so the batch drains on the cheap ones at the end
+  std::vector<int64_t> order;
+  // Coupled models: the structure block is factored at one frequency for a
+  // window of frequencies: the sweep's range cut into as many equal windows
+  // as it has slot-sized groups of frequencies, each factored at its centre.
+  // Frequencies are taken in descending order, so the windows are used one
+  // after the other and the next one is factored in the background.
+  bool cpl = false;
+  double ftop = 0, fbot = 0, wwidth = 0;
+  int64_t nwin = 1, cur_win = -1, want_win = -1;
+  bool need_refactor = false;
+  int freeze_batch = 1;
+
+  SlotRun(Solver<T, B> &S_, int64_t nf_, const double *freqs_, std::complex<double> Z_, double *X_, double *relres_,
+          int32_t *iters_)
+      : S(S_), m(S_.m), o(S_.o), s(S_.s), nf(nf_), freqs(freqs_), Z(Z_), X(X_), relres(relres_), iters(iters_),
+        nt(S_.m.n * B), cA(B), cP(B), cF(B), act(B, 0), fresh(B, 0), scale(B, 0.0), wgt(B, 0.0), fmap(B, 0),
+        cAn(B), sl(B), it_f(nf_, 0), order(nf_) {
+    std::vector<double2> hF(m.n);
+    for (int64_t r = 0; r < m.n; r++) hF[r] = {m.F[r].real(), m.F[r].imag()};
+    for (size_t k = 0; k < m.dir_rows.size(); k++) hF[m.dir_rows[k]] = {m.dir_vals[k].real(), m.dir_vals[k].imag()};
+    double nb2 = 0;
+    for (const auto &v : hF) nb2 += v.x * v.x + v.y * v.y;
+    nbn = std::sqrt(nb2);
+    if (!(nbn > 0)) {  // b = 0: x = 0 for every frequency
+      if (X) std::fill(X, X + 2 * nf * m.n, 0.0);
+      for (int64_t k = 0; k < nf; k++) {
+        if (relres) relres[k] = 0.0;
+        if (iters) iters[k] = 0;
+      }
+      done = true;
+      return;
+    }
+    F.upload(m.ctx, hF);
+    b64.alloc(m.ctx, nt);
+    x64.alloc(m.ctx, nt);
+    r64.alloc(m.ctx, nt);
+    if (X) col.alloc(m.ctx, m.n);
+    build_lift(m, lift);
+    for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
+    hipLaunchKernelGGL(bcast_rhs_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, F.p, b64.p);
+    ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
+    ELP_HIP(hipMemsetAsync(r64.p, 0, nt * sizeof(double2), s));
+    ELP_HIP(hipMemsetAsync(S.ri.p, 0, nt * sizeof(V), s));
+    ELP_HIP(hipMemsetAsync(S.xi.p, 0, nt * sizeof(V), s));
+    ELP_HIP(hipMemsetAsync(S.pi.p, 0, nt * sizeof(V), s));
+    const LevelData &L0 = *S.st.levels[0];
+    const bool use_perm = std::is_same<T, float>::value && L0.permuted;
+    perm = use_perm ? L0.perm.p : nullptr;
+    iperm = use_perm ? L0.iperm.p : nullptr;
+    if (S.st.fdm)
+      for (int d = 0; d < 3; d++) hmax = std::max(hmax, m.box.length[d] / m.box.nel[d]);
+    // slots left empty keep valid (unused) coefficients
+    for (int k = 0; k < B; k++) set_coefs(k, 0);
+    d_fmap.alloc(m.ctx, B);
+    d_cAn.alloc(m.ctx, B);
+    d_fresh.alloc(m.ctx, B);
+    d_scale.alloc(m.ctx, B);
+    d_wgt.alloc(m.ctx, B);
+    for (int64_t k = 0; k < nf; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return freqs[a] > freqs[b]; });
+    cpl = S.st.coupled;
+    ftop = freqs[order[0]];
+    fbot = freqs[order[nf - 1]];
+    nwin = cpl ? std::max<int64_t>(1, (nf + B - 1) / B) : 1;
+    wwidth = nwin > 1 ? (ftop - fbot) / (double)nwin : 0.0;
+    freeze_batch = std::max(1, B / 3);
+  }
+
   // coefficients of frequency fi in slot k
-  auto set_coefs = [&](int k, int64_t fi) {
+  void set_coefs(int k, int64_t fi) {
     const double w = 2.0 * M_PI * freqs[fi];
     double beta = o.shift;
     if (beta < 0) {
@@ -2240,67 +2356,13 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       const std::complex<double> tG = std::abs(Z) > 0 ? std::complex<double>(0, rho) / (w * Z) : 0.0;
       cF[k] = {1.0 / (w * w), {sM.real(), sM.imag()}, {tG.real(), tG.imag()}, rho};
     }
-  };
-  // slots left empty keep valid (unused) coefficients
-  for (int k = 0; k < B; k++) set_coefs(k, 0);
-  std::vector<int> act(B, 0), fresh(B, 0);
-  std::vector<double> scale(B, 0.0), wgt(B, 0.0);
-  // narrow FP64 residual of the frozen columns (allocated at first use)
-  std::vector<int> fmap(B, 0);
-  std::vector<OpCoef> cAn(B);
-  DArr<int> d_fmap;
-  DArr<OpCoef> d_cAn;
-  DArr<double2> xn, yn;
-  d_fmap.alloc(m.ctx, B);
-  d_cAn.alloc(m.ctx, B);
-  DArr<int> d_fresh;
-  DArr<double> d_scale, d_wgt;
-  d_fresh.alloc(m.ctx, B);
-  d_scale.alloc(m.ctx, B);
-  d_wgt.alloc(m.ctx, B);
-  struct Slot {
-    int64_t f = -1;   // frequency index, -1 = empty
-    int state = 0;    // 0 empty, 1 iterating, 2 frozen (inner solve done)
-    bool broken = false;
-    double prev = 1;  // FP64 relative residual before the current inner solve
-    double w = 0;     // ||r64|| the inner right-hand side was scaled by
-    double tol = 0;   // relative tolerance of the current inner solve
-    // The stopping test is ||r64|| <= rtol ref and the reported residual is
-    // ||r64|| / rep.  Without prescribed values ref = rep = ||F||.  With
-    // lifted values the system solved is A' x = b (b = F - A[:,D] g on the
-    // free rows, g on the Dirichlet rows), x64 = g holds the Dirichlet rows
-    // exactly, so r64 lives on the free rows: ref = ||b|| of the free rows,
-    // rep = ||b||.  The values' identity rows (scale 1) and the free rows
-    // (fluid rows divided by rho w^2, structure rows of order E h) can differ
-    // by many orders, so neither part may set the scale of the other.
-    double ref = 0, rep = 0;
-  };
-  std::vector<Slot> sl(B);
-  int64_t next = 0;
-  std::vector<int> it_f(nf, 0);
-  // slots take frequencies from the highest down: high frequencies need the
-  // most iterations, so the batch drains on the cheap ones at the end
-  std::vector<int64_t> order(nf);
-  for (int64_t k = 0; k < nf; k++) order[k] = k;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return freqs[a] > freqs[b]; });
-
-  // Coupled models: the structure block is factored at one frequency for a
-  // window of frequencies: the sweep's range cut into as many equal windows
-  // as it has slot-sized groups of frequencies, each factored at its centre.
-  // Frequencies are taken in descending order, so the windows are used one
-  // after the other and the next one is factored in the background.
-  const bool cpl = S.st.coupled;
-  const double ftop = freqs[order[0]], fbot = freqs[order[nf - 1]];
-  const int64_t nwin = cpl ? std::max<int64_t>(1, (nf + B - 1) / B) : 1;
-  const double wwidth = nwin > 1 ? (ftop - fbot) / (double)nwin : 0.0;
-  auto window_of = [&](double f) -> int64_t {
+  }
+  int64_t window_of(double f) const {
     if (!(wwidth > 0)) return 0;
     return std::min<int64_t>(nwin - 1, std::max<int64_t>(0, (int64_t)std::floor((ftop - f) / wwidth)));
-  };
-  auto centre_of = [&](int64_t w) { return nwin > 1 ? ftop - wwidth * (w + 0.5) : 0.5 * (ftop + fbot); };
-  int64_t cur_win = -1, want_win = -1;
-  bool need_refactor = false;
-  auto use_window = [&](int64_t w) {
+  }
+  double centre_of(int64_t w) const { return nwin > 1 ? ftop - wwidth * (w + 0.5) : 0.5 * (ftop + fbot); }
+  void use_window(int64_t w) {
     std::complex<double> ks, ms;
     const double fc = centre_of(w);
     if (!S.take_prefetch(fc)) {
@@ -2313,17 +2375,17 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       plate_coefs(m, fn, ks, ms);
       S.prefetch(fn, ks, ms);
     }
-  };
+  }
   // inner tolerance for a slot at FP64 relative residual rel: inner_rtol, or
   // just past the target when that is within reach of one FP32 solve
-  auto inner_tol = [&](double rel) {
+  double inner_tol(double rel) const {
     // FP64 inner solves aim straight below the target
     if (std::is_same<T, double>::value) return std::max(0.3 * o.rtol / rel, 1e-15);
-    const double finish = 0.5 * o.rtol / rel;
-    return finish >= 1e-4 ? std::min(o.inner_rtol, finish) : o.inner_rtol;
-  };
+    const double finish_at = 0.5 * o.rtol / rel;
+    return finish_at >= 1e-4 ? std::min(o.inner_rtol, finish_at) : o.inner_rtol;
+  }
   // assign the next frequency to slot k (stream idle)
-  auto assign = [&](int k) {
+  bool assign(int k) {
     if (next >= nf) {
       sl[k] = Slot();
       act[k] = 0;
@@ -2347,13 +2409,13 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
     act[k] = 2;
     scale[k] = 1.0 / nbn;
     return true;
-  };
-  auto upload_coefs = [&]() {
+  }
+  void upload_coefs() {
     ELP_HIP(hipMemcpyAsync(S.coefA.p, cA.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
     ELP_HIP(hipMemcpyAsync(S.coefP.p, cP.data(), sizeof(OpCoef) * B, hipMemcpyHostToDevice, s));
     if (S.st.fdm) ELP_HIP(hipMemcpyAsync(S.coefF.p, cF.data(), sizeof(FdmCoef) * B, hipMemcpyHostToDevice, s));
-  };
-  auto finish = [&](int k, double rel) {
+  }
+  void finish(int k, double rel) {
     const int64_t fi = sl[k].f;
     if (relres) relres[fi] = rel;
     if (iters) iters[fi] = it_f[fi];
@@ -2362,10 +2424,10 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
       ELP_HIP(hipGetLastError());
       ELP_HIP(hipMemcpyAsync(X + 2 * fi * m.n, col.p, sizeof(double2) * m.n, hipMemcpyDeviceToHost, s));
     }
-  };
+  }
   // start: fill the slots, x64 = g (0 without prescribed values), r64 = b -
   // A x64, inner rhs = r64 / ||r64|| (= b / ||b|| without prescribed values)
-  auto start_columns = [&]() {
+  void start_columns() {
     ELP_HIP(hipMemcpyAsync(d_fresh.p, fresh.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     launch_lift(m, lift, B, d_fresh.p, S.coefA.p, F.p, b64.p, s);
     hipLaunchKernelGGL(slot_fresh_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, d_fresh.p, b64.p,
@@ -2394,24 +2456,27 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
                        S.fused_box() ? S.lw[0].x.p : nullptr);
     ELP_HIP(hipGetLastError());
     std::fill(fresh.begin(), fresh.end(), 0);
-  };
-  for (int k = 0; k < B; k++) assign(k);
-  upload_coefs();
-  if (cpl) use_window(0);
-  start_columns();
-
-  const int freeze_batch = std::max(1, B / 3);
-  while (true) {
+  }
+  void start() {
+    if (done) return;
+    for (int k = 0; k < B; k++) assign(k);
+    upload_coefs();
+    if (cpl) use_window(0);
+    start_columns();
+  }
+  // another COCG iteration is due (else the run is over)
+  bool iterating() {
+    if (done) return false;
     int n_iter = 0;
     for (int k = 0; k < B; k++) n_iter += sl[k].state == 1;
-    if (n_iter == 0) break;
-    // z = M r, rho' = r.z, beta (0 for restarting columns), p = z + beta p
-    S.precond(S.ri.p, S.zi.p);
-    S.reduce(m.n, S.ri.p, S.zi.p, 0, S.rho_new.p);
-    hipLaunchKernelGGL(cocg_beta_kernel, dim3(1), dim3(64), 0, s, B, S.rho_new.p, S.rho.p, S.active.p, S.beta.p);
-    hipLaunchKernelGGL((cocg_dir_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, S.beta.p, S.zi.p,
-                       S.pi.p);
-    ELP_HIP(hipGetLastError());
+    if (n_iter == 0) {
+      ELP_HIP(hipStreamSynchronize(s));
+      done = true;
+    }
+    return !done;
+  }
+  // after p = z + beta p: columns that (re)started iterate from now on
+  void restart_flags() {
     bool restarted = false;
     for (int k = 0; k < B; k++)
       if (act[k] == 2) {
@@ -2419,151 +2484,186 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
         restarted = true;
       }
     if (restarted) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
-    // q = A p, alpha = rho / p.q, x += alpha p, r -= alpha q
-    if (S.timing) ELP_HIP(hipEventRecord(S.ev0.e, s));
-    S.template op<0>(0, S.coefA.p, S.pi.p, nullptr, S.qi.p);
-    if (S.timing) ELP_HIP(hipEventRecord(S.ev1.e, s));
-    S.spmv_calls++;
-    S.reduce(m.n, S.pi.p, S.qi.p, 0, S.mu.p);
-    hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
-    S.update_norm(S.fused_box());
-    S.fetch(S.red_out.p);
-    if (S.timing) {
-      float ms = 0;
-      ELP_HIP(hipEventElapsedTime(&ms, S.ev0.e, S.ev1.e));
-      S.spmv_ms += ms;
-    }
-    int n_frozen = 0;
-    n_iter = 0;
-    bool changed = false;
-    for (int k = 0; k < B; k++) {
-      Slot &q = sl[k];
-      if (q.state == 1) {
-        it_f[q.f]++;
-        const double nr = std::sqrt(S.h_red[k].x);
-        const bool bad = !std::isfinite(nr) || nr > 1e6;
-        if (bad || !(nr > q.tol) || it_f[q.f] >= o.max_iters) {
-          q.state = 2;
-          q.broken = bad;
-          act[k] = 0;
-          changed = true;
-        }
-      }
-      n_frozen += q.state == 2;
-      n_iter += q.state == 1;
-    }
-    if (changed) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
-    if (n_frozen == 0 || (n_frozen < freeze_batch && n_iter > 0)) continue;
+  }
+  // host side of an iteration, with ||r_f||^2 of every column in S.h_red:
+  // frozen columns take an FP64 refinement step together, finished
+  // frequencies are written out and their slots take the next ones
+  void after_norms();
+};
 
-    // FP64 refinement of the frozen columns
-    refinements++;
-    std::vector<int> was_iter(B, 0);
-    for (int k = 0; k < B; k++) was_iter[k] = sl[k].state == 1;
-    for (int k = 0; k < B; k++) wgt[k] = (sl[k].state == 2 && !sl[k].broken) ? sl[k].w : 0.0;
+template <typename T, int B>
+void SlotRun<T, B>::after_norms() {
+  int n_frozen = 0, n_iter = 0;
+  bool changed = false;
+  for (int k = 0; k < B; k++) {
+    Slot &q = sl[k];
+    if (q.state == 1) {
+      it_f[q.f]++;
+      const double nr = std::sqrt(S.h_red[k].x);
+      const bool bad = !std::isfinite(nr) || nr > 1e6;
+      if (bad || !(nr > q.tol) || it_f[q.f] >= o.max_iters) {
+        q.state = 2;
+        q.broken = bad;
+        act[k] = 0;
+        changed = true;
+      }
+    }
+    n_frozen += q.state == 2;
+    n_iter += q.state == 1;
+  }
+  if (changed) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+  if (n_frozen == 0 || (n_frozen < freeze_batch && n_iter > 0)) return;
+
+  // FP64 refinement of the frozen columns
+  refinements++;
+  std::vector<int> was_iter(B, 0);
+  for (int k = 0; k < B; k++) was_iter[k] = sl[k].state == 1;
+  for (int k = 0; k < B; k++) wgt[k] = (sl[k].state == 2 && !sl[k].broken) ? sl[k].w : 0.0;
+  ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
+                     iperm);
+  // r64 = b - A x64 on the frozen columns: on a narrow block of 8 or 16
+  // columns when few are frozen (the gathers of x shrink with the block)
+  int nfz = 0;
+  for (int k = 0; k < B; k++)
+    if (sl[k].state == 2) fmap[nfz++] = k;
+  if (B == 32 && nfz <= 16) {
+    const int CB = nfz <= 8 ? 8 : 16;
+    for (int f = nfz; f < CB; f++) fmap[f] = fmap[0];  // padding columns repeat a frozen one
+    for (int f = 0; f < CB; f++) cAn[f] = cA[fmap[f]];
+    if (!xn.p) {
+      xn.alloc(m.ctx, m.n * 16);
+      yn.alloc(m.ctx, m.n * 16);
+    }
+    ELP_HIP(hipMemcpyAsync(d_fmap.p, fmap.data(), sizeof(int) * CB, hipMemcpyHostToDevice, s));
+    ELP_HIP(hipMemcpyAsync(d_cAn.p, cAn.data(), sizeof(OpCoef) * CB, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gather_cols_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, d_fmap.p, x64.p,
+                       xn.p);
+    if (CB == 8) S.template op_fp64_narrow<8>(d_cAn.p, xn.p, yn.p);
+    else S.template op_fp64_narrow<16>(d_cAn.p, xn.p, yn.p);
+    hipLaunchKernelGGL(scatter_residual_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, nfz,
+                       d_fmap.p, b64.p, yn.p, r64.p);
+    ELP_HIP(hipGetLastError());
+  } else {
+    S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+  }
+  S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+  S.fetch(S.red_out.p);
+  std::vector<double> rel(B);
+  for (int k = 0; k < B; k++) rel[k] = std::sqrt(S.h_red[k].x) / (sl[k].ref > 0 ? sl[k].ref : nbn);
+  // a step that did not reduce the residual is taken back
+  bool any_undo = false;
+  std::vector<int> undo(B, 0);
+  for (int k = 0; k < B; k++) {
+    wgt[k] = 0.0;
+    if (sl[k].state != 2 || sl[k].broken) continue;
+    if (!(rel[k] < sl[k].prev)) {
+      undo[k] = 1;
+      wgt[k] = -sl[k].w;
+      any_undo = true;
+    }
+  }
+  if (any_undo) {
     ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
                        iperm);
-    // r64 = b - A x64 on the frozen columns: on a narrow block of 8 or 16
-    // columns when few are frozen (the gathers of x shrink with the block)
-    int nfz = 0;
-    for (int k = 0; k < B; k++)
-      if (sl[k].state == 2) fmap[nfz++] = k;
-    if (B == 32 && nfz <= 16) {
-      const int CB = nfz <= 8 ? 8 : 16;
-      for (int f = nfz; f < CB; f++) fmap[f] = fmap[0];  // padding columns repeat a frozen one
-      for (int f = 0; f < CB; f++) cAn[f] = cA[fmap[f]];
-      if (!xn.p) {
-        xn.alloc(m.ctx, m.n * 16);
-        yn.alloc(m.ctx, m.n * 16);
-      }
-      ELP_HIP(hipMemcpyAsync(d_fmap.p, fmap.data(), sizeof(int) * CB, hipMemcpyHostToDevice, s));
-      ELP_HIP(hipMemcpyAsync(d_cAn.p, cAn.data(), sizeof(OpCoef) * CB, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(gather_cols_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, d_fmap.p, x64.p,
-                         xn.p);
-      if (CB == 8) S.template op_fp64_narrow<8>(d_cAn.p, xn.p, yn.p);
-      else S.template op_fp64_narrow<16>(d_cAn.p, xn.p, yn.p);
-      hipLaunchKernelGGL(scatter_residual_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, nfz,
-                         d_fmap.p, b64.p, yn.p, r64.p);
-      ELP_HIP(hipGetLastError());
+    ELP_HIP(hipGetLastError());
+  }
+  bool any_new = false;
+  for (int k = 0; k < B; k++) {
+    Slot &q = sl[k];
+    if (q.state != 2) continue;
+    const bool kept_back = q.broken || undo[k];
+    const bool fin = kept_back || rel[k] <= o.rtol || rel[k] > 0.5 * q.prev || it_f[q.f] >= o.max_iters;
+    if (fin) {
+      finish(k, (kept_back ? q.prev : rel[k]) * q.ref / q.rep);
+      any_new |= assign(k);
     } else {
-      S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+      q.prev = rel[k];
+      q.tol = inner_tol(rel[k]);
+      q.w = rel[k] * q.ref;
+      q.state = 1;
+      act[k] = 2;
+      scale[k] = 1.0 / q.w;
     }
-    S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
-    S.fetch(S.red_out.p);
-    std::vector<double> rel(B);
-    for (int k = 0; k < B; k++) rel[k] = std::sqrt(S.h_red[k].x) / (sl[k].ref > 0 ? sl[k].ref : nbn);
-    // a step that did not reduce the residual is taken back
-    bool any_undo = false;
-    std::vector<int> undo(B, 0);
+  }
+  if (need_refactor) {
+    // slots in the middle of an inner solve keep their progress (x64 +=
+    // w d) and restart on the new residual under the new factorization
+    bool anyf = false;
     for (int k = 0; k < B; k++) {
-      wgt[k] = 0.0;
-      if (sl[k].state != 2 || sl[k].broken) continue;
-      if (!(rel[k] < sl[k].prev)) {
-        undo[k] = 1;
-        wgt[k] = -sl[k].w;
-        any_undo = true;
-      }
+      wgt[k] = was_iter[k] ? sl[k].w : 0.0;
+      anyf |= was_iter[k] != 0;
     }
-    if (any_undo) {
+    if (anyf) {
       ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p,
                          x64.p, iperm);
       ELP_HIP(hipGetLastError());
-    }
-    bool any_new = false;
-    for (int k = 0; k < B; k++) {
-      Slot &q = sl[k];
-      if (q.state != 2) continue;
-      const bool kept_back = q.broken || undo[k];
-      const bool done = kept_back || rel[k] <= o.rtol || rel[k] > 0.5 * q.prev || it_f[q.f] >= o.max_iters;
-      if (done) {
-        finish(k, (kept_back ? q.prev : rel[k]) * q.ref / q.rep);
-        any_new |= assign(k);
-      } else {
-        q.prev = rel[k];
-        q.tol = inner_tol(rel[k]);
-        q.w = rel[k] * q.ref;
-        q.state = 1;
-        act[k] = 2;
-        scale[k] = 1.0 / q.w;
-      }
-    }
-    if (need_refactor) {
-      // slots in the middle of an inner solve keep their progress (x64 +=
-      // w d) and restart on the new residual under the new factorization
-      bool anyf = false;
+      S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
+      S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+      S.fetch(S.red_out.p);
       for (int k = 0; k < B; k++) {
-        wgt[k] = was_iter[k] ? sl[k].w : 0.0;
-        anyf |= was_iter[k] != 0;
+        if (!was_iter[k]) continue;
+        Slot &q = sl[k];
+        const double rl = std::sqrt(S.h_red[k].x) / q.ref;
+        q.prev = rl;
+        q.tol = inner_tol(rl);
+        q.w = rl * q.ref;
+        act[k] = 2;
+        scale[k] = q.w > 0 ? 1.0 / q.w : 0.0;
       }
-      if (anyf) {
-        ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p,
-                           x64.p, iperm);
-        ELP_HIP(hipGetLastError());
-        S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
-        S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
-        S.fetch(S.red_out.p);
-        for (int k = 0; k < B; k++) {
-          if (!was_iter[k]) continue;
-          Slot &q = sl[k];
-          const double rl = std::sqrt(S.h_red[k].x) / q.ref;
-          q.prev = rl;
-          q.tol = inner_tol(rl);
-          q.w = rl * q.ref;
-          act[k] = 2;
-          scale[k] = q.w > 0 ? 1.0 / q.w : 0.0;
-        }
-      }
-      use_window(want_win);
-      need_refactor = false;
     }
-    // the stream is idle here only after a sync: the host mirrors are reused
-    ELP_HIP(hipStreamSynchronize(s));
-    if (any_new) upload_coefs();
-    start_columns();
+    use_window(want_win);
+    need_refactor = false;
   }
+  // the stream is idle here only after a sync: the host mirrors are reused
   ELP_HIP(hipStreamSynchronize(s));
+  if (any_new) upload_coefs();
+  start_columns();
+}
+
+// one COCG iteration of a run on its own: z = M r, rho' = r.z, beta (0 for
+// restarting columns), p = z + beta p; q = A p, alpha = rho / p.q, x +=
+// alpha p, r -= alpha q, ||r||^2 per column to the host
+template <typename T, int B>
+static void run_iteration(SlotRun<T, B> &R) {
+  using V = typename VT<T>::t;
+  Solver<T, B> &S = R.S;
+  hipStream_t s = S.s;
+  const int64_t n = S.m.n, nt = n * B;
+  S.precond(S.ri.p, S.zi.p);
+  S.reduce(n, S.ri.p, S.zi.p, 0, S.rho_new.p);
+  hipLaunchKernelGGL(cocg_beta_kernel, dim3(1), dim3(64), 0, s, B, S.rho_new.p, S.rho.p, S.active.p, S.beta.p);
+  hipLaunchKernelGGL((cocg_dir_kernel<V>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B, S.beta.p, S.zi.p, S.pi.p);
+  ELP_HIP(hipGetLastError());
+  R.restart_flags();
+  if (S.timing) ELP_HIP(hipEventRecord(S.ev0.e, s));
+  S.template op<0>(0, S.coefA.p, S.pi.p, nullptr, S.qi.p);
+  if (S.timing) ELP_HIP(hipEventRecord(S.ev1.e, s));
+  S.spmv_calls++;
+  S.reduce(n, S.pi.p, S.qi.p, 0, S.mu.p);
+  hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
+  S.update_norm(S.fused_box());
+  S.fetch(S.red_out.p);
+  if (S.timing) {
+    float ms = 0;
+    ELP_HIP(hipEventElapsedTime(&ms, S.ev0.e, S.ev1.e));
+    S.spmv_ms += ms;
+  }
+}
+
+// Continuous batching of one sweep on B slots (see SlotRun).
+template <typename T, int B>
+static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::complex<double> Z, double *X,
+                        double *relres, int32_t *iters, int &refinements) {
+  SlotRun<T, B> R(S, nf, freqs, Z, X, relres, iters);
+  R.start();
+  while (R.iterating()) {
+    run_iteration(R);
+    R.after_norms();
+  }
+  refinements = R.refinements;
 }
 
 template <typename T, int B>
