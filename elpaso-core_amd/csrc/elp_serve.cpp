@@ -1,24 +1,32 @@
 // Serving loop: continuous batching of frequency-sweep requests on one GPU.
 //
-// Requests queue in arrival order.  A scheduler thread admits the head of the
-// queue whenever a slot is free and the request's device footprint (assembled
-// matrices + solver work space for its frequency block) fits in the HBM
-// budget next to the requests already running; each admitted request runs on
-// its own slot (worker thread + HIP stream), so requests that arrive while
-// others are on the GPU overlap with them on the device.  Matrices stay
-// resident for the life of the request.  A request's arithmetic does not
-// depend on what else runs (private buffers, deterministic kernels), so its
-// result is the same alone and in the mix.
+// Requests queue in arrival order.  A scheduler admits the head of the queue
+// whenever fewer than max_slots requests are resident and its device
+// footprint (assembled matrices + solver work space) fits in the HBM budget
+// next to the resident ones.  An admitted request is set up (assembly if
+// needed, solver state, work space) on a set-up thread with its own stream,
+// so a large request's set-up does not stall the others, and then joins the
+// group of resident requests of its precision and slot count.  A group's
+// driver thread advances all of its requests together: each step is one
+// COCG iteration of every request, with the operator, the dot products and
+// the vector updates of all of them fused into single launches (the frequency
+// slots of different requests share each launch), and each request's slots
+// refilled with its next frequencies as they converge.  Requests that arrive
+// while others run join the group at the next step.  A request's arithmetic
+// is the same in any mix (each request keeps its own partition of every
+// launch), so its result is the same alone and in the mix.  Requests of the
+// plain FP64 precision (one frequency at a time) run alone on the set-up
+// thread.
 #include "elp_serve.h"
 
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <thread>
 #include <unordered_map>
 
 #include "elp_solver.h"
-
 
 namespace elp {
 
@@ -34,7 +42,21 @@ struct Job {
   int status = 0;
   std::string error;
   bool done = false;
+  int group = -1;  // key of the group it runs in
   std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+struct Server;
+
+// one group: a driver thread with its own stream
+struct Group {
+  Server *sv = nullptr;
+  int key = 0;
+  hipStream_t s = nullptr;
+  std::unique_ptr<GroupDriver> drv;
+  std::deque<std::unique_ptr<GroupItem>> inbox;
+  std::thread th;
+  void loop();
 };
 
 struct Server {
@@ -43,74 +65,38 @@ struct Server {
   int64_t budget = 0;
   int max_slots = 1;
   std::mutex mu;
-  std::condition_variable cv_sched, cv_done;
-  std::deque<std::shared_ptr<Job>> pending;
+  std::condition_variable cv_sched, cv_done, cv_setup, cv_group;
+  std::deque<std::shared_ptr<Job>> pending;   // submitted, not admitted
+  std::deque<std::shared_ptr<Job>> to_setup;  // admitted, waiting for set-up
   std::unordered_map<int64_t, std::shared_ptr<Job>> jobs;
+  std::map<int, std::unique_ptr<Group>> groups;
   int64_t next_id = 1, in_use = 0, admitted = 0;
-  int running = 0;
+  int running = 0;  // admitted and not finished
   int waiters = 0;  // callers inside elp_server_wait
   bool stop = false;
-  std::thread scheduler;
-  std::vector<std::thread> workers;
-  std::deque<std::shared_ptr<Job>> ready;  // admitted, waiting for a slot thread
-  std::condition_variable cv_ready;
+  std::thread scheduler, setup;
 
-  void run_job(Job &j, hipStream_t s) {
-    Model &m = *j.model->m;
-    try {
-      ELP_REQUIRE(s, ELP_EHIP, "the server's worker could not create its HIP stream");
-      ELP_HIP(hipSetDevice(ctx->device));
-      std::lock_guard<std::mutex> lk(m.mu);
-      if (!m.assembled) assemble_model(m, s);
-      const int64_t nf = (int64_t)j.freqs.size();
-      j.X.assign(2 * nf * m.n, 0.0);
-      j.relres.assign(nf, 0.0);
-      j.iters.assign(nf, 0);
-      elp_sweep_stats st{};
-      const int64_t bad = sweep(m, nf, j.freqs.data(), j.Z, j.opts, j.X.data(), j.relres.data(), j.iters.data(), &st, s);
-      j.status = bad ? ELP_ENOCONV : 0;
-      if (bad) j.error = std::to_string(bad) + " of " + std::to_string(nf) + " frequencies did not reach the tolerance";
-    } catch (const Error &e) {
-      j.status = e.code;
-      j.error = e.what();
-    } catch (const std::exception &e) {
-      j.status = ELP_EINVAL;
-      j.error = e.what();
+  // a request ended (under mu): results are in the job
+  void complete(Job &j, int status, const std::string &err) {
+    if (j.done) return;
+    j.status = status;
+    j.error = err;
+    if (j.model) {
+      delete j.model;  // the request's device memory goes with its model
+      j.model = nullptr;
     }
-    // the request's device memory is released with its model
-    {
-      std::lock_guard<std::mutex> lk(m.mu);
-      m.solver.reset();
-    }
-    delete j.model;
-    j.model = nullptr;
+    j.t_done = std::chrono::steady_clock::now();
+    j.done = true;
+    in_use -= j.bytes;
+    running--;
+    cv_done.notify_all();
+    cv_sched.notify_all();
   }
 
-  void worker_loop() {
-    hipStream_t s = nullptr;
-    (void)hipSetDevice(ctx->device);
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;  // jobs then fail
-    while (true) {
-      std::shared_ptr<Job> j;
-      {
-        std::unique_lock<std::mutex> lk(mu);
-        cv_ready.wait(lk, [&] { return stop || !ready.empty(); });
-        if (ready.empty()) break;
-        j = ready.front();
-        ready.pop_front();
-      }
-      run_job(*j, s);
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        j->t_done = std::chrono::steady_clock::now();
-        j->done = true;
-        in_use -= j->bytes;
-        running--;
-      }
-      cv_done.notify_all();
-      cv_sched.notify_all();
-    }
-    if (s) (void)hipStreamDestroy(s);
+  static int unconverged(const Job &j) {
+    int bad = 0;
+    for (double r : j.relres) bad += !(r <= j.opts.rtol);
+    return bad;
   }
 
   void scheduler_loop() {
@@ -126,11 +112,153 @@ struct Server {
       in_use += j->bytes;
       running++;
       admitted++;
-      ready.push_back(j);
-      cv_ready.notify_one();
+      to_setup.push_back(j);
+      cv_setup.notify_one();
     }
   }
+
+  void setup_loop() {
+    hipStream_t s = nullptr;
+    (void)hipSetDevice(ctx->device);
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    while (true) {
+      std::shared_ptr<Job> j;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_setup.wait(lk, [&] { return stop || !to_setup.empty(); });
+        if (to_setup.empty()) break;
+        j = to_setup.front();
+        to_setup.pop_front();
+      }
+      Model &m = *j->model->m;
+      const int64_t nf = (int64_t)j->freqs.size();
+      std::unique_ptr<GroupItem> item;
+      int key = -1;
+      try {
+        ELP_REQUIRE(s, ELP_EHIP, "the server could not create its set-up stream");
+        ELP_HIP(hipSetDevice(ctx->device));
+        std::lock_guard<std::mutex> mlk(m.mu);
+        if (!m.assembled) assemble_model(m, s);
+        j->X.assign(2 * nf * m.n, 0.0);
+        j->relres.assign(nf, 0.0);
+        j->iters.assign(nf, 0);
+        item = group_prepare(m, nf, j->freqs.data(), j->Z, j->opts, j->X.data(), j->relres.data(),
+                             j->iters.data(), s, &key);
+        if (!item) {
+          // one frequency at a time: solved here, alone
+          elp_sweep_stats st{};
+          sweep(m, nf, j->freqs.data(), j->Z, j->opts, j->X.data(), j->relres.data(), j->iters.data(), &st, s);
+        }
+      } catch (const Error &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        complete(*j, e.code, e.what());
+        continue;
+      } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        complete(*j, ELP_EINVAL, e.what());
+        continue;
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      if (!item) {
+        const int bad = unconverged(*j);
+        complete(*j, bad ? ELP_ENOCONV : 0,
+                 bad ? std::to_string(bad) + " frequencies did not reach the tolerance" : std::string());
+        continue;
+      }
+      item->tag = j->id;
+      j->group = key;
+      auto &g = groups[key];
+      if (!g) {
+        g.reset(new Group());
+        g->sv = this;
+        g->key = key;
+        g->th = std::thread([gp = g.get()] { gp->loop(); });
+      }
+      g->inbox.push_back(std::move(item));
+      cv_group.notify_all();
+    }
+    if (s) (void)hipStreamDestroy(s);
+  }
 };
+
+void Group::loop() {
+  Server &S = *sv;
+  (void)hipSetDevice(S.ctx->device);
+  std::string init_err;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) init_err = "cannot create a stream";
+  if (init_err.empty()) {
+    try {
+      drv = group_driver(key, S.ctx, s);
+    } catch (const std::exception &e) {
+      init_err = e.what();
+    }
+  }
+  std::vector<std::unique_ptr<GroupItem>> fresh, finished;
+  while (true) {
+    {
+      std::unique_lock<std::mutex> lk(S.mu);
+      S.cv_group.wait(lk, [&] { return S.stop || !inbox.empty() || (drv && !drv->empty()); });
+      if (S.stop && inbox.empty() && (!drv || drv->empty())) break;
+      while (!inbox.empty()) {
+        fresh.push_back(std::move(inbox.front()));
+        inbox.pop_front();
+      }
+    }
+    finished.clear();
+    std::string err;
+    int code = 0;
+    std::vector<int64_t> failed;
+    try {
+      ELP_REQUIRE(init_err.empty(), ELP_EHIP, init_err);
+      for (auto &it : fresh) {
+        const int64_t tag = it->tag;
+        try {
+          drv->add(std::move(it));
+        } catch (const Error &e) {
+          failed.push_back(tag);
+          code = e.code;
+          err = e.what();
+        }
+      }
+      fresh.clear();
+      drv->step(finished);
+    } catch (const Error &e) {
+      code = e.code;
+      err = e.what();
+    } catch (const std::exception &e) {
+      code = ELP_EINVAL;
+      err = e.what();
+    }
+    std::lock_guard<std::mutex> lk(S.mu);
+    for (auto &it : finished) {
+      auto jt = S.jobs.find(it->tag);
+      it.reset();  // the request's solver memory is released before its model
+      if (jt == S.jobs.end()) continue;
+      const int bad = Server::unconverged(*jt->second);
+      S.complete(*jt->second, bad ? ELP_ENOCONV : 0,
+                 bad ? std::to_string(bad) + " frequencies did not reach the tolerance" : std::string());
+    }
+    for (int64_t tag : failed) {
+      auto jt = S.jobs.find(tag);
+      if (jt != S.jobs.end()) S.complete(*jt->second, code, err);
+    }
+    if (code && failed.empty()) {
+      // a failure of the group's step: every request still in the group
+      // ends with the error
+      drv.reset();
+      fresh.clear();
+      for (auto &kv : S.jobs)
+        if (!kv.second->done && kv.second->group == key) S.complete(*kv.second, code, err);
+      try {
+        drv = group_driver(key, S.ctx, s);
+      } catch (const std::exception &e) {
+        init_err = e.what();
+      }
+    }
+  }
+  drv.reset();
+  if (s) (void)hipStreamDestroy(s);
+}
 
 }  // namespace elp
 
@@ -158,13 +286,13 @@ extern "C" {
 int elp_server_create(elp_ctx *ctx, int64_t hbm_budget_bytes, int max_slots, elp_server **out) {
   ELP_TRY
   ELP_REQUIRE(ctx && out, ELP_EINVAL, "null argument");
-  ELP_REQUIRE(hbm_budget_bytes > 0 && max_slots > 0 && max_slots <= 16, ELP_EINVAL, "invalid budget or slot count");
+  ELP_REQUIRE(hbm_budget_bytes > 0 && max_slots > 0 && max_slots <= 64, ELP_EINVAL, "invalid budget or slot count");
   auto *sv = new elp_server();
   sv->s.ctx = ctx->c.get();
   sv->s.ctx_ref = ctx->c;
   sv->s.budget = hbm_budget_bytes;
   sv->s.max_slots = max_slots;
-  for (int i = 0; i < max_slots; i++) sv->s.workers.emplace_back([sv] { sv->s.worker_loop(); });
+  sv->s.setup = std::thread([sv] { sv->s.setup_loop(); });
   sv->s.scheduler = std::thread([sv] { sv->s.scheduler_loop(); });
   *out = sv;
   ELP_CATCH
@@ -187,12 +315,16 @@ void elp_server_destroy(elp_server *sv) {
     }
     sv->s.pending.clear();
     sv->s.cv_done.notify_all();
+    sv->s.cv_setup.notify_all();
+    sv->s.cv_group.notify_all();
     sv->s.cv_done.wait(lk, [&] { return sv->s.running == 0 && sv->s.waiters == 0; });
   }
   sv->s.cv_sched.notify_all();
-  sv->s.cv_ready.notify_all();
+  sv->s.cv_setup.notify_all();
+  sv->s.cv_group.notify_all();
   sv->s.scheduler.join();
-  for (auto &w : sv->s.workers) w.join();
+  sv->s.setup.join();
+  for (auto &kv : sv->s.groups) kv.second->th.join();
   delete sv;
 }
 

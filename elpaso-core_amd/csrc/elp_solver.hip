@@ -2666,6 +2666,285 @@ static void solve_slots(Solver<T, B> &S, int64_t nf, const double *freqs, std::c
   refinements = R.refinements;
 }
 
+// ---------------------------------------------------------------------------
+// Grouped execution: the sweeps of several requests advance together, one
+// COCG iteration of every request per step, on one stream.  The operator,
+// the dot products, the direction and the update of all requests are single
+// launches over a device table of per-request segments; each segment is
+// computed with exactly the partition (rows per workgroup, reduction blocks)
+// of the request's own launches, so a request's results are the same bits
+// alone, in any mix and through elp_sweep.  Preconditioners and FP64
+// refinement steps stay per request.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RunSeg {
+  int64_t n = 0;
+  OpArgs<T> a{};
+  PadCsr pc{};  // row-padded FP32 operator (B = 32) or empty
+  const OpCoef *coefA = nullptr, *coefP = nullptr;
+  typename VT<T>::t *ri = nullptr, *zi = nullptr, *pi = nullptr, *qi = nullptr, *xi = nullptr, *u = nullptr;
+  double2 *red_part = nullptr, *red_out = nullptr, *rho = nullptr, *rho_new = nullptr, *mu = nullptr,
+          *alpha = nullptr, *beta = nullptr;
+  const int *active = nullptr;
+  int op_blk0 = 0, op_nblk = 0;  // operator workgroups (multiples of 8)
+  int e_blk0 = 0;                // element-wise workgroups of 256 threads
+};
+
+// segment of block b: the last segment whose first block is <= b
+template <typename T>
+__device__ inline int seg_of(const RunSeg<T> *__restrict__ segs, int G, int b, bool op) {
+  int g = 0;
+  for (int k = 1; k < G; k++)
+    if ((op ? segs[k].op_blk0 : segs[k].e_blk0) <= b) g = k;
+  return g;
+}
+
+template <typename T, int B, int WHICH>
+__global__ __launch_bounds__(256) void grp_dot_kernel(const RunSeg<T> *__restrict__ segs) {
+  const RunSeg<T> &q = segs[blockIdx.x / kRedBlocks];
+  const int blk = blockIdx.x % kRedBlocks;
+  if (WHICH == 0) dot_body(q.n, B, q.ri, q.zi, 0, q.red_part, blk, kRedBlocks);
+  else dot_body(q.n, B, q.pi, q.qi, 0, q.red_part, blk, kRedBlocks);
+}
+
+template <typename T, int B, int WHICH>
+__global__ __launch_bounds__(kRedFinalThreads) void grp_final_kernel(const RunSeg<T> *__restrict__ segs) {
+  const RunSeg<T> &q = segs[blockIdx.x];
+  dot_final_body(kRedBlocks, B, q.red_part, WHICH == 0 ? q.rho_new : WHICH == 1 ? q.mu : q.red_out);
+}
+
+template <typename T, int B>
+__global__ void grp_beta_kernel(const RunSeg<T> *__restrict__ segs) {
+  const RunSeg<T> &q = segs[blockIdx.x];
+  beta_body(B, q.rho_new, q.rho, q.active, q.beta);
+}
+
+template <typename T, int B>
+__global__ void grp_alpha_kernel(const RunSeg<T> *__restrict__ segs) {
+  const RunSeg<T> &q = segs[blockIdx.x];
+  alpha_body(B, q.rho, q.mu, q.active, q.alpha);
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void grp_dir_kernel(const RunSeg<T> *__restrict__ segs, int G) {
+  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, false)];
+  const int64_t i = (int64_t)(blockIdx.x - q.e_blk0) * blockDim.x + threadIdx.x;
+  dir_body(q.n * B, B, q.beta, q.zi, q.pi, i);
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void grp_update_kernel(const RunSeg<T> *__restrict__ segs) {
+  const RunSeg<T> &q = segs[blockIdx.x / kRedBlocks];
+  update_norm_body<T>(q.n, B, q.alpha, q.pi, q.qi, q.xi, q.ri, q.red_part, q.a, q.coefP, q.u,
+                      blockIdx.x % kRedBlocks, kRedBlocks);
+}
+
+// q = A p of every segment: the row-per-wavefront FP32 operator where the
+// segment has its row-padded copy (32 slots), the chunked operator otherwise
+template <typename T, int B>
+__global__ __launch_bounds__(256) void grp_op_kernel(const RunSeg<T> *__restrict__ segs, int G) {
+  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, true)];
+  const int local = xcd_map((int)blockIdx.x - q.op_blk0, q.op_nblk);
+  if constexpr (std::is_same<T, float>::value && B == 32) {
+    if (q.pc.rp) {
+      const int64_t r = __builtin_amdgcn_readfirstlane(local * (kRowWg / 64) + (int)(threadIdx.x >> 6));
+      if (r >= q.n) return;
+      row_apply<0>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
+                   reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
+      return;
+    }
+  }
+  op_body<T, B, 0>(q.a, q.coefA, q.pi, nullptr, q.qi, local);
+}
+
+template <typename T, int B>
+struct GroupRun : GroupItem {
+  Model &m;
+  std::vector<double> freqs;
+  std::complex<double> Z;
+  elp_solve_opts o;
+  double *X, *relres;
+  int32_t *iters;
+  std::unique_ptr<Solver<T, B>> S;
+  std::unique_ptr<SlotRun<T, B>> R;
+  GroupRun(Model &mm, const double *f, int64_t nf, std::complex<double> z, const elp_solve_opts &oo, double *x,
+           double *rel, int32_t *it)
+      : m(mm), freqs(f, f + nf), Z(z), o(oo), X(x), relres(rel), iters(it) {}
+  // finish: the returned results and the solver's memory released
+  ~GroupRun() override {
+    R.reset();
+    S.reset();
+  }
+};
+
+template <typename T, int B>
+struct GroupDriverT : GroupDriver {
+  Ctx *ctx;
+  hipStream_t s;
+  std::vector<std::unique_ptr<GroupRun<T, B>>> runs;
+  DArr<RunSeg<T>> d_segs;
+  std::vector<RunSeg<T>> h_segs;
+  bool dirty = true;
+  int op_blocks = 0, e_blocks = 0;
+  HipEvent ev0, ev1;
+  GroupDriverT(Ctx *c, hipStream_t st) : ctx(c), s(st) {}
+
+  void add(std::unique_ptr<GroupItem> it) override {
+    std::unique_ptr<GroupRun<T, B>> g(static_cast<GroupRun<T, B> *>(it.release()));
+    g->S->s = s;  // every request of the group runs on the group's stream
+    g->R.reset(new SlotRun<T, B>(*g->S, (int64_t)g->freqs.size(), g->freqs.data(), g->Z, g->X, g->relres,
+                                 g->iters));
+    g->R->start();
+    runs.push_back(std::move(g));
+    dirty = true;
+  }
+  bool empty() const override { return runs.empty(); }
+
+  void rebuild() {
+    h_segs.assign(runs.size(), RunSeg<T>());
+    int ob = 0, eb = 0;
+    for (size_t k = 0; k < runs.size(); k++) {
+      Solver<T, B> &S = *runs[k]->S;
+      const LevelData &L = *S.st.levels[0];
+      RunSeg<T> &q = h_segs[k];
+      q.n = S.m.n;
+      q.a = L.template args<T>();
+      if constexpr (std::is_same<T, float>::value && B == 32) q.pc = L.pad_rp.p ? L.padded() : PadCsr{};
+      q.coefA = S.coefA.p;
+      q.coefP = S.coefP.p;
+      q.ri = S.ri.p;
+      q.zi = S.zi.p;
+      q.pi = S.pi.p;
+      q.qi = S.qi.p;
+      q.xi = S.xi.p;
+      q.u = S.fused_box() ? S.lw[0].x.p : nullptr;
+      q.red_part = S.red_part.p;
+      q.red_out = S.red_out.p;
+      q.rho = S.rho.p;
+      q.rho_new = S.rho_new.p;
+      q.mu = S.mu.p;
+      q.alpha = S.alpha.p;
+      q.beta = S.beta.p;
+      q.active = S.active.p;
+      const int rows_per_blk = (q.pc.rp ? kRowWg / 64 : OpShape<B>::RPB);
+      q.op_blk0 = ob;
+      q.op_nblk = xcd_grid(nblk(q.n, rows_per_blk));
+      ob += q.op_nblk;
+      q.e_blk0 = eb;
+      eb += nblk(q.n * B, 256);
+    }
+    op_blocks = ob;
+    e_blocks = eb;
+    if ((int64_t)d_segs.n < (int64_t)h_segs.size()) d_segs.alloc(ctx, std::max<size_t>(h_segs.size(), 16));
+    ELP_HIP(hipMemcpyAsync(d_segs.p, h_segs.data(), sizeof(RunSeg<T>) * h_segs.size(), hipMemcpyHostToDevice, s));
+    dirty = false;
+  }
+
+  void step(std::vector<std::unique_ptr<GroupItem>> &finished) override {
+    // retire finished runs
+    for (size_t k = 0; k < runs.size();) {
+      if (!runs[k]->R->iterating()) {
+        runs[k]->R.reset();
+        finished.push_back(std::move(runs[k]));
+        runs.erase(runs.begin() + k);
+        dirty = true;
+      } else {
+        k++;
+      }
+    }
+    if (runs.empty()) return;
+    if (dirty) rebuild();
+    using V = typename VT<T>::t;
+    const int G = (int)runs.size();
+    const RunSeg<T> *sg = d_segs.p;
+    for (auto &g : runs) g->S->precond(g->S->ri.p, g->S->zi.p);
+    hipLaunchKernelGGL((grp_dot_kernel<T, B, 0>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
+    hipLaunchKernelGGL((grp_final_kernel<T, B, 0>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
+    hipLaunchKernelGGL((grp_beta_kernel<T, B>), dim3(G), dim3(64), 0, s, sg);
+    hipLaunchKernelGGL((grp_dir_kernel<T, B>), dim3(e_blocks), dim3(256), 0, s, sg, G);
+    ELP_HIP(hipGetLastError());
+    for (auto &g : runs) g->R->restart_flags();
+    ELP_HIP(hipEventRecord(ev0.e, s));
+    hipLaunchKernelGGL((grp_op_kernel<T, B>), dim3(op_blocks), dim3(256), 0, s, sg, G);
+    ELP_HIP(hipEventRecord(ev1.e, s));
+    hipLaunchKernelGGL((grp_dot_kernel<T, B, 1>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
+    hipLaunchKernelGGL((grp_final_kernel<T, B, 1>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
+    hipLaunchKernelGGL((grp_alpha_kernel<T, B>), dim3(G), dim3(64), 0, s, sg);
+    hipLaunchKernelGGL((grp_update_kernel<T, B>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
+    hipLaunchKernelGGL((grp_final_kernel<T, B, 2>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
+    ELP_HIP(hipGetLastError());
+    for (auto &g : runs) {
+      Solver<T, B> &S = *g->S;
+      S.u_of = S.fused_box() ? S.ri.p : nullptr;
+      ELP_HIP(hipMemcpyAsync(S.h_red.p, S.red_out.p, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
+    }
+    ELP_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    ELP_HIP(hipEventElapsedTime(&ms, ev0.e, ev1.e));
+    for (auto &g : runs) {
+      g->S->spmv_ms += ms;
+      g->S->spmv_calls++;
+      g->R->after_norms();
+    }
+  }
+};
+
+static int slot_index(int B) { return B == 4 ? 0 : B == 8 ? 1 : B == 16 ? 2 : B == 32 ? 3 : -1; }
+
+template <typename T, int B>
+static std::unique_ptr<GroupItem> make_group_run(Model &m, Model::SolverState &st, int64_t nf, const double *freqs,
+                                                 std::complex<double> Z, const elp_solve_opts &o, double *X,
+                                                 double *relres, int32_t *iters, hipStream_t s) {
+  auto g = std::make_unique<GroupRun<T, B>>(m, freqs, nf, Z, o, X, relres, iters);
+  g->S.reset(new Solver<T, B>(m, st, g->o, s));
+  return g;
+}
+
+template <typename T>
+static std::unique_ptr<GroupItem> make_group_run_b(int B, Model &m, Model::SolverState &st, int64_t nf,
+                                                   const double *freqs, std::complex<double> Z,
+                                                   const elp_solve_opts &o, double *X, double *relres,
+                                                   int32_t *iters, hipStream_t s) {
+  switch (B) {
+    case 4: return make_group_run<T, 4>(m, st, nf, freqs, Z, o, X, relres, iters, s);
+    case 8: return make_group_run<T, 8>(m, st, nf, freqs, Z, o, X, relres, iters, s);
+    case 16: return make_group_run<T, 16>(m, st, nf, freqs, Z, o, X, relres, iters, s);
+    default: return make_group_run<T, 32>(m, st, nf, freqs, Z, o, X, relres, iters, s);
+  }
+}
+
+std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *freqs, std::complex<double> Z,
+                                         const elp_solve_opts &o, double *X, double *relres, int32_t *iters,
+                                         hipStream_t setup, int *key) {
+  ELP_REQUIRE(o.method == ELP_METHOD_COCG, ELP_EINVAL, "method must be ELP_METHOD_COCG");
+  ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
+  for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
+  const int bi = slot_index(o.block);
+  if (o.precision == ELP_PREC_FP64 || bi < 0) {  // one frequency at a time: runs alone
+    *key = -1;
+    return nullptr;
+  }
+  Model::SolverState &st = solver_state(m, o, Z, setup);
+  const bool dbl = st.coupled;  // coupled models iterate in FP64 (see sweep)
+  *key = (dbl ? 4 : 0) + bi;
+  return dbl ? make_group_run_b<double>(o.block, m, st, nf, freqs, Z, o, X, relres, iters, setup)
+             : make_group_run_b<float>(o.block, m, st, nf, freqs, Z, o, X, relres, iters, setup);
+}
+
+std::unique_ptr<GroupDriver> group_driver(int key, Ctx *ctx, hipStream_t s) {
+  switch (key) {
+    case 0: return std::make_unique<GroupDriverT<float, 4>>(ctx, s);
+    case 1: return std::make_unique<GroupDriverT<float, 8>>(ctx, s);
+    case 2: return std::make_unique<GroupDriverT<float, 16>>(ctx, s);
+    case 3: return std::make_unique<GroupDriverT<float, 32>>(ctx, s);
+    case 4: return std::make_unique<GroupDriverT<double, 4>>(ctx, s);
+    case 5: return std::make_unique<GroupDriverT<double, 8>>(ctx, s);
+    case 6: return std::make_unique<GroupDriverT<double, 16>>(ctx, s);
+    case 7: return std::make_unique<GroupDriverT<double, 32>>(ctx, s);
+    default: throw Error(ELP_EINVAL, "unknown group");
+  }
+}
+
 template <typename T, int B>
 static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double *freqs, std::complex<double> Z,
                       const elp_solve_opts &o, double *X, double *relres, int32_t *iters, elp_sweep_stats *stats,
