@@ -346,6 +346,8 @@ int elp_server_submit(elp_server *sv, elp_model *m, int64_t n_freq, const double
   {
     std::lock_guard<std::mutex> lk(sv->s.mu);
     ELP_REQUIRE(!sv->s.stop, ELP_ESTATE, "server stopped");
+    for (auto &kv : sv->s.jobs)  // a model belongs to one request at a time
+      ELP_REQUIRE(kv.second->model != m, ELP_EINVAL, "model already submitted");
     j->id = sv->s.next_id++;
     sv->s.jobs[j->id] = j;
     sv->s.pending.push_back(j);

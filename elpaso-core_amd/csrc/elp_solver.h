@@ -38,7 +38,8 @@ struct GroupDriver {
   virtual void step(std::vector<std::unique_ptr<GroupItem>> &finished) = 0;
 };
 // Set-up of a request on `setup` (solver state, work space).  *key: the
-// group it runs in (-1: it runs alone through sweep(), nullptr returned).
+// group it runs in, by precision, slot count and size class (-1: it runs
+// alone through sweep(), nullptr returned).
 std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *freqs, std::complex<double> Z,
                                          const elp_solve_opts &o, double *X, double *relres, int32_t *iters,
                                          hipStream_t setup, int *key);

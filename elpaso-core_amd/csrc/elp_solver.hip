@@ -2767,13 +2767,20 @@ struct GroupRun : GroupItem {
   int32_t *iters;
   std::unique_ptr<Solver<T, B>> S;
   std::unique_ptr<SlotRun<T, B>> R;
+  // the request's own stream (preconditioner, refinement), joined to the
+  // group's stream through `ready` before each fused launch
+  hipStream_t own = nullptr;
+  std::unique_ptr<HipEvent> ready;
   GroupRun(Model &mm, const double *f, int64_t nf, std::complex<double> z, const elp_solve_opts &oo, double *x,
            double *rel, int32_t *it)
       : m(mm), freqs(f, f + nf), Z(z), o(oo), X(x), relres(rel), iters(it) {}
   // finish: the returned results and the solver's memory released
   ~GroupRun() override {
+    if (own) (void)hipStreamSynchronize(own);
     R.reset();
     S.reset();
+    ready.reset();
+    if (own) (void)hipStreamDestroy(own);
   }
 };
 
@@ -2791,7 +2798,9 @@ struct GroupDriverT : GroupDriver {
 
   void add(std::unique_ptr<GroupItem> it) override {
     std::unique_ptr<GroupRun<T, B>> g(static_cast<GroupRun<T, B> *>(it.release()));
-    g->S->s = s;  // every request of the group runs on the group's stream
+    ELP_HIP(hipStreamCreateWithFlags(&g->own, hipStreamNonBlocking));
+    g->ready.reset(new HipEvent);
+    g->S->s = g->own;
     g->R.reset(new SlotRun<T, B>(*g->S, (int64_t)g->freqs.size(), g->freqs.data(), g->Z, g->X, g->relres,
                                  g->iters));
     g->R->start();
@@ -2854,16 +2863,26 @@ struct GroupDriverT : GroupDriver {
     }
     if (runs.empty()) return;
     if (dirty) rebuild();
-    using V = typename VT<T>::t;
     const int G = (int)runs.size();
     const RunSeg<T> *sg = d_segs.p;
-    for (auto &g : runs) g->S->precond(g->S->ri.p, g->S->zi.p);
+    // the requests' preconditioners overlap on their own streams
+    for (auto &g : runs) {
+      g->S->precond(g->S->ri.p, g->S->zi.p);
+      ELP_HIP(hipEventRecord(g->ready->e, g->own));
+      ELP_HIP(hipStreamWaitEvent(s, g->ready->e, 0));
+    }
     hipLaunchKernelGGL((grp_dot_kernel<T, B, 0>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
     hipLaunchKernelGGL((grp_final_kernel<T, B, 0>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
     hipLaunchKernelGGL((grp_beta_kernel<T, B>), dim3(G), dim3(64), 0, s, sg);
     hipLaunchKernelGGL((grp_dir_kernel<T, B>), dim3(e_blocks), dim3(256), 0, s, sg, G);
     ELP_HIP(hipGetLastError());
-    for (auto &g : runs) g->R->restart_flags();
+    // restart flags go up on the group's stream (the requests' streams are
+    // idle until the next preconditioner)
+    for (auto &g : runs) {
+      g->R->s = s;
+      g->R->restart_flags();
+      g->R->s = g->own;
+    }
     ELP_HIP(hipEventRecord(ev0.e, s));
     hipLaunchKernelGGL((grp_op_kernel<T, B>), dim3(op_blocks), dim3(256), 0, s, sg, G);
     ELP_HIP(hipEventRecord(ev1.e, s));
@@ -2926,13 +2945,16 @@ std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *fre
   }
   Model::SolverState &st = solver_state(m, o, Z, setup);
   const bool dbl = st.coupled;  // coupled models iterate in FP64 (see sweep)
-  *key = (dbl ? 4 : 0) + bi;
+  // requests batch with others of their size class, so that a small
+  // request's step is not the length of a large one's
+  const int size_class = m.n <= 50000 ? 0 : m.n <= 500000 ? 1 : 2;
+  *key = (dbl ? 4 : 0) + bi + 8 * size_class;
   return dbl ? make_group_run_b<double>(o.block, m, st, nf, freqs, Z, o, X, relres, iters, setup)
              : make_group_run_b<float>(o.block, m, st, nf, freqs, Z, o, X, relres, iters, setup);
 }
 
 std::unique_ptr<GroupDriver> group_driver(int key, Ctx *ctx, hipStream_t s) {
-  switch (key) {
+  switch (key % 8) {
     case 0: return std::make_unique<GroupDriverT<float, 4>>(ctx, s);
     case 1: return std::make_unique<GroupDriverT<float, 8>>(ctx, s);
     case 2: return std::make_unique<GroupDriverT<float, 16>>(ctx, s);
@@ -2974,8 +2996,19 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
 }
 
 int64_t solver_bytes_estimate(const Model &m, int block, int precision) {
-  const int64_t vb = precision == ELP_PREC_FP64 ? 16 : 8;
+  // coupled models (structure rows beside a Hex8 box) iterate in FP64 and
+  // keep two factorizations of the structure block (the window in use and
+  // the next): about 7 ns m complex FP64 values each with their work space,
+  // m the half-bandwidth of the structure block, about 3 sqrt(ns) for plates
+  const bool cpl = coupled_possible(m);
+  const int64_t vb = (precision == ELP_PREC_FP64 || cpl) ? 16 : 8;
   int64_t b = m.nnz * 8 + m.n * 24;                    // FP32 shadows, l1 sums
+  if (cpl) {
+    int64_t ns = 0;
+    for (auto v : m.row_field) ns += v == 0;
+    const double mb = std::min<double>((double)ns, 3.0 * std::sqrt((double)ns));
+    b += (int64_t)(2 * 7.0 * (double)ns * mb * 16.0);
+  }
   b += m.nnz * 18;                                     // permuted columns, row-padded operator copy
   b += m.n * 512;                                      // box products A P, padded P rows, lattice
   b += (int64_t)m.n * block * (5 * vb * 2 + 4 * 16);   // level-0 + Krylov + FP64 outer vectors
