@@ -71,8 +71,9 @@ struct Server {
   std::unordered_map<int64_t, std::shared_ptr<Job>> jobs;
   std::map<int, std::unique_ptr<Group>> groups;
   int64_t next_id = 1, in_use = 0, admitted = 0;
-  int running = 0;  // admitted and not finished
-  int waiters = 0;  // callers inside elp_server_wait
+  int running = 0;     // admitted and not finished
+  int setting_up = 0;  // taken by the set-up thread, not yet in a group or ended
+  int waiters = 0;     // callers inside elp_server_wait
   bool stop = false;
   std::thread scheduler, setup;
 
@@ -92,6 +93,15 @@ struct Server {
     cv_done.notify_all();
     cv_sched.notify_all();
   }
+
+  // (under mu) a request left the set-up thread; groups waiting to shut
+  // down look again
+  void setup_done() {
+    setting_up--;
+    cv_group.notify_all();
+  }
+  // (under mu) no request can reach a group any more
+  bool setup_drained() const { return stop && to_setup.empty() && setting_up == 0; }
 
   static int unconverged(const Job &j) {
     int bad = 0;
@@ -129,6 +139,7 @@ struct Server {
         if (to_setup.empty()) break;
         j = to_setup.front();
         to_setup.pop_front();
+        setting_up++;
       }
       Model &m = *j->model->m;
       const int64_t nf = (int64_t)j->freqs.size();
@@ -152,13 +163,16 @@ struct Server {
       } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(mu);
         complete(*j, e.code, e.what());
+        setup_done();
         continue;
       } catch (const std::exception &e) {
         std::lock_guard<std::mutex> lk(mu);
         complete(*j, ELP_EINVAL, e.what());
+        setup_done();
         continue;
       }
       std::lock_guard<std::mutex> lk(mu);
+      setup_done();
       if (!item) {
         const int bad = unconverged(*j);
         complete(*j, bad ? ELP_ENOCONV : 0,
@@ -197,8 +211,10 @@ void Group::loop() {
   while (true) {
     {
       std::unique_lock<std::mutex> lk(S.mu);
-      S.cv_group.wait(lk, [&] { return S.stop || !inbox.empty() || (drv && !drv->empty()); });
-      if (S.stop && inbox.empty() && (!drv || drv->empty())) break;
+      // on shutdown a group ends only once the set-up thread can hand it no
+      // more requests (an admitted request may still be in set-up)
+      S.cv_group.wait(lk, [&] { return S.setup_drained() || !inbox.empty() || (drv && !drv->empty()); });
+      if (S.setup_drained() && inbox.empty() && (!drv || drv->empty())) break;
       while (!inbox.empty()) {
         fresh.push_back(std::move(inbox.front()));
         inbox.pop_front();

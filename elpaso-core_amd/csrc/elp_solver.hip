@@ -282,6 +282,21 @@ __device__ inline typename VT<T>::t jacobi0_value(const OpArgs<T> &a, const OpCo
   return t;
 }
 
+// 1 / (l1 row bound of A(w_f)) on free rows, 0 on Dirichlet rows: the
+// weights of the Jacobi-scaled residual norm ||D^-1/2 r||.  The rows of one
+// field share a scale (fluid rows divided by rho w^2, structure rows of order
+// E h), and fields can differ by 15 orders: a plain norm of a coupled
+// residual sees the structure rows only.
+template <typename T>
+__device__ inline double row_weight(const OpArgs<T> &a, const OpCoef *__restrict__ coef, int64_t r, int f) {
+  const uint8_t fld = a.field[r];
+  if (fld == 2) return 0.0;
+  const OpCoef &cf = coef[f];
+  const bool fl = fld == 1;
+  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+  return d > 0 ? 1.0 / d : 0.0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void jacobi0_kernel(OpArgs<T> a, int B, const OpCoef *__restrict__ coef,
                                                       const typename VT<T>::t *__restrict__ b,
@@ -616,16 +631,44 @@ __global__ __launch_bounds__(256) void dot_kernel(int64_t n, int B, const V *__r
   dot_body<V>(n, B, a, b, mode, partial, blockIdx.x, gridDim.x);
 }
 
-// x += alpha p ; r -= alpha q ; partial sums of |r|^2 (dot_kernel's layout);
-// with u != nullptr also u = D^-1 r, the first l1-Jacobi sweep of the
-// preconditioner (jacobi0_kernel's values)
+// partials of ||v||^2 (x, the same sums in the same order as dot_kernel's
+// mode 1) and of the Jacobi-scaled ||D^-1/2 v||^2 (y) per frequency, D the
+// l1 row bound of A(w_f) with the operator's coefficients coefa
+template <typename T, typename V>
+__device__ inline void wnorm_body(int64_t n, int B, const V *__restrict__ v, const OpArgs<T> &la,
+                                  const OpCoef *__restrict__ coefa, double2 *__restrict__ partial, int blk,
+                                  int nblocks) {
+  const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
+  const int64_t chunk = (n + nblocks - 1) / nblocks;
+  const int64_t r0 = blk * chunk, r1 = min(n, r0 + chunk);
+  double2 acc = {0, 0};
+  for (int64_t r = r0 + rl; r < r1; r += R) {
+    const V a = v[r * B + f];
+    const double q = (double)a.x * a.x + (double)a.y * a.y;
+    acc.x += q;
+    acc.y += q * row_weight<T>(la, coefa, r, f);
+  }
+  red_block_store(acc, B, partial, blk);
+}
+
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void wnorm_kernel(int64_t n, int B, const V *__restrict__ v, OpArgs<T> la,
+                                                    const OpCoef *__restrict__ coefa, double2 *__restrict__ partial) {
+  wnorm_body<T, V>(n, B, v, la, coefa, partial, blockIdx.x, gridDim.x);
+}
+
+// x += alpha p ; r -= alpha q ; partial sums of |r|^2 and of the
+// Jacobi-scaled |r|^2 (wnorm_kernel's layout); with u != nullptr also
+// u = D^-1 r, the first l1-Jacobi sweep of the preconditioner
+// (jacobi0_kernel's values)
 template <typename T>
 __device__ inline void update_norm_body(int64_t n, int B, const double2 *__restrict__ alpha,
                                         const typename VT<T>::t *__restrict__ p,
                                         const typename VT<T>::t *__restrict__ q, typename VT<T>::t *__restrict__ x,
                                         typename VT<T>::t *__restrict__ r, double2 *__restrict__ partial,
                                         const OpArgs<T> &la, const OpCoef *__restrict__ coefp,
-                                        typename VT<T>::t *__restrict__ u, int blk, int nblocks) {
+                                        const OpCoef *__restrict__ coefa, typename VT<T>::t *__restrict__ u, int blk,
+                                        int nblocks) {
   using V = typename VT<T>::t;
   const int t = threadIdx.x, f = t % B, rl = t / B, R = 256 / B;
   const int64_t chunk = (n + nblocks - 1) / nblocks;
@@ -647,7 +690,9 @@ __device__ inline void update_norm_body(int64_t n, int B, const double2 *__restr
       r[i] = rv;
     }
     if (u) u[i] = jacobi0_value<T>(la, coefp, row, f, rv);
-    acc.x += (double)rv.x * rv.x + (double)rv.y * rv.y;
+    const double q2 = (double)rv.x * rv.x + (double)rv.y * rv.y;
+    acc.x += q2;
+    acc.y += q2 * row_weight<T>(la, coefa, row, f);
   }
   red_block_store(acc, B, partial, blk);
 }
@@ -660,8 +705,9 @@ __global__ __launch_bounds__(256) void cocg_update_norm_kernel(int64_t n, int B,
                                                                typename VT<T>::t *__restrict__ r,
                                                                double2 *__restrict__ partial, OpArgs<T> la,
                                                                const OpCoef *__restrict__ coefp,
+                                                               const OpCoef *__restrict__ coefa,
                                                                typename VT<T>::t *__restrict__ u) {
-  update_norm_body<T>(n, B, alpha, p, q, x, r, partial, la, coefp, u, blockIdx.x, gridDim.x);
+  update_norm_body<T>(n, B, alpha, p, q, x, r, partial, la, coefp, coefa, u, blockIdx.x, gridDim.x);
 }
 
 __device__ inline void dot_final_body(int nblk, int B, const double2 *__restrict__ partial,
@@ -955,6 +1001,15 @@ __global__ void extra_shadow_kernel(int64_t n, const int32_t *__restrict__ xrp, 
 
 static int nblk(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
+// relative residual of an inner solve: the larger of the plain and the
+// Jacobi-scaled reductions, from ||r||^2, ||D^-1/2 r||^2 (h) and the norms
+// of its right-hand side (n0)
+static double inner_rel(double2 h, double2 n0) {
+  const double a = n0.x > 0 ? std::sqrt(h.x) / n0.x : 0.0;
+  const double b = n0.y > 0 ? std::sqrt(h.y) / n0.y : 0.0;
+  return std::max(a, b);
+}
+
 // ---------------------------------------------------------------------------
 // solver state: levels of the hierarchy
 // ---------------------------------------------------------------------------
@@ -1034,6 +1089,8 @@ struct elp::Model::SolverState {
   // the structure block
   bool coupled = false, cpl_requested = false;
   std::unique_ptr<elp::BcrPlan, void (*)(elp::BcrPlan *)> bcr{nullptr, nullptr};
+  // models with structure and fluid rows iterate in FP64 (see sweep)
+  bool fp64_inner = false;
 };
 
 namespace elp {
@@ -1492,8 +1549,8 @@ static bool mg_possible(const Model &m) {
 }
 
 // a Hex8 fluid box (the box hint) coupled to structure rows
-static bool coupled_possible(const Model &m) {
-  if (!m.box.set || m.box.order != 1) return false;
+// structure and fluid rows in one model
+static bool two_fields(const Model &m) {
   bool has_s = false, has_f = false;
   for (auto v : m.row_field) {
     has_s |= v == 0;
@@ -1501,6 +1558,8 @@ static bool coupled_possible(const Model &m) {
   }
   return has_s && has_f;
 }
+
+static bool coupled_possible(const Model &m) { return m.box.set && m.box.order == 1 && two_fields(m); }
 
 static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::complex<double> Z, hipStream_t s) {
   const bool want_mg = o.precond == ELP_PC_MG && mg_possible(m);
@@ -1514,6 +1573,7 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   st->cpl_requested = want_cpl;
   st->mg_levels = o.mg_levels;
   st->mg_min_nodes = o.mg_min_nodes;
+  st->fp64_inner = two_fields(m);
   auto L0 = std::make_unique<LevelData>();
   L0->model = &m;
   prepare_level(*L0, s);
@@ -1698,6 +1758,19 @@ struct Solver {
     hipLaunchKernelGGL(dot_kernel<VV>, dim3(kRedBlocks), dim3(256), 0, s, n, B, a, b, mode, red_part.p);
     final_reduce(dev_out);
   }
+  // ||v||^2 and ||D^-1/2 v||^2 per frequency into dev_out (.x, .y): v in
+  // the inner solve's row order, or (wreduce64) an FP64 vector in the
+  // model's row order
+  void wreduce(const V *v, double2 *dev_out) {
+    hipLaunchKernelGGL((wnorm_kernel<T, V>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, v,
+                       st.levels[0]->template args<T>(), coefA.p, red_part.p);
+    final_reduce(dev_out);
+  }
+  void wreduce64(const double2 *v, double2 *dev_out) {
+    hipLaunchKernelGGL((wnorm_kernel<double, double2>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, v,
+                       st.levels[0]->template args<double>(), coefA.p, red_part.p);
+    final_reduce(dev_out);
+  }
   void final_reduce(double2 *dev_out) {
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kRedFinalThreads), 0, s, kRedBlocks, B, red_part.p, dev_out);
     ELP_HIP(hipGetLastError());
@@ -1707,7 +1780,7 @@ struct Solver {
   // level-0 work vector x (box_cycle then skips it)
   void update_norm(bool track_u = false) {
     hipLaunchKernelGGL((cocg_update_norm_kernel<T>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, alpha.p, pi.p, qi.p,
-                       xi.p, ri.p, red_part.p, st.levels[0]->template args<T>(), coefP.p,
+                       xi.p, ri.p, red_part.p, st.levels[0]->template args<T>(), coefP.p, coefA.p,
                        track_u ? lw[0].x.p : nullptr);
     final_reduce(red_out.p);
     u_of = track_u ? ri.p : nullptr;
@@ -1861,16 +1934,17 @@ struct Solver {
   }
 
   // COCG on A (level 0, precision T) for the block: solves A d = rhs (rhs in
-  // ri on entry) from d = 0 until ||r_f|| <= tol * ||rhs_f|| for all active f.
-  // iters[f] accumulates iterations while f is active.
+  // ri on entry) from d = 0 until both ||r_f|| <= tol ||rhs_f|| and
+  // ||D^-1/2 r_f|| <= tol ||D^-1/2 rhs_f|| for all active f.  iters[f]
+  // accumulates iterations while f is active.
   void cocg(const std::vector<int> &act0, double tol, int maxit, std::vector<int> &iters) {
     const int64_t nt = m.n * B;
     std::vector<int> act = act0;
     ELP_HIP(hipMemsetAsync(xi.p, 0, nt * sizeof(V), s));
-    reduce(m.n, ri.p, ri.p, 1, red_out.p);
+    wreduce(ri.p, red_out.p);
     fetch(red_out.p);
-    std::vector<double> nr0(B);
-    for (int f = 0; f < B; f++) nr0[f] = std::sqrt(h_red[f].x);
+    std::vector<double2> nr0(B);
+    for (int f = 0; f < B; f++) nr0[f] = {std::sqrt(h_red[f].x), std::sqrt(h_red[f].y)};
     ELP_HIP(hipMemcpyAsync(active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     precond(ri.p, zi.p);
     ELP_HIP(hipMemcpyAsync(pi.p, zi.p, nt * sizeof(V), hipMemcpyDeviceToDevice, s));
@@ -1894,8 +1968,8 @@ struct Solver {
       for (int f = 0; f < B; f++) {
         if (!act[f]) continue;
         iters[f]++;
-        const double nr = std::sqrt(h_red[f].x);
-        if (!(nr > tol * nr0[f]) || !std::isfinite(nr) || nr > 1e6 * nr0[f]) {
+        const double nr = inner_rel(h_red[f], nr0[f]);
+        if (!(nr > tol) || !std::isfinite(nr) || nr > 1e6) {
           act[f] = 0;
           changed = true;
         } else {
@@ -2075,11 +2149,15 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   ELP_HIP(hipMemsetAsync(x64.p, 0, nt * sizeof(double2), s));
   // the FP64 operator lives in the FP64 solver when T = float; otherwise S itself
   auto op64 = [&](const double2 *x, const double2 *b, double2 *y) { S.template op<1, double>(0, S.coefA.p, x, b, y); };
-  // residuals relative to ||F|| (= ||b|| without prescribed values)
-  S.reduce(m.n, b64.p, b64.p, 1, S.red_out.p);
+  // residuals relative to ||F|| (= ||b|| without prescribed values), plain
+  // and Jacobi-scaled (see SlotRun::Slot)
+  S.wreduce64(b64.p, S.red_out.p);
   S.fetch(S.red_out.p);
-  std::vector<double> nb2(B);
-  for (int f = 0; f < B; f++) nb2[f] = std::sqrt(S.h_red[f].x);
+  std::vector<double> nb2(B), nrefs(B);
+  for (int f = 0; f < B; f++) {
+    nb2[f] = std::sqrt(S.h_red[f].x);
+    nrefs[f] = std::sqrt(S.h_red[f].y);
+  }
   std::vector<double> nref(nb2), nrep(nb2);
   Lift lift;
   build_lift(m, lift);
@@ -2095,7 +2173,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     ELP_HIP(hipGetLastError());
     // stop against the free rows' lifted right-hand side, report against the
     // whole lifted right-hand side (see Slot::ref in solve_slots)
-    S.reduce(m.n, r0.p, r0.p, 1, S.red_out.p);
+    S.wreduce64(r0.p, S.red_out.p);
     S.fetch(S.red_out.p);
     double gnorm2 = 0;
     for (auto &v : m.dir_vals) gnorm2 += std::norm(v);
@@ -2103,10 +2181,19 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
       const double w0 = std::sqrt(S.h_red[f].x);
       if (w0 > 0) nref[f] = w0;
       nrep[f] = std::sqrt(w0 * w0 + gnorm2);
+      nrefs[f] = std::sqrt(S.h_red[f].y);
     }
   }
   std::vector<int> it(B, 0);
-  std::vector<double> rel(B, 1.0);
+  // rel = max(relp, rels): the plain and the Jacobi-scaled relative residual
+  std::vector<double> rel(B, 1.0), relp(B, 1.0), rels(B, 1.0);
+  auto rel_from_red = [&]() {
+    for (int f = 0; f < B; f++) {
+      relp[f] = nref[f] > 0 ? std::sqrt(S.h_red[f].x) / nref[f] : 0.0;
+      rels[f] = nrefs[f] > 0 ? std::sqrt(S.h_red[f].y) / nrefs[f] : 0.0;
+      rel[f] = std::max(relp[f], rels[f]);
+    }
+  };
   DArr<double> scl;
   scl.alloc(m.ctx, B);
   const int max_outer = 30;
@@ -2118,9 +2205,9 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   std::vector<int> stalled(B, 0), lastact(B, 0);
   for (int outer = 0; outer <= max_outer; outer++) {
     op64(x64.p, b64.p, r64.p);
-    S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+    S.wreduce64(r64.p, S.red_out.p);
     S.fetch(S.red_out.p);
-    for (int f = 0; f < B; f++) rel[f] = nref[f] > 0 ? std::sqrt(S.h_red[f].x) / nref[f] : 0.0;
+    rel_from_red();
     // a refinement step that did not reduce the residual is taken back and
     // the frequency is not refined further
     std::vector<double> undo(B, 0.0);
@@ -2140,16 +2227,16 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
       hipLaunchKernelGGL((add_scaled_kernel<typename VT<T>::t>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B,
                          scl.p, S.xi.p, x64.p, iperm);
       op64(x64.p, b64.p, r64.p);
-      S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+      S.wreduce64(r64.p, S.red_out.p);
       S.fetch(S.red_out.p);
-      for (int f = 0; f < B; f++) rel[f] = nref[f] > 0 ? std::sqrt(S.h_red[f].x) / nref[f] : 0.0;
+      rel_from_red();
     }
     std::vector<int> act(B, 0);
     std::vector<double> inv(B, 0.0), wgt(B, 0.0);
     bool any = false;
     for (int f = 0; f < B; f++) {
-      const double nr = rel[f] * nref[f];
-      if (f < nb && !stalled[f] && rel[f] > o.rtol && std::isfinite(rel[f])) {
+      const double nr = relp[f] * nref[f];  // ||r64||: the inner right-hand side is r64 / ||r64||
+      if (f < nb && !stalled[f] && rel[f] > o.rtol && std::isfinite(rel[f]) && nr > 0) {
         act[f] = 1;
         any = true;
         inv[f] = 1.0 / nr;
@@ -2185,7 +2272,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
     }
   }
   for (int f = 0; f < nb; f++) {
-    if (relres) relres[f] = nrep[f] > 0 ? rel[f] * nref[f] / nrep[f] : rel[f];
+    if (relres) relres[f] = std::max(nrep[f] > 0 ? relp[f] * nref[f] / nrep[f] : relp[f], rels[f]);
     if (iters) iters[f] = it[f];
   }
   if (X) {
@@ -2261,6 +2348,15 @@ struct SlotRun {
     // (fluid rows divided by rho w^2, structure rows of order E h) can differ
     // by many orders, so neither part may set the scale of the other.
     double ref = 0, rep = 0;
+    // The Jacobi-scaled residual ||D^-1/2 r64|| (D the l1 row bound of
+    // A(w)) is held to rtol as well, relative to ||D^-1/2 b|| of the free
+    // rows (refs): within one field the two measures agree, but fields can
+    // differ in scale by 15 orders (a coupled model's fluid rows against its
+    // structure rows), and the plain norm then sees one field only.  prev is
+    // the larger of the two relative residuals (relp, rels); n0 = the plain
+    // and scaled norms of the inner right-hand side r64 / w.
+    double refs = 0, relp = 1, rels = 1;
+    double2 n0 = {1, 0};
   };
   std::vector<Slot> sl;
   int64_t next = 0;
@@ -2384,6 +2480,18 @@ struct SlotRun {
     const double finish_at = 0.5 * o.rtol / rel;
     return finish_at >= 1e-4 ? std::min(o.inner_rtol, finish_at) : o.inner_rtol;
   }
+  // the reported residual: the plain one relative to ||b|| (F with the
+  // prescribed values) or the scaled one, whichever is larger
+  static double reported(const Slot &q, double rp, double rs) { return std::max(rp * q.ref / q.rep, rs); }
+  // slot q restarts its inner solve on r64 with these relative residuals
+  void new_rhs(Slot &q, double rp, double rs) const {
+    q.relp = rp;
+    q.rels = rs;
+    q.prev = std::max(rp, rs);
+    q.tol = inner_tol(q.prev);
+    q.w = rp * q.ref;
+    q.n0 = {1.0, q.w > 0 ? rs * q.refs / q.w : 0.0};
+  }
   // assign the next frequency to slot k (stream idle)
   bool assign(int k) {
     if (next >= nf) {
@@ -2432,19 +2540,29 @@ struct SlotRun {
     launch_lift(m, lift, B, d_fresh.p, S.coefA.p, F.p, b64.p, s);
     hipLaunchKernelGGL(slot_fresh_kernel, dim3(nblk(nt, tpb)), dim3(tpb), 0, s, m.n, B, d_fresh.p, b64.p,
                        lift.on ? lift.g.p : nullptr, m.d_row_field.p, x64.p, r64.p);
-    if (lift.on) {
-      // the lifted residual of a fresh column is not ||F||: scale by its norm
-      S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+    bool any_fresh = false;
+    for (int k = 0; k < B; k++) any_fresh |= fresh[k] != 0;
+    if (any_fresh) {
+      // the norms of each fresh column's right-hand side: the scaled one,
+      // and with lifted values the plain one (not ||F||) too
+      S.wreduce64(r64.p, S.red_out.p);
       S.fetch(S.red_out.p);
       for (int k = 0; k < B; k++) {
         if (!fresh[k]) continue;
-        const double w0 = std::sqrt(S.h_red[k].x);
-        sl[k].w = w0;
-        sl[k].ref = w0 > 0 ? w0 : nbn;
-        sl[k].rep = std::sqrt(w0 * w0 + gnorm2);
-        sl[k].prev = w0 / sl[k].ref;
-        sl[k].tol = inner_tol(std::max(sl[k].prev, 1e-300));
-        scale[k] = w0 > 0 ? 1.0 / w0 : 0.0;
+        Slot &q = sl[k];
+        const double w0 = std::sqrt(S.h_red[k].x), ws0 = std::sqrt(S.h_red[k].y);
+        if (lift.on) {
+          q.w = w0;
+          q.ref = w0 > 0 ? w0 : nbn;
+          q.rep = std::sqrt(w0 * w0 + gnorm2);
+          q.relp = w0 / q.ref;
+          scale[k] = w0 > 0 ? 1.0 / w0 : 0.0;
+        }
+        q.refs = ws0;
+        q.rels = ws0 > 0 ? 1.0 : 0.0;
+        q.prev = std::max(q.relp, q.rels);
+        q.n0 = {1.0, q.w > 0 ? ws0 / q.w : 0.0};
+        q.tol = inner_tol(std::max(q.prev, 1e-300));
       }
     }
     ELP_HIP(hipMemcpyAsync(d_scale.p, scale.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
@@ -2499,7 +2617,7 @@ void SlotRun<T, B>::after_norms() {
     Slot &q = sl[k];
     if (q.state == 1) {
       it_f[q.f]++;
-      const double nr = std::sqrt(S.h_red[k].x);
+      const double nr = inner_rel(S.h_red[k], q.n0);
       const bool bad = !std::isfinite(nr) || nr > 1e6;
       if (bad || !(nr > q.tol) || it_f[q.f] >= o.max_iters) {
         q.state = 2;
@@ -2547,10 +2665,14 @@ void SlotRun<T, B>::after_norms() {
   } else {
     S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
   }
-  S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+  S.wreduce64(r64.p, S.red_out.p);
   S.fetch(S.red_out.p);
-  std::vector<double> rel(B);
-  for (int k = 0; k < B; k++) rel[k] = std::sqrt(S.h_red[k].x) / (sl[k].ref > 0 ? sl[k].ref : nbn);
+  std::vector<double> rel(B), relp(B), rels(B);
+  for (int k = 0; k < B; k++) {
+    relp[k] = std::sqrt(S.h_red[k].x) / (sl[k].ref > 0 ? sl[k].ref : nbn);
+    rels[k] = sl[k].refs > 0 ? std::sqrt(S.h_red[k].y) / sl[k].refs : 0.0;
+    rel[k] = std::max(relp[k], rels[k]);
+  }
   // a step that did not reduce the residual is taken back
   bool any_undo = false;
   std::vector<int> undo(B, 0);
@@ -2576,12 +2698,10 @@ void SlotRun<T, B>::after_norms() {
     const bool kept_back = q.broken || undo[k];
     const bool fin = kept_back || rel[k] <= o.rtol || rel[k] > 0.5 * q.prev || it_f[q.f] >= o.max_iters;
     if (fin) {
-      finish(k, (kept_back ? q.prev : rel[k]) * q.ref / q.rep);
+      finish(k, kept_back ? reported(q, q.relp, q.rels) : reported(q, relp[k], rels[k]));
       any_new |= assign(k);
     } else {
-      q.prev = rel[k];
-      q.tol = inner_tol(rel[k]);
-      q.w = rel[k] * q.ref;
+      new_rhs(q, relp[k], rels[k]);
       q.state = 1;
       act[k] = 2;
       scale[k] = 1.0 / q.w;
@@ -2601,15 +2721,12 @@ void SlotRun<T, B>::after_norms() {
                          x64.p, iperm);
       ELP_HIP(hipGetLastError());
       S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
-      S.reduce(m.n, r64.p, r64.p, 1, S.red_out.p);
+      S.wreduce64(r64.p, S.red_out.p);
       S.fetch(S.red_out.p);
       for (int k = 0; k < B; k++) {
         if (!was_iter[k]) continue;
         Slot &q = sl[k];
-        const double rl = std::sqrt(S.h_red[k].x) / q.ref;
-        q.prev = rl;
-        q.tol = inner_tol(rl);
-        q.w = rl * q.ref;
+        new_rhs(q, std::sqrt(S.h_red[k].x) / q.ref, q.refs > 0 ? std::sqrt(S.h_red[k].y) / q.refs : 0.0);
         act[k] = 2;
         scale[k] = q.w > 0 ? 1.0 / q.w : 0.0;
       }
@@ -2735,7 +2852,7 @@ __global__ __launch_bounds__(256) void grp_dir_kernel(const RunSeg<T> *__restric
 template <typename T, int B>
 __global__ __launch_bounds__(256) void grp_update_kernel(const RunSeg<T> *__restrict__ segs) {
   const RunSeg<T> &q = segs[blockIdx.x / kRedBlocks];
-  update_norm_body<T>(q.n, B, q.alpha, q.pi, q.qi, q.xi, q.ri, q.red_part, q.a, q.coefP, q.u,
+  update_norm_body<T>(q.n, B, q.alpha, q.pi, q.qi, q.xi, q.ri, q.red_part, q.a, q.coefP, q.coefA, q.u,
                       blockIdx.x % kRedBlocks, kRedBlocks);
 }
 
@@ -2944,7 +3061,7 @@ std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *fre
     return nullptr;
   }
   Model::SolverState &st = solver_state(m, o, Z, setup);
-  const bool dbl = st.coupled;  // coupled models iterate in FP64 (see sweep)
+  const bool dbl = st.fp64_inner;  // coupled models iterate in FP64 (see sweep)
   // requests batch with others of their size class, so that a small
   // request's step is not the length of a large one's
   const int size_class = m.n <= 50000 ? 0 : m.n <= 500000 ? 1 : 2;
@@ -2996,12 +3113,13 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
 }
 
 int64_t solver_bytes_estimate(const Model &m, int block, int precision) {
-  // coupled models (structure rows beside a Hex8 box) iterate in FP64 and
-  // keep two factorizations of the structure block (the window in use and
-  // the next): about 7 ns m complex FP64 values each with their work space,
-  // m the half-bandwidth of the structure block, about 3 sqrt(ns) for plates
+  // models with structure and fluid rows iterate in FP64; beside a Hex8
+  // box they keep two factorizations of the structure block (the window in
+  // use and the next): about 7 ns m complex FP64 values each with their work
+  // space, m the half-bandwidth of the structure block, about 3 sqrt(ns)
+  // for plates
   const bool cpl = coupled_possible(m);
-  const int64_t vb = (precision == ELP_PREC_FP64 || cpl) ? 16 : 8;
+  const int64_t vb = (precision == ELP_PREC_FP64 || two_fields(m)) ? 16 : 8;
   int64_t b = m.nnz * 8 + m.n * 24;                    // FP32 shadows, l1 sums
   if (cpl) {
     int64_t ns = 0;
@@ -3029,11 +3147,12 @@ int64_t sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z,
   const int B = o.precision == ELP_PREC_FP64 ? 1 : o.block;
   if (o.precision == ELP_PREC_FP64) {
     run_sweep<double, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s);
-  } else if (st.coupled) {
+  } else if (st.fp64_inner) {
     // Coupled models batch their frequencies like the mixed solver but keep
     // the inner Krylov in FP64: a thin plate's stiffness has cond ~1e8, so
     // the FP32 operator's own rounding (~6e-8 relative) leaves FP32 inner
-    // solves that no FP64 refinement step can improve on
+    // solves that no FP64 refinement step can improve on, and the pressure
+    // (held to rtol through the scaled residual) is driven by the plate
     switch (B) {
       case 1: run_sweep<double, 1>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;
       case 4: run_sweep<double, 4>(m, st, nf, freqs, Z, o, X, relres, iters, stats, s); break;

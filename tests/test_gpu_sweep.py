@@ -111,6 +111,8 @@ def test_coupled_sweep_jacobi_small(ctx):
     # the thin clamped plate makes A(w) ill-conditioned (cond ~ 1e5): a 1e-8
     # residual bounds the error only to ~1e-3
     _check_solution(m, freqs, X, rel, err_tol=1e-3)
+    e = _check_fields(m, freqs, X, 1e-5)
+    print(f"[sweep] coupled (l1-Jacobi) fields vs scaled direct solve: {e:.2e} iters {it.tolist()}", flush=True)
 
 
 @pytest.mark.parametrize("quadratic", [False, True])
@@ -161,6 +163,33 @@ def _check_forward_error(m, freqs, X):
         x = sla.splu(Am.tocsc()).solve(F)
         bound = 2 * np.linalg.cond(Am.toarray()) * max(r, 1e-16)
         assert np.linalg.norm(X[k] - x) <= bound * np.linalg.norm(x), (f, bound)
+
+
+def _check_fields(m, freqs, X, tol):
+    """Each field of the solution (displacements, pressure) against a direct
+    solve of the symmetrically scaled system (S A S) y = S F, x = S y,
+    S = |diag A|^-1/2.  A residual relative to the whole right-hand side
+    cannot stand for the pressure of a coupled model: its fluid rows carry
+    elPaSo's 1/(rho w^2), 1e-6 against the structure rows' 1e9, so a
+    solution with the pressure left at 0 passes a 1e-8 residual bar."""
+    import scipy.sparse as sp
+    num = O.Numbering(m)
+    pat = O.pattern(m, num)
+    dof = np.full(num.n, -1)
+    nodes, dd = np.nonzero(num.rows >= 0)
+    dof[num.rows[nodes, dd]] = dd
+    worst = 0.0
+    for k, f in enumerate(freqs):
+        rp, ci, A, F = O.dynamic_matrix(m, 2 * np.pi * f, num, pat)
+        Am = O.csr(rp, ci, A, num.n)
+        s = 1.0 / np.sqrt(np.abs(Am.diagonal()))
+        S = sp.diags(s)
+        x = s * sla.splu((S @ Am @ S).tocsc()).solve(s * F)
+        for fld in (dof == O.ELEM_DOFS[O.FLUID8][0], dof != O.ELEM_DOFS[O.FLUID8][0]):
+            e = np.linalg.norm(X[k][fld] - x[fld]) / np.linalg.norm(x[fld])
+            worst = max(worst, e)
+            assert e <= tol, (f, e)
+    return worst
 
 
 @pytest.mark.parametrize("quadratic,nz", [(False, 40), (True, 12)])
@@ -255,3 +284,10 @@ def test_coupled_sweep_block_preconditioner(ctx, block):
     assert (rel <= 1e-8).all() and st.unconverged == 0, (rel, it)
     assert st.refactors >= 1
     _check_forward_error(m, freqs, X)
+    e = _check_fields(m, freqs, X, 1e-5)
+    print(f"[sweep] coupled fields vs scaled direct solve: {e:.2e}", flush=True)
+    # the per-frequency FP64 path (the plate factored at each frequency)
+    X64, rel64, _, st64 = g.sweep(freqs, precision=_lib.PREC_FP64)
+    assert (rel64 <= 1e-8).all() and st64.unconverged == 0, rel64
+    e64 = _check_fields(m, freqs, X64, 1e-5)
+    print(f"[sweep] coupled fields, per-frequency FP64: {e64:.2e}", flush=True)
