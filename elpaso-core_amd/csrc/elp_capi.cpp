@@ -217,6 +217,24 @@ int elp_model_set_box(elp_model *mm, const double *origin, const double *length,
   ELP_CATCH
 }
 
+int elp_model_get_box(const elp_model *mm, int *set, int *detected, double *origin, double *length, int32_t *nel,
+                      int *order, int *robin_sides) {
+  ELP_TRY
+  ELP_ARG(mm && mm->m, "null model");
+  const auto &m = *mm->m;
+  ELP_REQUIRE(m.finalized, ELP_ESTATE, "model not finalized");
+  if (set) *set = m.box.set ? 1 : 0;
+  if (detected) *detected = m.box.detected ? 1 : 0;
+  for (int d = 0; d < 3; d++) {
+    if (origin) origin[d] = m.box.set ? m.box.origin[d] : 0.0;
+    if (length) length[d] = m.box.set ? m.box.length[d] : 0.0;
+    if (nel) nel[d] = m.box.set ? m.box.nel[d] : 0;
+  }
+  if (order) *order = m.box.set ? m.box.order : 0;
+  if (robin_sides) *robin_sides = m.box.set ? m.box.robin_sides : 0;
+  ELP_CATCH
+}
+
 int elp_model_finalize(elp_model *mm) {
   ELP_TRY
   auto &m = M(mm);

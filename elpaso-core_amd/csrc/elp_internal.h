@@ -179,6 +179,7 @@ struct KindBlock {
 
 struct BoxInfo {
   bool set = false;
+  bool detected = false;  // read from the mesh at finalize (no hint given)
   double origin[3], length[3];
   int nel[3], order = 1, robin_sides = 0;
 };

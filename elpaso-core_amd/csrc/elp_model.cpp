@@ -41,6 +41,99 @@ void parallel_for(int64_t n, F f) {
   for (auto &x : th) x.join();
 }
 
+// The box description (origin, lengths, elements per axis, order, Robin
+// sides) read from the mesh when the caller gave none, so that meshes read
+// from files get the box preconditioners too.  Found only when the fluid
+// elements are of one kind and tile a uniform axis-aligned lattice exactly:
+// each element occupies its own cell with its nodes at the cell's lattice
+// points (corners, and for Hex20 the edge midpoints), and every Robin face
+// lies on a side of the box.  Anything else leaves the box unset.
+void detect_box(Model &m) {
+  if (m.box.set) return;
+  const KindBlock *fk = nullptr;
+  for (auto &kb : m.kinds)
+    if (is_fluid(kb.kind)) {
+      if (fk) return;  // Fluid8 and Fluid20 together
+      fk = &kb;
+    }
+  if (!fk || fk->n_elem == 0) return;
+  const int order = fk->kind == ELP_FLUID20 ? 2 : 1;
+  const int npe = fk->npe;
+  const int(*nat)[3] = npe == 8 ? kHex8Nat : kHex20Nat;
+  auto X = [&](int32_t r, int d) { return m.xyz[(int64_t)m.rank_to_input[r] * 3 + d]; };
+  // the lattice coordinates of each axis: the distinct values over the
+  // fluid nodes, evenly spaced
+  double lo[3], hi[3], h[3];
+  int64_t nel[3];
+  for (int d = 0; d < 3; d++) {
+    std::vector<double> v;
+    v.reserve(fk->conn.size());
+    for (int32_t r : fk->conn) v.push_back(X(r, d));
+    std::sort(v.begin(), v.end());
+    lo[d] = v.front();
+    hi[d] = v.back();
+    const double ext = hi[d] - lo[d];
+    if (!(ext > 0)) return;
+    const double tol = 1e-9 * ext;
+    std::vector<double> u;
+    for (double x : v)
+      if (u.empty() || x - u.back() > tol) u.push_back(x);
+    const int64_t p = (int64_t)u.size();
+    if ((p - 1) % order) return;
+    nel[d] = (p - 1) / order;
+    const double step = ext / (double)(p - 1);
+    for (int64_t k = 0; k < p; k++)
+      if (std::fabs(u[k] - (lo[d] + step * (double)k)) > tol) return;
+    h[d] = ext / (double)nel[d];
+  }
+  if (nel[0] * nel[1] * nel[2] != fk->n_elem) return;
+  // each element on its own cell, its nodes at the cell's lattice points
+  std::vector<uint8_t> taken((size_t)fk->n_elem, 0);
+  for (int64_t e = 0; e < fk->n_elem; e++) {
+    const int32_t *c = &fk->conn[e * npe];
+    int64_t cell[3];
+    for (int d = 0; d < 3; d++) {
+      double mn = X(c[0], d);
+      for (int a = 1; a < npe; a++) mn = std::min(mn, X(c[a], d));
+      cell[d] = (int64_t)std::llround((mn - lo[d]) / h[d]);
+      if (cell[d] < 0 || cell[d] >= nel[d]) return;
+      for (int a = 0; a < npe; a++) {
+        const double want = lo[d] + h[d] * ((double)cell[d] + 0.5 * (double)(nat[a][d] + 1));
+        if (std::fabs(X(c[a], d) - want) > 1e-9 * (hi[d] - lo[d])) return;
+      }
+    }
+    uint8_t &t = taken[(size_t)((cell[2] * nel[1] + cell[1]) * nel[0] + cell[0])];
+    if (t) return;
+    t = 1;
+  }
+  // Robin faces on the box sides (bit k: 0-based Hex8 face k's normal,
+  // -y, +y, -z, +x, +z, -x)
+  static const int side_bit[3][2] = {{5, 3}, {0, 1}, {2, 4}};
+  int sides = 0;
+  for (int64_t f = 0; f < m.n_faces; f++) {
+    const int32_t *fn = &m.face_nodes[f * m.nfn];
+    int bit = -1;
+    for (int d = 0; d < 3 && bit < 0; d++)
+      for (int s = 0; s < 2 && bit < 0; s++) {
+        const double at = s ? hi[d] : lo[d];
+        bool all = true;
+        for (int k = 0; k < m.nfn; k++) all &= std::fabs(X(fn[k], d) - at) <= 1e-9 * (hi[d] - lo[d]);
+        if (all) bit = side_bit[d][s];
+      }
+    if (bit < 0) return;  // a Robin face inside the box
+    sides |= 1 << bit;
+  }
+  for (int d = 0; d < 3; d++) {
+    m.box.origin[d] = lo[d];
+    m.box.length[d] = hi[d] - lo[d];
+    m.box.nel[d] = (int)nel[d];
+  }
+  m.box.order = order;
+  m.box.robin_sides = sides;
+  m.box.set = true;
+  m.box.detected = true;
+}
+
 }  // namespace
 
 void finalize_model(Model &m) {
@@ -503,6 +596,7 @@ void finalize_model(Model &m) {
   m.d_if_fnodes.upload(c, m.if_fnodes);
   m.d_if_factor.upload(c, m.if_factor);
   post_prepare(m);
+  detect_box(m);
   m.finalized = true;
 }
 

@@ -20,7 +20,7 @@ EXPORTED_SYMBOLS = [
     "elp_model_create", "elp_model_destroy", "elp_model_set_nodes", "elp_model_add_fluid_material",
     "elp_model_add_structure_material", "elp_model_add_elements", "elp_model_add_impedance_faces",
     "elp_model_add_interfaces", "elp_model_add_point_source", "elp_model_add_point_force",
-    "elp_model_add_dirichlet", "elp_model_set_rayleigh", "elp_model_set_box", "elp_model_finalize",
+    "elp_model_add_dirichlet", "elp_model_set_rayleigh", "elp_model_set_box", "elp_model_get_box", "elp_model_finalize",
     "elp_model_info", "elp_model_dof_map", "elp_assemble", "elp_assemble_timed", "elp_get_pattern",
     "elp_get_values", "elp_get_device_csr", "elp_dynamic_matrix", "elp_element_matrices", "elp_sweep",
     "elp_results_sizes", "elp_node_results", "elp_node_results_noalloc",
@@ -112,6 +112,7 @@ def load_library():
     L.elp_model_add_dirichlet.argtypes = [vp, i64, C.c_int, dbl, dbl]
     L.elp_model_set_rayleigh.argtypes = [vp, dbl, dbl]
     L.elp_model_set_box.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int]
+    L.elp_model_get_box.argtypes = [vp, P(C.c_int), P(C.c_int), vp, vp, vp, P(C.c_int), P(C.c_int)]
     L.elp_model_finalize.argtypes = [vp]
     L.elp_model_info.argtypes = [vp, P(i64), P(i64), P(i64)]
     L.elp_model_dof_map.argtypes = [vp, vp, vp]
@@ -234,7 +235,9 @@ class GpuModel:
         a, b = model.rayleigh
         if a or b:
             _check(L.elp_model_set_rayleigh(self._h, a, b))
-        box = box if box is not None else getattr(model, "box", None)
+        # box: None = the model's own hint (if any), False = no hint (the
+        # library reads the box from the mesh when it can)
+        box = None if box is False else (box if box is not None else getattr(model, "box", None))
         if box is not None:
             self.set_box(box)
         if finalize:
@@ -258,6 +261,18 @@ class GpuModel:
         ne = np.ascontiguousarray(box["nel"], dtype=np.int32)
         _check(load_library().elp_model_set_box(self._h, _ptr(o), _ptr(ln), _ptr(ne), int(box["order"]),
                                                 int(box.get("robin_sides", 0))))
+
+    def get_box(self):
+        """The box the solver uses: None, or a dict like set_box's with
+        "detected" = True when it was read from the mesh."""
+        st, det, order, sides = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        o, ln, ne = np.zeros(3), np.zeros(3), np.zeros(3, dtype=np.int32)
+        _check(load_library().elp_model_get_box(self._h, C.byref(st), C.byref(det), _ptr(o), _ptr(ln), _ptr(ne),
+                                                C.byref(order), C.byref(sides)))
+        if not st.value:
+            return None
+        return dict(origin=tuple(o), length=tuple(ln), nel=tuple(int(v) for v in ne), order=order.value,
+                    robin_sides=sides.value, detected=bool(det.value))
 
     def finalize(self):
         L = load_library()

@@ -114,7 +114,8 @@ def read_analysis(path) -> Analysis:
 def read_model(ctx: Context, path, box=None, finalize=True):
     """Model of an elPaSo input file on the device.  Returns (GpuModel,
     Analysis, counts dict).  ``box`` optionally declares the mesh a structured
-    box (enables the multigrid preconditioner), as for GpuModel."""
+    box, as for GpuModel; without it the library reads the box from the mesh
+    at finalize when the fluid elements tile a uniform lattice."""
     L = load_library_h5()
     g = GpuModel.empty(ctx)
     a, c = _Analysis(), _Counts()

@@ -149,9 +149,18 @@ int elp_model_set_rayleigh(elp_model *m, double alpha, double beta);
  * Enables the multigrid preconditioner.  For a model with structure rows the
  * box describes its Hex8 fluid cavity (order 1): the sweep then uses the
  * block-diagonal coupled preconditioner (exact box solve of the fluid rows,
- * direct block cyclic reduction of the structure block). */
+ * direct block cyclic reduction of the structure block).  Without it,
+ * elp_model_finalize reads the box from the mesh when the fluid elements
+ * (one kind) tile a uniform axis-aligned lattice and every Robin face lies on
+ * a box side (models read from elPaSo input files included). */
 int elp_model_set_box(elp_model *m, const double *origin, const double *length,
                       const int32_t *nel, int order, int robin_sides);
+/* The model's box (finalized model): *set = 0 when it has none, *detected =
+ * 1 when it was read from the mesh.  Any output pointer may be NULL.  No
+ * counterpart in the reference (elPaSo solves every model with its direct
+ * solver). */
+int elp_model_get_box(const elp_model *m, int *set, int *detected, double *origin, double *length,
+                      int32_t *nel, int *order, int *robin_sides);
 /* DOF numbering, symbolic phase (sparsity pattern, adjacency), upload.
  * ELP_EINVAL for a node coupled to more than 256 neighbour nodes or columns,
  * ELP_EDETJ (at assembly) for an element with detJ <= 0. */
