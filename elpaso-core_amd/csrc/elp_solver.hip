@@ -1548,7 +1548,6 @@ static bool mg_possible(const Model &m) {
   return m.dir_rows.empty();
 }
 
-// a Hex8 fluid box (the box hint) coupled to structure rows
 // structure and fluid rows in one model
 static bool two_fields(const Model &m) {
   bool has_s = false, has_f = false;
@@ -1559,6 +1558,7 @@ static bool two_fields(const Model &m) {
   return has_s && has_f;
 }
 
+// a Hex8 fluid box (given or read from the mesh) coupled to structure rows
 static bool coupled_possible(const Model &m) { return m.box.set && m.box.order == 1 && two_fields(m); }
 
 static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::complex<double> Z, hipStream_t s) {
