@@ -27,20 +27,6 @@ constexpr uint32_t kFluidMask = 1u << 16;     // fluid
 
 uint32_t kind_mask(int kind) { return is_fluid(kind) ? kFluidMask : kStructMask; }
 
-template <typename F>
-void parallel_for(int64_t n, F f) {
-  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 4096) nt = 1;
-  std::vector<std::thread> th;
-  int64_t chunk = (n + nt - 1) / nt;
-  for (unsigned t = 0; t < nt; t++) {
-    int64_t a = t * chunk, b = std::min<int64_t>(n, a + chunk);
-    if (a >= b) break;
-    th.emplace_back([=]() { f(a, b); });
-  }
-  for (auto &x : th) x.join();
-}
-
 // The box description (origin, lengths, elements per axis, order, Robin
 // sides) read from the mesh when the caller gave none, so that meshes read
 // from files get the box preconditioners too.  Found only when the fluid

@@ -264,6 +264,19 @@ struct PadCsr {
   const float *K, *M;
 };
 
+// Row tiles of the padded FP32 operator (tile_op_kernel): consecutive rows
+// (in the operator's locality order) grouped so that the distinct columns
+// of a tile fit in LDS; each padded entry holds the byte offset in LDS of
+// its column's line: 256 (1 + the position of the column in its tile's
+// sorted column list), 0 (a zero line) for padding.
+struct TileCsr {
+  const int32_t *row;   // tiles + 1: first row of each tile
+  const int32_t *cptr;  // tiles + 1: first column of each tile in col
+  const int32_t *col;   // the tiles' distinct columns
+  const uint32_t *loff; // per padded entry
+  int64_t tiles;
+};
+
 // First l1-Jacobi sweep from a zero guess, y = b / D, one thread per
 // (row, frequency): the same values as op_kernel's MODE 3.
 template <typename T>
@@ -339,6 +352,83 @@ __device__ inline void row_batches(const PadCsr &pc, int32_t base, int32_t p1, u
   }
 }
 
+// row_batches with the row's x offsets, K and M read by scalar loads into
+// SGPRs (constant address space: the row base is wave-uniform), 16 entries
+// per batch (8 when fewer remain): per entry one buffer load of x with the
+// offset as its SGPR soffset and two FMAs with SGPR operands, where the
+// vector form needs three v_readlane per entry besides.  The same entries
+// in the same order as row_batches: the same sums bit for bit.
+typedef const __attribute__((address_space(4))) uint32_t *ScalarU32;
+typedef const __attribute__((address_space(4))) float *ScalarF32;
+__device__ inline void row_batches_s(const PadCsr &pc, int32_t base, int32_t p1, __amdgpu_buffer_rsrc_t xr,
+                                     uint32_t vo, float &aK, float &aM) {
+  const ScalarU32 so = (ScalarU32)pc.off;
+  const ScalarF32 sk = (ScalarF32)pc.K, sm = (ScalarF32)pc.M;
+  for (; base + 16 <= p1; base += 16) {
+    uint32_t off[16];
+    float k[16], m[16], xv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      off[u] = so[base + u];
+      k[u] = sk[base + u];
+      m[u] = sm[base + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, off[u], 0));
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      aK = fmaf(k[u], xv[u], aK);
+      aM = fmaf(m[u], xv[u], aM);
+    }
+  }
+  if (base < p1) {  // 8 entries left (rows are padded to multiples of 8)
+    uint32_t off[8];
+    float k[8], m[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      off[u] = so[base + u];
+      k[u] = sk[base + u];
+      m[u] = sm[base + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, off[u], 0));
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      aK = fmaf(k[u], xv[u], aK);
+      aM = fmaf(m[u], xv[u], aM);
+    }
+  }
+}
+
+// row_batches with x read from the tile's lines in LDS: the entries' LDS
+// line offsets, K and M loaded 64 at a time with coalesced vector loads and
+// broadcast with v_readlane
+template <int NB>
+__device__ inline void row_batches_lds(const PadCsr &pc, const uint32_t *__restrict__ loff, int32_t base,
+                                         int32_t p1, const float *lds_lane, int lane, float &aK, float &aM) {
+  static_assert(2 * NB == 64, "two batches per wavefront-wide index load");
+  const char *lb = reinterpret_cast<const char *>(lds_lane);
+  for (; base < p1; base += 2 * NB) {
+    const bool in = base + lane < p1;
+    const uint32_t myoff = in ? loff[base + lane] : 0u;
+    const float myk = in ? pc.K[base + lane] : 0.f, mym = in ? pc.M[base + lane] : 0.f;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (h == 1 && base + NB >= p1) break;  // uniform: the row ends in the first batch
+      float xv[NB];
+#pragma unroll
+      for (int u = 0; u < NB; u++)
+        xv[u] = *reinterpret_cast<const float *>(lb + __builtin_amdgcn_readlane(myoff, h * NB + u));
+#pragma unroll
+      for (int u = 0; u < NB; u++) {
+        const int e = h * NB + u;
+        aK = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, myk), e)), xv[u], aK);
+        aM = fmaf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mym), e)), xv[u], aM);
+      }
+    }
+  }
+}
+
 // Row-per-wavefront operator for B = 32 (FP32): lane l holds float l of the
 // row's 32 complex frequencies (frequency l / 2, real part for even l,
 // imaginary part for odd l).  K and M are real, so both parts accumulate
@@ -349,10 +439,11 @@ __device__ inline void row_batches(const PadCsr &pc, int32_t base, int32_t p1, u
 // partner lane's part.  Entries are summed in row order.
 constexpr int kRowWg = 256;  // threads per workgroup: kRowWg / 64 consecutive rows
 
-template <int MODE>
+template <int MODE, bool TILE = false, bool SCALAR = false>
 __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const OpCoef *__restrict__ coef,
                                  const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
-                                 int64_t r, int lane) {
+                                 int64_t r, int lane, const uint32_t *__restrict__ tloff = nullptr,
+                                 const float *lds = nullptr) {
   constexpr int W = 64;  // floats per row: 32 complex frequencies
   const int64_t o = r * W + lane;
   const uint8_t fld = a.field[r];
@@ -371,7 +462,9 @@ __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, xbytes, 0x00020000);
     const uint32_t vo = (uint32_t)lane * sizeof(float);
     const int32_t base = pc.rp[r], p1 = pc.rp[r + 1];
-    row_batches<32>(pc, base, p1, (uint32_t)xbytes, xr, vo, lane, aK, aM);
+    if constexpr (TILE) row_batches_lds<32>(pc, tloff, base, p1, lds + lane, lane, aK, aM);
+    else if constexpr (SCALAR) row_batches_s(pc, __builtin_amdgcn_readfirstlane(base), __builtin_amdgcn_readfirstlane(p1), xr, vo, aK, aM);
+    else row_batches<32>(pc, base, p1, (uint32_t)xbytes, xr, vo, lane, aK, aM);
     const float *xl = x + lane;
     for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
       const float xv = xl[(uint32_t)a.xcol[k] * (uint32_t)W];
@@ -409,13 +502,57 @@ __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const
   y[o] = xo + (bv - res) * inv;
 }
 
-template <int MODE>
+template <int MODE, bool SCALAR = false>
 __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr pc, const OpCoef *__restrict__ coef,
                                                      const float *__restrict__ x, const float *__restrict__ b,
                                                      float *__restrict__ y) {
   const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
   if (r >= a.n) return;
-  row_apply<MODE>(a, pc, coef, x, b, y, r, threadIdx.x & 63);
+  row_apply<MODE, false, SCALAR>(a, pc, coef, x, b, y, r, threadIdx.x & 63);
+}
+
+// The row operator on row tiles: a workgroup copies the x lines of its
+// tile's distinct columns into LDS once (line 0 zero), then each wavefront
+// applies rows of the tile with row_apply reading x from LDS.  A row's x
+// lines come from HBM/L2 once per tile instead of once per entry (a Hex20
+// row has about 57 entries and neighbouring rows share most columns).
+// Dynamic LDS: 256 bytes per line (32 complex FP32 frequencies).
+constexpr int kTileWg = 1024;
+constexpr int kTileStage = 4;  // float4 of x per thread in the staging: (255 + 1) lines x 16 / 1024
+template <int MODE>
+__global__ __launch_bounds__(kTileWg) void tile_op_kernel(OpArgs<float> a, PadCsr pc, TileCsr tc,
+                                                          const OpCoef *__restrict__ coef,
+                                                          const float *__restrict__ x, const float *__restrict__ b,
+                                                          float *__restrict__ y) {
+  extern __shared__ float4 tile_lds[];
+  const int64_t t = xcd_block();
+  if (t >= tc.tiles) return;  // whole workgroup
+  const int32_t c0 = tc.cptr[t], nl = tc.cptr[t + 1] - c0;
+  const float4 *x4 = reinterpret_cast<const float4 *>(x);
+  // stage: 16 float4 per line, at most kTileStage per thread; the column
+  // indices first, then every x load in flight together
+  const int nq = (nl + 1) * 16;
+  int32_t col[kTileStage];
+#pragma unroll
+  for (int u = 0; u < kTileStage; u++) {
+    const int i = threadIdx.x + u * kTileWg, l = i >> 4;
+    col[u] = (i < nq && l > 0) ? tc.col[c0 + l - 1] : -1;
+  }
+  float4 v[kTileStage];
+#pragma unroll
+  for (int u = 0; u < kTileStage; u++) {
+    const int i = threadIdx.x + u * kTileWg;
+    v[u] = col[u] >= 0 ? x4[(int64_t)col[u] * 16 + (i & 15)] : float4{0.f, 0.f, 0.f, 0.f};
+  }
+#pragma unroll
+  for (int u = 0; u < kTileStage; u++)
+    if (threadIdx.x + u * kTileWg < nq) tile_lds[threadIdx.x + u * kTileWg] = v[u];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const float *lds = reinterpret_cast<const float *>(tile_lds);
+  const int32_t r1 = tc.row[t + 1];
+  for (int32_t r = tc.row[t] + (int)(threadIdx.x >> 6); r < r1; r += kTileWg / 64)
+    row_apply<MODE, true>(a, pc, coef, x, b, y, __builtin_amdgcn_readfirstlane(r), lane, tc.loff, lds);
 }
 
 // Post-smoothing of the two-level box cycle fused with the prolongation,
@@ -1038,6 +1175,13 @@ struct LevelData {
   DArr<uint32_t> pad_off;
   DArr<float> pad_K, pad_M;
   PadCsr padded() const { return {pad_rp.p, pad_off.p, pad_K.p, pad_M.p}; }
+  // row tiles of the padded operator (tile_op_kernel); n_tiles = 0: none
+  DArr<int32_t> t_row, t_cptr, t_col;
+  DArr<uint32_t> t_loff;
+  int64_t n_tiles = 0;
+  int t_rows = 0, t_lines = 0;  // rows per tile (before splits), most lines of a tile
+  bool op_scalar = false;        // row kernel with the row's entries in SGPRs
+  TileCsr tiles() const { return {t_row.p, t_cptr.p, t_col.p, t_loff.p, n_tiles}; }
   // box cycle: K P, M P, C_Gamma P (row-padded, permuted rows, lattice
   // columns as byte offsets) and P rows padded to 8 (box_post_kernel)
   DArr<int32_t> ap_rp;
@@ -1091,6 +1235,8 @@ struct elp::Model::SolverState {
   std::unique_ptr<elp::BcrPlan, void (*)(elp::BcrPlan *)> bcr{nullptr, nullptr};
   // models with structure and fluid rows iterate in FP64 (see sweep)
   bool fp64_inner = false;
+  int op_tile_rows = 0;  // requested (the cache key)
+  bool op_tiles_wanted = false;
 };
 
 namespace elp {
@@ -1108,6 +1254,7 @@ void default_opts(elp_solve_opts &o) {
   o.coarse_sweeps = 20;
   o.mg_levels = 0;
   o.mg_min_nodes = 2000;
+  o.op_tile_rows = 0;
 }
 
 static void prepare_level(LevelData &L, hipStream_t s) {
@@ -1474,6 +1621,70 @@ static void permute_level_float(LevelData &L, hipStream_t s) {
   }
 }
 
+// Row tiles for tile_op_kernel over the padded FP32 operator (rows in the
+// operator's order): chunks of `rows` consecutive rows, each halved until its
+// distinct columns number at most max_lines (a single row always fits: rows
+// hold at most 256 columns).  Per padded entry the 1-based position of its
+// column in the tile's sorted column list times 256 (its line's byte offset
+// in LDS), 0 for padding.
+static void build_tiles(LevelData &L, const std::vector<int32_t> &perm, const std::vector<int32_t> &iperm,
+                        const std::vector<int32_t> &rp8, int rows, int max_lines) {
+  const Model &m = *L.model;
+  const int64_t n = m.n, nch = (n + rows - 1) / rows;
+  struct Sub {
+    int32_t r0, r1;
+    std::vector<int32_t> cols;
+  };
+  std::vector<std::vector<Sub>> per(nch);
+  std::vector<uint32_t> idx(rp8[n], 0);
+  parallel_for(nch, [&](int64_t a, int64_t b) {
+    std::vector<int32_t> u;
+    for (int64_t ch = a; ch < b; ch++) {
+      std::vector<std::pair<int32_t, int32_t>> todo{{(int32_t)(ch * rows), (int32_t)std::min<int64_t>(n, (ch + 1) * rows)}};
+      while (!todo.empty()) {  // depth first, first half first: tiles stay in row order
+        const auto rr = todo.back();
+        todo.pop_back();
+        u.clear();
+        for (int32_t r = rr.first; r < rr.second; r++)
+          for (int32_t p = m.rowptr[perm[r]]; p < m.rowptr[perm[r] + 1]; p++) u.push_back(iperm[m.colind[p]]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if ((int)u.size() > max_lines && rr.second - rr.first > 1) {
+          const int32_t mid = rr.first + (rr.second - rr.first) / 2;
+          todo.push_back({mid, rr.second});
+          todo.push_back({rr.first, mid});
+          continue;
+        }
+        for (int32_t r = rr.first; r < rr.second; r++) {
+          int32_t q = rp8[r];
+          for (int32_t p = m.rowptr[perm[r]]; p < m.rowptr[perm[r] + 1]; p++, q++)
+            idx[q] = 256u * (uint32_t)(1 + (std::lower_bound(u.begin(), u.end(), iperm[m.colind[p]]) - u.begin()));
+        }
+        per[ch].push_back({rr.first, rr.second, u});
+      }
+    }
+  });
+  std::vector<int32_t> trow{0}, tcp{0}, tcol;
+  int most = 0;
+  for (auto &v : per)
+    for (auto &t : v) {
+      trow.back() = t.r0;
+      trow.push_back(t.r1);
+      tcol.insert(tcol.end(), t.cols.begin(), t.cols.end());
+      tcp.push_back((int32_t)tcol.size());
+      most = std::max(most, (int)t.cols.size());
+    }
+  Ctx *c = m.ctx;
+  L.t_row.upload(c, trow);
+  L.t_cptr.upload(c, tcp);
+  L.t_col.upload(c, tcol);
+  L.t_loff.upload(c, idx);
+  L.n_tiles = (int64_t)trow.size() - 1;
+  L.t_rows = rows;
+  L.t_lines = most;
+  if (most > max_lines) L.n_tiles = 0;  // a row alone has more columns than fit: no tiles
+}
+
 // trilinear interpolation from a coarse box lattice to the rows of `fine`
 static void build_interp(LevelData &F, const Model &coarse_like_box, const int nelc[3], bool fluid_only = false) {
   const Model &fm = *F.model;
@@ -1548,6 +1759,11 @@ static bool mg_possible(const Model &m) {
   return m.dir_rows.empty();
 }
 
+// the 32-slot FP32 operator may run on row tiles
+static bool tiles_wanted(const elp_solve_opts &o) {
+  return o.precision != ELP_PREC_FP64 && o.block == 32 && o.op_tile_rows != -1;
+}
+
 // structure and fluid rows in one model
 static bool two_fields(const Model &m) {
   bool has_s = false, has_f = false;
@@ -1561,11 +1777,14 @@ static bool two_fields(const Model &m) {
 // a Hex8 fluid box (given or read from the mesh) coupled to structure rows
 static bool coupled_possible(const Model &m) { return m.box.set && m.box.order == 1 && two_fields(m); }
 
+static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_opts &o, hipStream_t s);
+
 static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::complex<double> Z, hipStream_t s) {
   const bool want_mg = o.precond == ELP_PC_MG && mg_possible(m);
   const bool want_cpl = o.precond == ELP_PC_MG && !want_mg && coupled_possible(m);
   if (m.solver && m.solver->mg == want_mg && m.solver->mg_levels == o.mg_levels &&
-      m.solver->mg_min_nodes == o.mg_min_nodes && m.solver->cpl_requested == want_cpl)
+      m.solver->mg_min_nodes == o.mg_min_nodes && m.solver->cpl_requested == want_cpl &&
+      m.solver->op_tile_rows == o.op_tile_rows && m.solver->op_tiles_wanted == tiles_wanted(o))
     return *m.solver;
   m.solver.reset();
   auto st = std::make_shared<Model::SolverState>();
@@ -1574,6 +1793,8 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   st->mg_levels = o.mg_levels;
   st->mg_min_nodes = o.mg_min_nodes;
   st->fp64_inner = two_fields(m);
+  st->op_tile_rows = o.op_tile_rows;
+  st->op_tiles_wanted = tiles_wanted(o) && !st->fp64_inner;
   auto L0 = std::make_unique<LevelData>();
   L0->model = &m;
   prepare_level(*L0, s);
@@ -1631,6 +1852,7 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
       for (int d = 0; d < 3; d++) nel[d] = nc[d];
     }
   }
+  choose_operator(m, *st, o, s);
   ELP_HIP(hipStreamSynchronize(s));
   m.solver = st;
   return *st;
@@ -1644,6 +1866,99 @@ struct LevelWork {
   using V = typename VT<T>::t;
   DArr<V> x, b, r, e, ta, tb;
 };
+
+static OpCoef make_coef(const Model &m, double w, std::complex<double> Z, double shift);
+
+// Launch of the 32-slot FP32 operator of a level: on its row tiles when it
+// has them, else the row kernel (both give the same bits)
+constexpr int kTileMaxLines = kTileStage * kTileWg / 16 - 1;  // 64 KB of LDS per tile with the zero line
+template <int MODE>
+static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x, const float *b, float *y,
+                          hipStream_t s) {
+  if (L.n_tiles > 0) {
+    hipLaunchKernelGGL((tile_op_kernel<MODE>), dim3(xcd_grid(L.n_tiles)), dim3(kTileWg),
+                       (size_t)(L.t_lines + 1) * 256, s, L.template args<float>(), L.padded(), L.tiles(), coef, x, b,
+                       y);
+  } else if (L.op_scalar) {
+    hipLaunchKernelGGL((row_op_kernel<MODE, true>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
+                       L.template args<float>(), L.padded(), coef, x, b, y);
+  } else {
+    hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
+                       L.template args<float>(), L.padded(), coef, x, b, y);
+  }
+  ELP_HIP(hipGetLastError());
+}
+
+// The 32-slot FP32 operator of level 0: the row kernel or row tiles of 16,
+// 32 or 64 rows, as requested, or (op_tile_rows = 0) whichever is fastest on
+// this model: each candidate is timed over 3 launches (after one untimed) on
+// a vector of ones with the coefficients of a mid-range frequency.
+static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_opts &o, hipStream_t s) {
+  LevelData &L = *st.levels[0];
+  if (!st.op_tiles_wanted || !L.pad_rp.p) return;
+  std::vector<int32_t> perm, iperm, rp8;
+  L.perm.download(perm);
+  L.iperm.download(iperm);
+  L.pad_rp.download(rp8);
+  if (o.op_tile_rows > 0) {
+    build_tiles(L, perm, iperm, rp8, o.op_tile_rows, kTileMaxLines);
+    return;
+  }
+  if (o.op_tile_rows == -2) {
+    L.op_scalar = true;
+    return;
+  }
+
+  const int64_t nt = m.n * 64;
+  DArr<float> xv, yv;
+  xv.alloc(m.ctx, nt);
+  yv.alloc(m.ctx, nt);
+  std::vector<float> ones(nt, 1.0f);
+  ELP_HIP(hipMemcpyAsync(xv.p, ones.data(), sizeof(float) * nt, hipMemcpyHostToDevice, s));
+  std::vector<OpCoef> hc(32, make_coef(m, 2 * M_PI * 500.0, m.Z, 0.0));
+  DArr<OpCoef> dc;
+  dc.upload(m.ctx, hc);
+  HipEvent e0, e1;
+  auto time_it = [&]() {
+    launch_row_op<0>(L, dc.p, xv.p, nullptr, yv.p, s);
+    ELP_HIP(hipEventRecord(e0.e, s));
+    for (int k = 0; k < 3; k++) launch_row_op<0>(L, dc.p, xv.p, nullptr, yv.p, s);
+    ELP_HIP(hipEventRecord(e1.e, s));
+    ELP_HIP(hipEventSynchronize(e1.e));
+    float ms = 0;
+    ELP_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+    return ms;
+  };
+  float best = time_it();  // the row kernel
+  int best_rows = -1;
+  L.op_scalar = true;
+  {
+    const float ms = time_it();
+    if (ms < best) {
+      best = ms;
+      best_rows = -2;
+    }
+  }
+  L.op_scalar = false;
+  for (int rows : {16, 32, 64}) {
+    build_tiles(L, perm, iperm, rp8, rows, kTileMaxLines);
+    const float ms = time_it();
+    if (ms < best) {
+      best = ms;
+      best_rows = rows;
+    }
+  }
+  if (best_rows < 0) {
+    L.op_scalar = best_rows == -2;
+    L.n_tiles = 0;
+    L.t_row.release();
+    L.t_cptr.release();
+    L.t_col.release();
+    L.t_loff.release();
+  } else if (best_rows != L.t_rows) {
+    build_tiles(L, perm, iperm, rp8, best_rows, kTileMaxLines);
+  }
+}
 
 template <typename T, int B>
 struct Solver {
@@ -1724,10 +2039,8 @@ struct Solver {
     }
     if constexpr (std::is_same<TT, float>::value && B == 32) {
       if (L.pad_rp.p) {
-        hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
-                           L.template args<float>(), L.padded(), coef, reinterpret_cast<const float *>(x),
-                           reinterpret_cast<const float *>(b), reinterpret_cast<float *>(y));
-        ELP_HIP(hipGetLastError());
+        launch_row_op<MODE>(L, coef, reinterpret_cast<const float *>(x), reinterpret_cast<const float *>(b),
+                            reinterpret_cast<float *>(y), s);
         return;
       }
     }
@@ -2805,6 +3118,7 @@ struct RunSeg {
   const int *active = nullptr;
   int op_blk0 = 0, op_nblk = 0;  // operator workgroups (multiples of 8)
   int e_blk0 = 0;                // element-wise workgroups of 256 threads
+  int scalar = 0;                // the row kernel with the row's entries in SGPRs (same bits)
 };
 
 // segment of block b: the last segment whose first block is <= b
@@ -2866,8 +3180,12 @@ __global__ __launch_bounds__(256) void grp_op_kernel(const RunSeg<T> *__restrict
     if (q.pc.rp) {
       const int64_t r = __builtin_amdgcn_readfirstlane(local * (kRowWg / 64) + (int)(threadIdx.x >> 6));
       if (r >= q.n) return;
-      row_apply<0>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
-                   reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
+      if (q.scalar)
+        row_apply<0, false, true>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
+                                  reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
+      else
+        row_apply<0>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
+                     reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
       return;
     }
   }
@@ -2935,7 +3253,10 @@ struct GroupDriverT : GroupDriver {
       RunSeg<T> &q = h_segs[k];
       q.n = S.m.n;
       q.a = L.template args<T>();
-      if constexpr (std::is_same<T, float>::value && B == 32) q.pc = L.pad_rp.p ? L.padded() : PadCsr{};
+      if constexpr (std::is_same<T, float>::value && B == 32) {
+        q.pc = L.pad_rp.p ? L.padded() : PadCsr{};
+        q.scalar = L.op_scalar ? 1 : 0;
+      }
       q.coefA = S.coefA.p;
       q.coefP = S.coefP.p;
       q.ri = S.ri.p;
@@ -3109,6 +3430,10 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
     stats->refinements = refinements;
     stats->refactors = S.refactors;
     stats->refactor_ms = S.refactor_ms;
+    const LevelData &L0 = *st.levels[0];
+    stats->op_tile_rows = (std::is_same<T, float>::value && B == 32 && L0.pad_rp.p)
+                              ? (L0.n_tiles > 0 ? L0.t_rows : L0.op_scalar ? -2 : -1)
+                              : 0;
   }
 }
 

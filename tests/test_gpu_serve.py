@@ -34,10 +34,10 @@ def _model(ctx, kind, nel, Z):
     return GpuModel(ctx, MG.coupled_plate_cavity(n_xy=nel[0], n_plate_z=nel[1], n_cav_z=nel[2]))
 
 
-@pytest.mark.parametrize("budget_mb,slots", [(8192, 8), (1, 2)])
-def test_serve_alone_equals_mixed(ctx, budget_mb, slots):
+@pytest.mark.parametrize("budget_mb,slots,block", [(8192, 8, 8), (1, 2, 8), (8192, 8, 32)])
+def test_serve_alone_equals_mixed(ctx, budget_mb, slots, block):
     reqs = _requests()
-    opts = _lib.default_opts(block=8)
+    opts = _lib.default_opts(block=block)
     alone = []
     for kind, nel, Z, freqs in reqs:
         g = _model(ctx, kind, nel, Z)

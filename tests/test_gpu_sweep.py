@@ -291,3 +291,25 @@ def test_coupled_sweep_block_preconditioner(ctx, block):
     assert (rel64 <= 1e-8).all() and st64.unconverged == 0, rel64
     e64 = _check_fields(m, freqs, X64, 1e-5)
     print(f"[sweep] coupled fields, per-frequency FP64: {e64:.2e}", flush=True)
+
+
+def test_operator_variants_same_bits(ctx):
+    """The 32-slot FP32 operator's variants (row kernel, row entries in scalar
+    registers, LDS row tiles of 16, 32 and 64 rows) sum the same entries in
+    the same order: the sweeps are the same bits, and the set-up's timing
+    choice (op_tile_rows = 0) is one of them."""
+    m = MG.acoustic_cavity(16, 12, 10, quadratic=True)
+    g = GpuModel(ctx, m)
+    g.assemble()
+    freqs = MG.linear_frequencies(10, 2000, 48)
+    ref = None
+    for rows in (-1, -2, 16, 32, 64, 0):
+        X, rel, it, st = g.sweep(freqs, opts=_lib.default_opts(block=32, op_tile_rows=rows))
+        print(f"[sweep] operator variant {rows}: used {st.op_tile_rows}", flush=True)
+        assert (rel <= 1e-8).all()
+        assert st.op_tile_rows == rows if rows != 0 else st.op_tile_rows in (-1, -2, 16, 32, 64)
+        if ref is None:
+            ref = (X, rel, it)
+        assert np.array_equal(X.view(np.uint64), ref[0].view(np.uint64)), rows
+        assert np.array_equal(rel, ref[1]) and np.array_equal(it, ref[2]), rows
+
