@@ -352,54 +352,6 @@ __device__ inline void row_batches(const PadCsr &pc, int32_t base, int32_t p1, u
   }
 }
 
-// row_batches with the row's x offsets, K and M read by scalar loads into
-// SGPRs (constant address space: the row base is wave-uniform), 16 entries
-// per batch (8 when fewer remain): per entry one buffer load of x with the
-// offset as its SGPR soffset and two FMAs with SGPR operands, where the
-// vector form needs three v_readlane per entry besides.  The same entries
-// in the same order as row_batches: the same sums bit for bit.
-typedef const __attribute__((address_space(4))) uint32_t *ScalarU32;
-typedef const __attribute__((address_space(4))) float *ScalarF32;
-__device__ inline void row_batches_s(const PadCsr &pc, int32_t base, int32_t p1, __amdgpu_buffer_rsrc_t xr,
-                                     uint32_t vo, float &aK, float &aM) {
-  const ScalarU32 so = (ScalarU32)pc.off;
-  const ScalarF32 sk = (ScalarF32)pc.K, sm = (ScalarF32)pc.M;
-  for (; base + 16 <= p1; base += 16) {
-    uint32_t off[16];
-    float k[16], m[16], xv[16];
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-      off[u] = so[base + u];
-      k[u] = sk[base + u];
-      m[u] = sm[base + u];
-    }
-#pragma unroll
-    for (int u = 0; u < 16; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, off[u], 0));
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-      aK = fmaf(k[u], xv[u], aK);
-      aM = fmaf(m[u], xv[u], aM);
-    }
-  }
-  if (base < p1) {  // 8 entries left (rows are padded to multiples of 8)
-    uint32_t off[8];
-    float k[8], m[8], xv[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      off[u] = so[base + u];
-      k[u] = sk[base + u];
-      m[u] = sm[base + u];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) xv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, vo, off[u], 0));
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      aK = fmaf(k[u], xv[u], aK);
-      aM = fmaf(m[u], xv[u], aM);
-    }
-  }
-}
-
 // row_batches with x read from the tile's lines in LDS: the entries' LDS
 // line offsets, K and M loaded 64 at a time with coalesced vector loads and
 // broadcast with v_readlane
@@ -439,7 +391,7 @@ __device__ inline void row_batches_lds(const PadCsr &pc, const uint32_t *__restr
 // partner lane's part.  Entries are summed in row order.
 constexpr int kRowWg = 256;  // threads per workgroup: kRowWg / 64 consecutive rows
 
-template <int MODE, bool TILE = false, bool SCALAR = false>
+template <int MODE, bool TILE = false>
 __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const OpCoef *__restrict__ coef,
                                  const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
                                  int64_t r, int lane, const uint32_t *__restrict__ tloff = nullptr,
@@ -463,7 +415,6 @@ __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const
     const uint32_t vo = (uint32_t)lane * sizeof(float);
     const int32_t base = pc.rp[r], p1 = pc.rp[r + 1];
     if constexpr (TILE) row_batches_lds<32>(pc, tloff, base, p1, lds + lane, lane, aK, aM);
-    else if constexpr (SCALAR) row_batches_s(pc, __builtin_amdgcn_readfirstlane(base), __builtin_amdgcn_readfirstlane(p1), xr, vo, aK, aM);
     else row_batches<32>(pc, base, p1, (uint32_t)xbytes, xr, vo, lane, aK, aM);
     const float *xl = x + lane;
     for (int32_t k = a.xrp[r]; k < a.xrp[r + 1]; k++) {
@@ -502,13 +453,13 @@ __device__ inline void row_apply(const OpArgs<float> &a, const PadCsr &pc, const
   y[o] = xo + (bv - res) * inv;
 }
 
-template <int MODE, bool SCALAR = false>
+template <int MODE>
 __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr pc, const OpCoef *__restrict__ coef,
                                                      const float *__restrict__ x, const float *__restrict__ b,
                                                      float *__restrict__ y) {
   const int64_t r = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
   if (r >= a.n) return;
-  row_apply<MODE, false, SCALAR>(a, pc, coef, x, b, y, r, threadIdx.x & 63);
+  row_apply<MODE>(a, pc, coef, x, b, y, r, threadIdx.x & 63);
 }
 
 // The row operator on row tiles: a workgroup copies the x lines of its
@@ -1180,7 +1131,6 @@ struct LevelData {
   DArr<uint32_t> t_loff;
   int64_t n_tiles = 0;
   int t_rows = 0, t_lines = 0;  // rows per tile (before splits), most lines of a tile
-  bool op_scalar = false;        // row kernel with the row's entries in SGPRs
   TileCsr tiles() const { return {t_row.p, t_cptr.p, t_col.p, t_loff.p, n_tiles}; }
   // box cycle: K P, M P, C_Gamma P (row-padded, permuted rows, lattice
   // columns as byte offsets) and P rows padded to 8 (box_post_kernel)
@@ -1879,9 +1829,6 @@ static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x
     hipLaunchKernelGGL((tile_op_kernel<MODE>), dim3(xcd_grid(L.n_tiles)), dim3(kTileWg),
                        (size_t)(L.t_lines + 1) * 256, s, L.template args<float>(), L.padded(), L.tiles(), coef, x, b,
                        y);
-  } else if (L.op_scalar) {
-    hipLaunchKernelGGL((row_op_kernel<MODE, true>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
-                       L.template args<float>(), L.padded(), coef, x, b, y);
   } else {
     hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
                        L.template args<float>(), L.padded(), coef, x, b, y);
@@ -1902,10 +1849,6 @@ static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_op
   L.pad_rp.download(rp8);
   if (o.op_tile_rows > 0) {
     build_tiles(L, perm, iperm, rp8, o.op_tile_rows, kTileMaxLines);
-    return;
-  }
-  if (o.op_tile_rows == -2) {
-    L.op_scalar = true;
     return;
   }
 
@@ -1931,15 +1874,6 @@ static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_op
   };
   float best = time_it();  // the row kernel
   int best_rows = -1;
-  L.op_scalar = true;
-  {
-    const float ms = time_it();
-    if (ms < best) {
-      best = ms;
-      best_rows = -2;
-    }
-  }
-  L.op_scalar = false;
   for (int rows : {16, 32, 64}) {
     build_tiles(L, perm, iperm, rp8, rows, kTileMaxLines);
     const float ms = time_it();
@@ -1949,7 +1883,6 @@ static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_op
     }
   }
   if (best_rows < 0) {
-    L.op_scalar = best_rows == -2;
     L.n_tiles = 0;
     L.t_row.release();
     L.t_cptr.release();
@@ -3118,7 +3051,6 @@ struct RunSeg {
   const int *active = nullptr;
   int op_blk0 = 0, op_nblk = 0;  // operator workgroups (multiples of 8)
   int e_blk0 = 0;                // element-wise workgroups of 256 threads
-  int scalar = 0;                // the row kernel with the row's entries in SGPRs (same bits)
 };
 
 // segment of block b: the last segment whose first block is <= b
@@ -3180,12 +3112,8 @@ __global__ __launch_bounds__(256) void grp_op_kernel(const RunSeg<T> *__restrict
     if (q.pc.rp) {
       const int64_t r = __builtin_amdgcn_readfirstlane(local * (kRowWg / 64) + (int)(threadIdx.x >> 6));
       if (r >= q.n) return;
-      if (q.scalar)
-        row_apply<0, false, true>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
-                                  reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
-      else
-        row_apply<0>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
-                     reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
+      row_apply<0>(q.a, q.pc, q.coefA, reinterpret_cast<const float *>(q.pi), nullptr,
+                   reinterpret_cast<float *>(q.qi), r, threadIdx.x & 63);
       return;
     }
   }
@@ -3255,7 +3183,6 @@ struct GroupDriverT : GroupDriver {
       q.a = L.template args<T>();
       if constexpr (std::is_same<T, float>::value && B == 32) {
         q.pc = L.pad_rp.p ? L.padded() : PadCsr{};
-        q.scalar = L.op_scalar ? 1 : 0;
       }
       q.coefA = S.coefA.p;
       q.coefP = S.coefP.p;
@@ -3432,7 +3359,7 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
     stats->refactor_ms = S.refactor_ms;
     const LevelData &L0 = *st.levels[0];
     stats->op_tile_rows = (std::is_same<T, float>::value && B == 32 && L0.pad_rp.p)
-                              ? (L0.n_tiles > 0 ? L0.t_rows : L0.op_scalar ? -2 : -1)
+                              ? (L0.n_tiles > 0 ? L0.t_rows : -1)
                               : 0;
   }
 }

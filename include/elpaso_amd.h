@@ -97,9 +97,8 @@ typedef struct elp_solve_opts {
   int mg_levels;     /* MG: maximum number of levels (0 = automatic)      */
   int mg_min_nodes;  /* MG: stop coarsening below this many nodes         */
   int op_tile_rows;  /* 32-slot FP32 operator: rows per LDS tile (16, 32, */
-                     /* 64), -1 = row kernel, -2 = row kernel with the    */
-                     /* row's entries in scalar registers, 0 = chosen at  */
-                     /* set-up by timing the candidates on the model      */
+                     /* 64), -1 = row kernel, 0 = chosen at set-up by     */
+                     /* timing the candidates on the model                */
 } elp_solve_opts;
 
 typedef struct elp_sweep_stats {
@@ -113,8 +112,7 @@ typedef struct elp_sweep_stats {
   int refactors;       /* coupled: factorizations of the structure block  */
   double refactor_ms;  /* coupled: time spent in them                     */
   int op_tile_rows;    /* fine-level operator used: rows per tile, -1 row */
-                       /* kernel, -2 scalar-entry row kernel, 0 another   */
-                       /* operator (other slot counts)                    */
+                       /* kernel, 0 another operator (other slot counts)  */
 } elp_sweep_stats;
 
 /* ---- context -------------------------------------------------------------- */
