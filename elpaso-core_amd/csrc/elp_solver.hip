@@ -663,6 +663,13 @@ __global__ __launch_bounds__(256) void transfer_kernel(int64_t n, const int32_t 
 // taking rows rl, rl + R, ... (R = 256 / B) in order, four loads in flight;
 // the per-block partials are combined by dot_final_kernel in a fixed tree.
 constexpr int kRedBlocks = 2048;
+// blocks of a reduction over n rows: at most kRedBlocks, at least 64 rows
+// per block (small models would otherwise spend the reduction on a final
+// sum over thousands of near-empty partials); a fixed function of n, so the
+// partition, and with it every sum, is the same for the same model
+static inline int red_blocks(int64_t n) {
+  return (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(8, (n + 63) / 64));
+}
 constexpr int kRedFinalThreads = 1024;
 
 template <typename V>
@@ -867,6 +874,12 @@ __device__ inline void beta_body(int B, const double2 *rho_new, double2 *rho, co
 }
 __global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, const int *active, double2 *beta) {
   beta_body(B, rho_new, rho, active, beta);
+}
+
+// after p = z + beta p: columns that (re)started (2) iterate from now on (1)
+__global__ void clear_restart_kernel(int B, int *active) {
+  const int k = threadIdx.x;
+  if (k < B && active[k] == 2) active[k] = 1;
 }
 
 // p = z + beta p
@@ -1187,6 +1200,12 @@ struct elp::Model::SolverState {
   bool fp64_inner = false;
   int op_tile_rows = 0;  // requested (the cache key)
   bool op_tiles_wanted = false;
+  // the last sweep's solver of a small model (work vectors, captured
+  // iteration graph), kept for the next sweep with the same options;
+  // declared last: destroyed before the levels it refers to
+  std::unique_ptr<void, void (*)(void *)> kept_solver{nullptr, nullptr};
+  int kept_kind = -1;
+  elp_solve_opts kept_opts{};
 };
 
 namespace elp {
@@ -1205,6 +1224,7 @@ void default_opts(elp_solve_opts &o) {
   o.mg_levels = 0;
   o.mg_min_nodes = 2000;
   o.op_tile_rows = 0;
+  o.graph = 0;
 }
 
 static void prepare_level(LevelData &L, hipStream_t s) {
@@ -1819,6 +1839,17 @@ struct LevelWork {
 
 static OpCoef make_coef(const Model &m, double w, std::complex<double> Z, double shift);
 
+static bool same_opts(const elp_solve_opts &a, const elp_solve_opts &b) {
+  return a.precision == b.precision && a.method == b.method && a.precond == b.precond && a.block == b.block &&
+         a.rtol == b.rtol && a.max_iters == b.max_iters && a.inner_rtol == b.inner_rtol && a.shift == b.shift &&
+         a.smooth == b.smooth && a.coarse_sweeps == b.coarse_sweeps && a.mg_levels == b.mg_levels &&
+         a.mg_min_nodes == b.mg_min_nodes && a.op_tile_rows == b.op_tile_rows && a.graph == b.graph;
+}
+
+// models up to this many rows replay their FP32 iterations from a captured
+// HIP graph by default (their kernels are short next to launch overhead)
+constexpr int64_t kGraphAutoRows = 100000;
+
 // Launch of the 32-slot FP32 operator of a level: on its row tiles when it
 // has them, else the row kernel (both give the same bits)
 constexpr int kTileMaxLines = kTileStage * kTileWg / 16 - 1;  // 64 KB of LDS per tile with the zero line
@@ -1898,7 +1929,7 @@ struct Solver {
   using V = typename VT<T>::t;
   Model &m;
   Model::SolverState &st;
-  const elp_solve_opts &o;
+  elp_solve_opts o;
   hipStream_t s;
   Ctx *ctx;
   std::vector<LevelWork<T>> lw;
@@ -1930,6 +1961,12 @@ struct Solver {
   int64_t spmv_calls = 0;
   bool timing = true;
   const V *u_of = nullptr;  // level-0 x holds D^-1 u_of (first smoothing sweep)
+  // one COCG iteration captured in a HIP graph (FP32 sweeps of launch-bound
+  // models): replayed instead of launching its ~25 kernels one by one
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_replays = 0;
+  bool use_graph = false;
 
   Solver(Model &mm, Model::SolverState &ss, const elp_solve_opts &oo, hipStream_t ss2)
       : m(mm), st(ss), o(oo), s(ss2), ctx(mm.ctx) {
@@ -2001,34 +2038,34 @@ struct Solver {
   // per-frequency reduction into h_red (host, after sync)
   template <typename VV>
   void reduce(int64_t n, const VV *a, const VV *b, int mode, double2 *dev_out) {
-    hipLaunchKernelGGL(dot_kernel<VV>, dim3(kRedBlocks), dim3(256), 0, s, n, B, a, b, mode, red_part.p);
-    final_reduce(dev_out);
+    hipLaunchKernelGGL(dot_kernel<VV>, dim3(red_blocks(n)), dim3(256), 0, s, n, B, a, b, mode, red_part.p);
+    final_reduce(dev_out, red_blocks(n));
   }
   // ||v||^2 and ||D^-1/2 v||^2 per frequency into dev_out (.x, .y): v in
   // the inner solve's row order, or (wreduce64) an FP64 vector in the
   // model's row order
   void wreduce(const V *v, double2 *dev_out) {
-    hipLaunchKernelGGL((wnorm_kernel<T, V>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, v,
+    hipLaunchKernelGGL((wnorm_kernel<T, V>), dim3(red_blocks(m.n)), dim3(256), 0, s, m.n, B, v,
                        st.levels[0]->template args<T>(), coefA.p, red_part.p);
-    final_reduce(dev_out);
+    final_reduce(dev_out, red_blocks(m.n));
   }
   void wreduce64(const double2 *v, double2 *dev_out) {
-    hipLaunchKernelGGL((wnorm_kernel<double, double2>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, v,
+    hipLaunchKernelGGL((wnorm_kernel<double, double2>), dim3(red_blocks(m.n)), dim3(256), 0, s, m.n, B, v,
                        st.levels[0]->template args<double>(), coefA.p, red_part.p);
-    final_reduce(dev_out);
+    final_reduce(dev_out, red_blocks(m.n));
   }
-  void final_reduce(double2 *dev_out) {
-    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kRedFinalThreads), 0, s, kRedBlocks, B, red_part.p, dev_out);
+  void final_reduce(double2 *dev_out, int nblocks) {
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kRedFinalThreads), 0, s, nblocks, B, red_part.p, dev_out);
     ELP_HIP(hipGetLastError());
   }
   // x += alpha p, r -= alpha q, red_out = |r|^2 per frequency (level 0);
   // with track_u, also the preconditioner's first sweep u = D^-1 r into the
   // level-0 work vector x (box_cycle then skips it)
   void update_norm(bool track_u = false) {
-    hipLaunchKernelGGL((cocg_update_norm_kernel<T>), dim3(kRedBlocks), dim3(256), 0, s, m.n, B, alpha.p, pi.p, qi.p,
+    hipLaunchKernelGGL((cocg_update_norm_kernel<T>), dim3(red_blocks(m.n)), dim3(256), 0, s, m.n, B, alpha.p, pi.p, qi.p,
                        xi.p, ri.p, red_part.p, st.levels[0]->template args<T>(), coefP.p, coefA.p,
                        track_u ? lw[0].x.p : nullptr);
-    final_reduce(red_out.p);
+    final_reduce(red_out.p, red_blocks(m.n));
     u_of = track_u ? ri.p : nullptr;
   }
   // the fused box cycle (32 frequencies, Hex20 box with products A P) is used
@@ -2160,8 +2197,23 @@ struct Solver {
     refactors++;
     return true;
   }
+  // ready for another sweep on stream ss (a solver kept from the last one)
+  void rearm(hipStream_t ss) {
+    join_prefetch();
+    s = ss;
+    spmv_ms = 0;
+    spmv_calls = 0;
+    graph_replays = 0;
+    refactors = 0;
+    refactor_ms = 0;
+    fac_hz = -1;
+    next_hz = -1;
+    u_of = nullptr;
+  }
   ~Solver() {
     join_prefetch();
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (s2) (void)hipStreamDestroy(s2);
   }
 
@@ -2839,15 +2891,20 @@ struct SlotRun {
     }
     return !done;
   }
-  // after p = z + beta p: columns that (re)started iterate from now on
-  void restart_flags() {
+  // after p = z + beta p: columns that (re)started iterate from now on (the
+  // device flags by clear_restart_kernel, unless a captured iteration does
+  // it: device = false)
+  void restart_flags(bool device = true) {
     bool restarted = false;
     for (int k = 0; k < B; k++)
       if (act[k] == 2) {
         act[k] = 1;
         restarted = true;
       }
-    if (restarted) ELP_HIP(hipMemcpyAsync(S.active.p, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    if (restarted && device) {
+      hipLaunchKernelGGL(clear_restart_kernel, dim3(1), dim3(64), 0, s, B, S.active.p);
+      ELP_HIP(hipGetLastError());
+    }
   }
   // host side of an iteration, with ||r_f||^2 of every column in S.h_red:
   // frozen columns take an FP64 refinement step together, finished
@@ -2989,8 +3046,11 @@ void SlotRun<T, B>::after_norms() {
 // one COCG iteration of a run on its own: z = M r, rho' = r.z, beta (0 for
 // restarting columns), p = z + beta p; q = A p, alpha = rho / p.q, x +=
 // alpha p, r -= alpha q, ||r||^2 per column to the host
+// the iteration's launches, in the order of run_iteration, with the restart
+// flags cleared on the device; graph capture of them ends with the copy of
+// the norms to the host (no synchronization inside a capture)
 template <typename T, int B>
-static void run_iteration(SlotRun<T, B> &R) {
+static void iteration_body(SlotRun<T, B> &R, bool events) {
   using V = typename VT<T>::t;
   Solver<T, B> &S = R.S;
   hipStream_t s = S.s;
@@ -2999,16 +3059,58 @@ static void run_iteration(SlotRun<T, B> &R) {
   S.reduce(n, S.ri.p, S.zi.p, 0, S.rho_new.p);
   hipLaunchKernelGGL(cocg_beta_kernel, dim3(1), dim3(64), 0, s, B, S.rho_new.p, S.rho.p, S.active.p, S.beta.p);
   hipLaunchKernelGGL((cocg_dir_kernel<V>), dim3(nblk(nt, 256)), dim3(256), 0, s, nt, B, S.beta.p, S.zi.p, S.pi.p);
+  hipLaunchKernelGGL(clear_restart_kernel, dim3(1), dim3(64), 0, s, B, S.active.p);
   ELP_HIP(hipGetLastError());
-  R.restart_flags();
-  if (S.timing) ELP_HIP(hipEventRecord(S.ev0.e, s));
+  if (events) ELP_HIP(hipEventRecord(S.ev0.e, s));
   S.template op<0>(0, S.coefA.p, S.pi.p, nullptr, S.qi.p);
-  if (S.timing) ELP_HIP(hipEventRecord(S.ev1.e, s));
-  S.spmv_calls++;
+  if (events) ELP_HIP(hipEventRecord(S.ev1.e, s));
   S.reduce(n, S.pi.p, S.qi.p, 0, S.mu.p);
   hipLaunchKernelGGL(cocg_alpha_kernel, dim3(1), dim3(64), 0, s, B, S.rho.p, S.mu.p, S.active.p, S.alpha.p);
   S.update_norm(S.fused_box());
-  S.fetch(S.red_out.p);
+  ELP_HIP(hipMemcpyAsync(S.h_red.p, S.red_out.p, sizeof(double2) * B, hipMemcpyDeviceToHost, s));
+}
+
+// One COCG iteration from the captured graph: the first iteration in the
+// steady state (the preconditioner's first sweep kept by the update when the
+// fused box cycle runs) is captured, later ones replay it.  The same kernels
+// on the same buffers as run_iteration: the same bits.
+template <typename T, int B>
+static bool run_iteration_graph(SlotRun<T, B> &R) {
+  Solver<T, B> &S = R.S;
+  if (S.fused_box() && S.u_of != S.ri.p) return false;  // not yet steady: run it eagerly
+  hipStream_t s = S.s;
+  if (!S.gexec) {
+    ELP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      iteration_body(R, false);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    ELP_HIP(hipStreamEndCapture(s, &S.graph));
+    ELP_HIP(hipGraphInstantiate(&S.gexec, S.graph, nullptr, nullptr, 0));
+  }
+  ELP_HIP(hipGraphLaunch(S.gexec, s));
+  ELP_HIP(hipStreamSynchronize(s));
+  R.restart_flags(false);  // the graph cleared them on the device
+  S.u_of = S.fused_box() ? S.ri.p : nullptr;
+  S.spmv_calls++;
+  S.graph_replays++;
+  return true;
+}
+
+template <typename T, int B>
+static void run_iteration(SlotRun<T, B> &R) {
+  Solver<T, B> &S = R.S;
+  if constexpr (std::is_same<T, float>::value) {
+    if (S.use_graph && run_iteration_graph(R)) return;
+  }
+  iteration_body(R, S.timing);
+  ELP_HIP(hipStreamSynchronize(S.s));
+  R.restart_flags(false);  // cleared on the device by the iteration
+  S.spmv_calls++;
   if (S.timing) {
     float ms = 0;
     ELP_HIP(hipEventElapsedTime(&ms, S.ev0.e, S.ev1.e));
@@ -3051,29 +3153,30 @@ struct RunSeg {
   const int *active = nullptr;
   int op_blk0 = 0, op_nblk = 0;  // operator workgroups (multiples of 8)
   int e_blk0 = 0;                // element-wise workgroups of 256 threads
+  int r_blk0 = 0, r_nblk = 0;    // reduction workgroups (red_blocks(n))
 };
 
 // segment of block b: the last segment whose first block is <= b
 template <typename T>
-__device__ inline int seg_of(const RunSeg<T> *__restrict__ segs, int G, int b, bool op) {
+__device__ inline int seg_of(const RunSeg<T> *__restrict__ segs, int G, int b, int which) {
   int g = 0;
   for (int k = 1; k < G; k++)
-    if ((op ? segs[k].op_blk0 : segs[k].e_blk0) <= b) g = k;
+    if ((which == 0 ? segs[k].op_blk0 : which == 1 ? segs[k].e_blk0 : segs[k].r_blk0) <= b) g = k;
   return g;
 }
 
 template <typename T, int B, int WHICH>
-__global__ __launch_bounds__(256) void grp_dot_kernel(const RunSeg<T> *__restrict__ segs) {
-  const RunSeg<T> &q = segs[blockIdx.x / kRedBlocks];
-  const int blk = blockIdx.x % kRedBlocks;
-  if (WHICH == 0) dot_body(q.n, B, q.ri, q.zi, 0, q.red_part, blk, kRedBlocks);
-  else dot_body(q.n, B, q.pi, q.qi, 0, q.red_part, blk, kRedBlocks);
+__global__ __launch_bounds__(256) void grp_dot_kernel(const RunSeg<T> *__restrict__ segs, int G) {
+  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, 2)];
+  const int blk = (int)blockIdx.x - q.r_blk0;
+  if (WHICH == 0) dot_body(q.n, B, q.ri, q.zi, 0, q.red_part, blk, q.r_nblk);
+  else dot_body(q.n, B, q.pi, q.qi, 0, q.red_part, blk, q.r_nblk);
 }
 
 template <typename T, int B, int WHICH>
 __global__ __launch_bounds__(kRedFinalThreads) void grp_final_kernel(const RunSeg<T> *__restrict__ segs) {
   const RunSeg<T> &q = segs[blockIdx.x];
-  dot_final_body(kRedBlocks, B, q.red_part, WHICH == 0 ? q.rho_new : WHICH == 1 ? q.mu : q.red_out);
+  dot_final_body(q.r_nblk, B, q.red_part, WHICH == 0 ? q.rho_new : WHICH == 1 ? q.mu : q.red_out);
 }
 
 template <typename T, int B>
@@ -3090,23 +3193,23 @@ __global__ void grp_alpha_kernel(const RunSeg<T> *__restrict__ segs) {
 
 template <typename T, int B>
 __global__ __launch_bounds__(256) void grp_dir_kernel(const RunSeg<T> *__restrict__ segs, int G) {
-  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, false)];
+  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, 1)];
   const int64_t i = (int64_t)(blockIdx.x - q.e_blk0) * blockDim.x + threadIdx.x;
   dir_body(q.n * B, B, q.beta, q.zi, q.pi, i);
 }
 
 template <typename T, int B>
-__global__ __launch_bounds__(256) void grp_update_kernel(const RunSeg<T> *__restrict__ segs) {
-  const RunSeg<T> &q = segs[blockIdx.x / kRedBlocks];
+__global__ __launch_bounds__(256) void grp_update_kernel(const RunSeg<T> *__restrict__ segs, int G) {
+  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, 2)];
   update_norm_body<T>(q.n, B, q.alpha, q.pi, q.qi, q.xi, q.ri, q.red_part, q.a, q.coefP, q.coefA, q.u,
-                      blockIdx.x % kRedBlocks, kRedBlocks);
+                      (int)blockIdx.x - q.r_blk0, q.r_nblk);
 }
 
 // q = A p of every segment: the row-per-wavefront FP32 operator where the
 // segment has its row-padded copy (32 slots), the chunked operator otherwise
 template <typename T, int B>
 __global__ __launch_bounds__(256) void grp_op_kernel(const RunSeg<T> *__restrict__ segs, int G) {
-  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, true)];
+  const RunSeg<T> &q = segs[seg_of(segs, G, blockIdx.x, 0)];
   const int local = xcd_map((int)blockIdx.x - q.op_blk0, q.op_nblk);
   if constexpr (std::is_same<T, float>::value && B == 32) {
     if (q.pc.rp) {
@@ -3155,7 +3258,7 @@ struct GroupDriverT : GroupDriver {
   DArr<RunSeg<T>> d_segs;
   std::vector<RunSeg<T>> h_segs;
   bool dirty = true;
-  int op_blocks = 0, e_blocks = 0;
+  int op_blocks = 0, e_blocks = 0, r_blocks = 0;
   HipEvent ev0, ev1;
   GroupDriverT(Ctx *c, hipStream_t st) : ctx(c), s(st) {}
 
@@ -3174,7 +3277,7 @@ struct GroupDriverT : GroupDriver {
 
   void rebuild() {
     h_segs.assign(runs.size(), RunSeg<T>());
-    int ob = 0, eb = 0;
+    int ob = 0, eb = 0, rb = 0;
     for (size_t k = 0; k < runs.size(); k++) {
       Solver<T, B> &S = *runs[k]->S;
       const LevelData &L = *S.st.levels[0];
@@ -3206,9 +3309,13 @@ struct GroupDriverT : GroupDriver {
       ob += q.op_nblk;
       q.e_blk0 = eb;
       eb += nblk(q.n * B, 256);
+      q.r_blk0 = rb;
+      q.r_nblk = red_blocks(q.n);
+      rb += q.r_nblk;
     }
     op_blocks = ob;
     e_blocks = eb;
+    r_blocks = rb;
     if ((int64_t)d_segs.n < (int64_t)h_segs.size()) d_segs.alloc(ctx, std::max<size_t>(h_segs.size(), 16));
     ELP_HIP(hipMemcpyAsync(d_segs.p, h_segs.data(), sizeof(RunSeg<T>) * h_segs.size(), hipMemcpyHostToDevice, s));
     dirty = false;
@@ -3236,7 +3343,7 @@ struct GroupDriverT : GroupDriver {
       ELP_HIP(hipEventRecord(g->ready->e, g->own));
       ELP_HIP(hipStreamWaitEvent(s, g->ready->e, 0));
     }
-    hipLaunchKernelGGL((grp_dot_kernel<T, B, 0>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
+    hipLaunchKernelGGL((grp_dot_kernel<T, B, 0>), dim3(r_blocks), dim3(256), 0, s, sg, G);
     hipLaunchKernelGGL((grp_final_kernel<T, B, 0>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
     hipLaunchKernelGGL((grp_beta_kernel<T, B>), dim3(G), dim3(64), 0, s, sg);
     hipLaunchKernelGGL((grp_dir_kernel<T, B>), dim3(e_blocks), dim3(256), 0, s, sg, G);
@@ -3251,10 +3358,10 @@ struct GroupDriverT : GroupDriver {
     ELP_HIP(hipEventRecord(ev0.e, s));
     hipLaunchKernelGGL((grp_op_kernel<T, B>), dim3(op_blocks), dim3(256), 0, s, sg, G);
     ELP_HIP(hipEventRecord(ev1.e, s));
-    hipLaunchKernelGGL((grp_dot_kernel<T, B, 1>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
+    hipLaunchKernelGGL((grp_dot_kernel<T, B, 1>), dim3(r_blocks), dim3(256), 0, s, sg, G);
     hipLaunchKernelGGL((grp_final_kernel<T, B, 1>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
     hipLaunchKernelGGL((grp_alpha_kernel<T, B>), dim3(G), dim3(64), 0, s, sg);
-    hipLaunchKernelGGL((grp_update_kernel<T, B>), dim3(G * kRedBlocks), dim3(256), 0, s, sg);
+    hipLaunchKernelGGL((grp_update_kernel<T, B>), dim3(r_blocks), dim3(256), 0, s, sg, G);
     hipLaunchKernelGGL((grp_final_kernel<T, B, 2>), dim3(G), dim3(kRedFinalThreads), 0, s, sg);
     ELP_HIP(hipGetLastError());
     for (auto &g : runs) {
@@ -3336,7 +3443,23 @@ template <typename T, int B>
 static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double *freqs, std::complex<double> Z,
                       const elp_solve_opts &o, double *X, double *relres, int32_t *iters, elp_sweep_stats *stats,
                       hipStream_t s) {
-  Solver<T, B> S(m, st, o, s);
+  // small models keep their solver (work vectors, captured graph) for the
+  // next sweep with the same options: their sweeps are bound by host work
+  const int kind = (std::is_same<T, float>::value ? 0 : 64) + B;
+  const bool keep = m.n <= kGraphAutoRows;
+  std::unique_ptr<Solver<T, B>> own;
+  Solver<T, B> *sp = nullptr;
+  if (keep && st.kept_solver && st.kept_kind == kind && same_opts(st.kept_opts, o)) {
+    own.reset(static_cast<Solver<T, B> *>(st.kept_solver.release()));  // back only if the sweep completes
+    own->rearm(s);
+  } else {
+    st.kept_solver.reset();
+    own.reset(new Solver<T, B>(m, st, o, s));
+  }
+  sp = own.get();
+  Solver<T, B> &S = *sp;
+  // launch-bound models replay each FP32 iteration from a captured graph
+  S.use_graph = std::is_same<T, float>::value && (o.graph > 0 || (o.graph == 0 && m.n <= kGraphAutoRows));
   int refinements = 0;
   if (std::is_same<T, float>::value || B > 1) {
     solve_slots<T, B>(S, nf, freqs, Z, X, relres, iters, refinements);
@@ -3355,12 +3478,19 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
     stats->spmv_bytes = m.nnz * (int64_t)(4 + 2 * sizeof(T)) + (m.n + 1) * 4 * 2 + m.n +
                         nx * (int64_t)(4 + 1 + sizeof(T)) + 2 * m.n * B * (int64_t)sizeof(V);
     stats->refinements = refinements;
+    stats->graph_replays = S.graph_replays;
     stats->refactors = S.refactors;
     stats->refactor_ms = S.refactor_ms;
     const LevelData &L0 = *st.levels[0];
     stats->op_tile_rows = (std::is_same<T, float>::value && B == 32 && L0.pad_rp.p)
                               ? (L0.n_tiles > 0 ? L0.t_rows : -1)
                               : 0;
+  }
+  if (keep) {
+    st.kept_solver = std::unique_ptr<void, void (*)(void *)>(
+        own.release(), [](void *p) { delete static_cast<Solver<T, B> *>(p); });
+    st.kept_kind = kind;
+    st.kept_opts = o;
   }
 }
 

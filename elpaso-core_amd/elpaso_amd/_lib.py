@@ -69,14 +69,14 @@ class SolveOpts(C.Structure):
                 ("rtol", C.c_double), ("max_iters", C.c_int), ("inner_rtol", C.c_double),
                 ("shift", C.c_double), ("smooth", C.c_int),
                 ("coarse_sweeps", C.c_int), ("mg_levels", C.c_int), ("mg_min_nodes", C.c_int),
-                ("op_tile_rows", C.c_int)]
+                ("op_tile_rows", C.c_int), ("graph", C.c_int)]
 
 
 class SweepStats(C.Structure):
     _fields_ = [("setup_ms", C.c_double), ("solve_ms", C.c_double), ("spmv_calls", C.c_int64),
                 ("spmv_ms", C.c_double), ("spmv_bytes", C.c_int64), ("refinements", C.c_int),
                 ("unconverged", C.c_int), ("refactors", C.c_int), ("refactor_ms", C.c_double),
-                ("op_tile_rows", C.c_int)]
+                ("op_tile_rows", C.c_int), ("graph_replays", C.c_int64)]
 
 
 def library_path():

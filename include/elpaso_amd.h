@@ -99,6 +99,9 @@ typedef struct elp_solve_opts {
   int op_tile_rows;  /* 32-slot FP32 operator: rows per LDS tile (16, 32, */
                      /* 64), -1 = row kernel, 0 = chosen at set-up by     */
                      /* timing the candidates on the model                */
+  int graph;         /* FP32 sweeps: each Krylov iteration replayed from  */
+                     /* a captured HIP graph: 1 on, -1 off, 0 for models  */
+                     /* up to 100,000 rows (launch-bound)                 */
 } elp_solve_opts;
 
 typedef struct elp_sweep_stats {
@@ -113,6 +116,7 @@ typedef struct elp_sweep_stats {
   double refactor_ms;  /* coupled: time spent in them                     */
   int op_tile_rows;    /* fine-level operator used: rows per tile, -1 row */
                        /* kernel, 0 another operator (other slot counts)  */
+  int64_t graph_replays; /* iterations replayed from the captured graph   */
 } elp_sweep_stats;
 
 /* ---- context -------------------------------------------------------------- */
