@@ -332,10 +332,11 @@ void finalize_model(Model &m) {
     rowlen[r] = off;
     std::vector<int32_t>().swap(lists[r]);
   }
-  // max row length the gather kernel supports (LDS accumulators); the fused
-  // Hex8 kernel's per-adjacency slot bytes need at most 256 neighbours too
+  // the assembly kernels address a node's neighbours by one-byte slots
+  // (rows themselves may be of any length)
   for (int64_t r = 0; r < N; r++)
-    ELP_REQUIRE(rowlen[r] <= 256 && deg[r] <= 256, ELP_EINVAL, "node with more than 256 coupled columns");
+    ELP_REQUIRE(deg[r] <= 256, ELP_EINVAL,
+                "node " + std::to_string(m.sorted_ids[r]) + " is coupled to more than 256 nodes");
 
   // ---- CSR
   m.rowptr.assign(m.n + 1, 0);

@@ -171,7 +171,8 @@ int elp_model_set_box(elp_model *m, const double *origin, const double *length,
 int elp_model_get_box(const elp_model *m, int *set, int *detected, double *origin, double *length,
                       int32_t *nel, int *order, int *robin_sides);
 /* DOF numbering, symbolic phase (sparsity pattern, adjacency), upload.
- * ELP_EINVAL for a node coupled to more than 256 neighbour nodes or columns,
+ * ELP_EINVAL for a node coupled to more than 256 nodes (rows may have any
+ * number of columns),
  * ELP_EDETJ (at assembly) for an element with detJ <= 0. */
 int elp_model_finalize(elp_model *m);
 int elp_model_info(const elp_model *m, int64_t *n_dof, int64_t *nnz, int64_t *device_bytes);

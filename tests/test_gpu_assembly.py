@@ -70,6 +70,38 @@ def test_brick20_block(ctx):
     _parity(ctx, m)
 
 
+@pytest.mark.parametrize("kind", [BRICK20, BRICK8])
+def test_rows_longer_than_256_columns(ctx, kind):
+    """Coarse elements around the centre node of a regular block, spanning
+    several cells in each octant with their nodes on the block's lattice:
+    that node is coupled to more than 86 nodes, rows of more than 256
+    columns, which elPaSo assembles like any other."""
+    quad = kind == BRICK20
+    n = 4 if quad else 8
+    nid, xyz, eid, conn, _ = MG.box(n, n, n, (0.01 * n, 0.01 * n, 0.01 * n), quadratic=quad)
+    step = 0.01 / (2 if quad else 1)  # lattice spacing
+    at = {tuple(np.rint(p / step).astype(int)): int(i) for p, i in zip(xyz, nid)}
+    nat = MG.HEX20_NAT if quad else MG.HEX8_NAT
+    c = n  # centre node's lattice index (n cells: 2n Hex20 lattice steps, n Hex8 ones)
+    if not quad:
+        c = n // 2
+    spans = (4,) if quad else (2, 3, 4)  # lattice steps per coarse element edge
+    extra = []
+    for span in spans:
+        for sx in (0, 1):
+            for sy in (0, 1):
+                for sz in (0, 1):
+                    o = np.array([c - span + span * sx, c - span + span * sy, c - span + span * sz])
+                    extra.append([at[tuple(o + (np.asarray(q) + 1) * span // 2)] for q in nat])
+    extra = np.array(extra, dtype=np.int64)
+    m = MG.Model(node_ids=nid, xyz=xyz, name=f"long-rows-{'brick20' if quad else 'brick8'}")
+    m.materials[1] = MG.STEEL
+    m.groups.append(MG.ElementGroup(kind, 1, eid, conn))
+    m.groups.append(MG.ElementGroup(kind, 1, np.arange(len(extra), dtype=np.int64) + int(eid.max()) + 1, extra))
+    g, num, (rp, ci), ref = _parity(ctx, m)
+    assert np.diff(rp).max() > 256, np.diff(rp).max()
+
+
 def test_coupled_small(ctx):
     m = MG.coupled_plate_cavity(n_xy=4, n_plate_z=2, n_cav_z=3)
     _parity(ctx, m, (_lib.MAT_K, _lib.MAT_M, _lib.MAT_S))
