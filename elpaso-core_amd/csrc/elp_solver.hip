@@ -3417,9 +3417,11 @@ std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *fre
   }
   Model::SolverState &st = solver_state(m, o, Z, setup);
   const bool dbl = st.fp64_inner;  // coupled models iterate in FP64 (see sweep)
-  // requests batch with others of their size class, so that a small
-  // request's step is not the length of a large one's
-  const int size_class = m.n <= 50000 ? 0 : m.n <= 500000 ? 1 : 2;
+  // requests batch with others of their size class (row counts within a
+  // factor of 4: floor(log4 n)), so that a small request's step is not the
+  // length of a large one's
+  int size_class = 0;
+  for (int64_t v = m.n; v >= 4; v /= 4) size_class++;
   *key = (dbl ? 4 : 0) + bi + 8 * size_class;
   return dbl ? make_group_run_b<double>(o.block, m, st, nf, freqs, Z, o, X, relres, iters, setup)
              : make_group_run_b<float>(o.block, m, st, nf, freqs, Z, o, X, relres, iters, setup);
