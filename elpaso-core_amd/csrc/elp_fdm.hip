@@ -62,9 +62,10 @@ std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid
     }
   if (n_fluid != 1 || (!fluid_part && m.mats.size() != 1)) return none;
   if (!fluid_part) {
-    if (!m.dir_rows.empty()) return none;
+    // fluid rows, and rows of prescribed pressures (left out of the
+    // correction by the interpolation)
     for (auto v : m.row_field)
-      if (v != 1) return none;
+      if (v == 0) return none;
   } else {
     // the box lattice covers exactly the fluid rows (a constrained fluid row
     // is not a fluid row: the count then differs)

@@ -1722,11 +1722,13 @@ static void build_interp(LevelData &F, const Model &coarse_like_box, const int n
   F.R32.upload(c, tv32);
 }
 
+// one fluid in a box (prescribed pressures allowed: the box solve then
+// leaves their rows out, and the rediscretized hierarchy is not used)
 static bool mg_possible(const Model &m) {
   if (!m.box.set || m.mats.size() != 1 || !m.mats.begin()->second.fluid) return false;
   for (auto v : m.row_field)
     if (v == 0) return false;
-  return m.dir_rows.empty();
+  return true;
 }
 
 // the 32-slot FP32 operator may run on row tiles
@@ -1791,10 +1793,12 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
     // prolongation from the Hex8 lattice of the box to the model's rows
     Model lattice;
     lattice.box = m.box;
-    build_interp(*st->levels[0], lattice, m.box.nel);
+    // rows of prescribed pressures take no share of the box correction
+    build_interp(*st->levels[0], lattice, m.box.nel, !m.dir_rows.empty());
     permute_level_float(*st->levels[0], s);
     if (m.box.order == 2) build_box_products(*st->levels[0], s);
-  } else if (!want_mg) {
+  } else if (!want_mg || !m.dir_rows.empty()) {
+    // l1-Jacobi (the rediscretized hierarchy knows no prescribed values)
     permute_level_float(*st->levels[0], s);
   } else if (want_mg) {
     int nel[3] = {m.box.nel[0], m.box.nel[1], m.box.nel[2]};
