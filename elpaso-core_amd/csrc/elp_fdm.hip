@@ -12,7 +12,6 @@
 
 namespace elp {
 
-constexpr int kFdmMaxAxis = 256;  // nodes along x and y of the box lattice
 
 struct FdmBox {
   int dims[3];  // px, py, pz (nodes per direction)
@@ -50,8 +49,12 @@ static void modes_1d(int n, double h, std::vector<double> &V, std::vector<double
 }
 
 std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid_part) {
+  return fdm_build(m, m.box, fluid_part);
+}
+
+std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, const BoxInfo &bx, bool fluid_part) {
   std::unique_ptr<FdmBox, void (*)(FdmBox *)> none(nullptr, fdm_free);
-  if (!m.box.set) return none;
+  if (!bx.set) return none;
   // the box's fluid: the model's one fluid material
   const Material *fluid = nullptr;
   int n_fluid = 0;
@@ -71,28 +74,29 @@ std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid
     // is not a fluid row: the count then differs)
     int64_t nf = 0;
     for (auto v : m.row_field) nf += v == 1;
-    const int64_t lattice = (int64_t)(m.box.nel[0] + 1) * (m.box.nel[1] + 1) * (m.box.nel[2] + 1);
-    if (m.box.order != 1 || nf != lattice) return none;
+    const int64_t lattice = (int64_t)(bx.nel[0] + 1) * (bx.nel[1] + 1) * (bx.nel[2] + 1);
+    if (bx.order != 1 || nf != lattice) return none;
     for (auto &kb : m.kinds)
       if (is_fluid(kb.kind) && kb.kind != ELP_FLUID8) return none;
   }
-  if (m.box.robin_sides & ~((1 << 2) | (1 << 4))) return none;  // Robin only on z sides
-  if (m.box.nel[0] + 1 > kFdmMaxAxis || m.box.nel[1] + 1 > kFdmMaxAxis) return none;
-  // the model's corner nodes must form the uniform lattice of the box
+  if (bx.robin_sides & ~((1 << 2) | (1 << 4))) return none;  // Robin only on z sides
+  if (bx.nel[0] + 1 > kFdmMaxAxis || bx.nel[1] + 1 > kFdmMaxAxis) return none;
+  // the uniform lattice of the box (the model's rows are mapped onto it by
+  // the solver's interpolation)
   auto f = std::unique_ptr<FdmBox, void (*)(FdmBox *)>(new FdmBox(), fdm_free);
   for (int d = 0; d < 3; d++) {
-    f->dims[d] = m.box.nel[d] + 1;
-    f->h[d] = m.box.length[d] / m.box.nel[d];
+    f->dims[d] = bx.nel[d] + 1;
+    f->h[d] = bx.length[d] / bx.nel[d];
   }
   f->rho = fluid->rho;
   f->c = fluid->c;
-  f->robin_lo = (m.box.robin_sides >> 2) & 1;
-  f->robin_hi = (m.box.robin_sides >> 4) & 1;
+  f->robin_lo = (bx.robin_sides >> 2) & 1;
+  f->robin_hi = (bx.robin_sides >> 4) & 1;
   Ctx *c = m.ctx;
   auto upload = [&](int d, DArr<float> &fw32, DArr<float> &iv32, DArr<double> &fw64, DArr<double> &iv64,
                     DArr<double> *lam) {
     std::vector<double> V, L;
-    modes_1d(m.box.nel[d], f->h[d], V, L);
+    modes_1d(bx.nel[d], f->h[d], V, L);
     const int p = f->dims[d];
     std::vector<double> Vt((size_t)p * p);
     for (int i = 0; i < p; i++)

@@ -1198,6 +1198,11 @@ struct elp::Model::SolverState {
   std::unique_ptr<elp::BcrPlan, void (*)(elp::BcrPlan *)> bcr{nullptr, nullptr};
   // models with structure and fluid rows iterate in FP64 (see sweep)
   bool fp64_inner = false;
+  // the box of the box solve: the model's, or for a fluid mesh that is no
+  // box a stand-in lattice over its bounding box (standin = true)
+  BoxInfo box;
+  bool standin = false;
+  int precond = -1;  // requested (the cache key)
   int op_tile_rows = 0;  // requested (the cache key)
   bool op_tiles_wanted = false;
   // the last sweep's solver of a small model (work vectors, captured
@@ -1746,6 +1751,59 @@ static bool two_fields(const Model &m) {
   return has_s && has_f;
 }
 
+// one fluid material and no structure rows: a mesh without a box of its own
+// gets a stand-in box for the two-level cycle
+static bool fluid_mesh(const Model &m) {
+  if (m.mats.size() != 1 || !m.mats.begin()->second.fluid) return false;
+  for (auto v : m.row_field)
+    if (v == 0) return false;
+  return true;
+}
+
+// The stand-in box of a fluid mesh: its bounding box with a uniform Hex8
+// lattice of about the mesh's element size ((box volume / elements)^(1/3)),
+// at most kFdmMaxAxis points across x and y, Robin faces on its z sides
+// where the mesh has Robin faces lying on them (others are left to the
+// smoothing steps and the Krylov iteration).
+static BoxInfo standin_box(const Model &m) {
+  BoxInfo b;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t r = 0; r < m.n_nodes; r++) {
+    if (m.node_row[r] < 0) continue;
+    const double *x = &m.xyz[(int64_t)m.rank_to_input[r] * 3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], x[d]);
+      hi[d] = std::max(hi[d], x[d]);
+    }
+  }
+  int64_t ne = 0;
+  for (auto &kb : m.kinds) ne += kb.n_elem;
+  double vol = 1;
+  for (int d = 0; d < 3; d++) vol *= hi[d] - lo[d];
+  if (!(vol > 0) || ne <= 0) return b;
+  const double h = std::cbrt(vol / (double)ne);
+  for (int d = 0; d < 3; d++) {
+    const int cap = d < 2 ? kFdmMaxAxis - 1 : 4096;
+    b.origin[d] = lo[d];
+    b.length[d] = hi[d] - lo[d];
+    b.nel[d] = (int)std::min<double>(cap, std::max(1.0, std::round(b.length[d] / h)));
+  }
+  const double tol = 1e-9 * b.length[2];
+  for (int64_t f = 0; f < m.n_faces; f++) {
+    bool on_lo = true, on_hi = true;
+    for (int k = 0; k < m.nfn; k++) {
+      const double z = m.xyz[(int64_t)m.rank_to_input[m.face_nodes[f * m.nfn + k]] * 3 + 2];
+      on_lo &= std::fabs(z - lo[2]) <= tol;
+      on_hi &= std::fabs(z - hi[2]) <= tol;
+    }
+    if (on_lo) b.robin_sides |= 1 << 2;
+    if (on_hi) b.robin_sides |= 1 << 4;
+  }
+  b.order = 2;  // the smoothing two-level cycle
+  b.set = true;
+  return b;
+}
+
 // a Hex8 fluid box (given or read from the mesh) coupled to structure rows
 static bool coupled_possible(const Model &m) { return m.box.set && m.box.order == 1 && two_fields(m); }
 
@@ -1756,11 +1814,13 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   const bool want_cpl = o.precond == ELP_PC_MG && !want_mg && coupled_possible(m);
   if (m.solver && m.solver->mg == want_mg && m.solver->mg_levels == o.mg_levels &&
       m.solver->mg_min_nodes == o.mg_min_nodes && m.solver->cpl_requested == want_cpl &&
-      m.solver->op_tile_rows == o.op_tile_rows && m.solver->op_tiles_wanted == tiles_wanted(o))
+      m.solver->op_tile_rows == o.op_tile_rows && m.solver->op_tiles_wanted == tiles_wanted(o) &&
+      m.solver->precond == o.precond)
     return *m.solver;
   m.solver.reset();
   auto st = std::make_shared<Model::SolverState>();
   st->mg = want_mg;
+  st->precond = o.precond;
   st->cpl_requested = want_cpl;
   st->mg_levels = o.mg_levels;
   st->mg_min_nodes = o.mg_min_nodes;
@@ -1771,8 +1831,19 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   L0->model = &m;
   prepare_level(*L0, s);
   st->levels.push_back(std::move(L0));
-  if (want_mg) st->fdm = fdm_build(m);
-  st->order = m.box.order;
+  st->box = m.box;
+  if (want_mg) {
+    st->fdm = fdm_build(m, st->box, false);
+  } else if (o.precond == ELP_PC_MG && fluid_mesh(m)) {
+    // a fluid mesh that is no box: the two-level cycle on a stand-in box
+    const BoxInfo b = standin_box(m);
+    if (b.set) st->fdm = fdm_build(m, b, false);
+    if (st->fdm) {
+      st->box = b;
+      st->standin = true;
+    }
+  }
+  st->order = st->box.order;
   if (want_cpl) {
     LevelData &L = *st->levels[0];
     st->fdm = fdm_build(m, true);
@@ -1792,11 +1863,11 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   } else if (st->fdm) {
     // prolongation from the Hex8 lattice of the box to the model's rows
     Model lattice;
-    lattice.box = m.box;
+    lattice.box = st->box;
     // rows of prescribed pressures take no share of the box correction
-    build_interp(*st->levels[0], lattice, m.box.nel, !m.dir_rows.empty());
+    build_interp(*st->levels[0], lattice, st->box.nel, !m.dir_rows.empty());
     permute_level_float(*st->levels[0], s);
-    if (m.box.order == 2) build_box_products(*st->levels[0], s);
+    if (st->box.order == 2) build_box_products(*st->levels[0], s);
   } else if (!want_mg || !m.dir_rows.empty()) {
     // l1-Jacobi (the rediscretized hierarchy knows no prescribed values)
     permute_level_float(*st->levels[0], s);
@@ -2399,7 +2470,7 @@ static void solve_block(Solver<T, B> &S, const double *freqs, int nb,
   std::vector<FdmCoef> cF(B);
   double hmax = 0;
   if (S.st.fdm)
-    for (int d = 0; d < 3; d++) hmax = std::max(hmax, m.box.length[d] / m.box.nel[d]);
+    for (int d = 0; d < 3; d++) hmax = std::max(hmax, S.st.box.length[d] / S.st.box.nel[d]);
   for (int f = 0; f < B; f++) {
     const double w = 2.0 * M_PI * freqs[std::min(f, nb - 1)];
     // automatic shift: none with the exact box solve while the mesh resolves
@@ -2715,7 +2786,7 @@ struct SlotRun {
     perm = use_perm ? L0.perm.p : nullptr;
     iperm = use_perm ? L0.iperm.p : nullptr;
     if (S.st.fdm)
-      for (int d = 0; d < 3; d++) hmax = std::max(hmax, m.box.length[d] / m.box.nel[d]);
+      for (int d = 0; d < 3; d++) hmax = std::max(hmax, S.st.box.length[d] / S.st.box.nel[d]);
     // slots left empty keep valid (unused) coefficients
     for (int k = 0; k < B; k++) set_coefs(k, 0);
     d_fmap.alloc(m.ctx, B);

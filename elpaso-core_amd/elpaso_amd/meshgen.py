@@ -116,6 +116,35 @@ def acoustic_cavity(nx=20, ny=16, nz=12, L=(1.0, 0.8, 0.6), quadratic=False,
     return m
 
 
+def irregular_cavity(nx=12, ny=10, nz=8, L=(1.0, 0.8, 0.6), quadratic=False, jitter=0.2, seed=0,
+                     l_shape=False, Z=None):
+    """An air cavity whose mesh is no uniform box: interior nodes moved by
+    U(-jitter, +jitter) element edges, and with l_shape the elements of the
+    quarter x > Lx/2, y > Ly/2 removed (an L-shaped floor plan).  Robin face
+    (Z, default rho c (1 + 0.5i)) on the top z = Lz, unit source near
+    (0.2, 0.3, 0.1); no box description."""
+    nid, xyz, eid, conn, lat = box(nx, ny, nz, L, quadratic=quadratic, jitter=jitter, seed=seed)
+    keep = np.ones(len(eid), dtype=bool)
+    if l_shape:
+        keep = ~((lat[:, 0] >= nx // 2) & (lat[:, 1] >= ny // 2))
+    eid, conn, lat = eid[keep], conn[keep], lat[keep]
+    used = np.unique(conn)
+    sel = np.isin(nid, used)
+    nid, xyz = nid[sel], xyz[sel]
+    kind = FLUID20 if quadratic else FLUID8
+    m = Model(node_ids=nid, xyz=xyz,
+              name=f"irregular_{'hex20' if quadratic else 'hex8'}_{nx}x{ny}x{nz}{'_L' if l_shape else ''}")
+    m.materials[1] = AIR
+    m.groups.append(ElementGroup(kind, 1, eid, conn))
+    top = lat[:, 2] == nz - 1
+    m.imp_elem_ids = eid[top]
+    m.imp_faces = np.full(int(top.sum()), FACE_PLUS_Z + 1, dtype=np.int32)
+    m.impedance = complex(AIR.rho * AIR.c * (1 + 0.5j)) if Z is None else complex(Z)
+    m.imp_lumped = not quadratic
+    m.sources.append((nearest_node(nid, xyz, (0.2, 0.3, 0.1)), 1.0))
+    return m
+
+
 def steel_block(nx=200, ny=200, nz=4, L=(1.0, 1.0, 0.02), jitter=0.1, seed=0,
                 rayleigh=(1e-5, 2.0)):
     """Linear-elastic steel block of Brick8 elements with seeded jitter of the
