@@ -375,7 +375,7 @@ def measure_peaks(copy_bytes=1 << 30, n=8192, repeats=5):
     return out
 
 
-def serve_stream(ctx, n_req, rate=20.0, seed=0, budget_gb=200, resident=8):
+def serve_stream(ctx, n_req, rate=20.0, seed=0, budget_gb=200, resident=16):
     """The serving workload: a seeded Poisson stream (mean 20 requests/s) of
     sweep requests, each an acoustic cavity of the Fluid8 20x16x12 family
     (1.0x0.8x0.6 m, Robin face, point source) or a coupled plate/cavity of
@@ -384,7 +384,7 @@ def serve_stream(ctx, n_req, rate=20.0, seed=0, budget_gb=200, resident=8):
     16-512 frequencies uniform in 10-2000 Hz and Z = rho c (a + bi), a, b
     uniform in [0.2, 2].  Models are built (not assembled) before the stream
     starts; each request is submitted at its arrival time to a server with a
-    200 GB HBM budget and up to 8 resident requests, which assembles and
+    200 GB HBM budget and up to 16 resident requests, which assembles and
     sets it up and batches its frequency slots with the other resident
     requests of its precision.  Latency = submit to result."""
     from elpaso_amd import GpuModel, Server
