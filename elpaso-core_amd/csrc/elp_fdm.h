@@ -31,9 +31,11 @@ constexpr int kFdmMaxAxis = 256;  // nodes along x and y of the box lattice
 // sides only; with fluid_part, the box describes the fluid rows of a model
 // that has other rows too (a Hex8 cavity coupled to a structure)
 std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid_part = false);
-// the same for a box description bx instead of the model's own (the
-// solver's stand-in box of a fluid mesh that is not a box)
-std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, const BoxInfo &bx, bool fluid_part);
+// the same for a box description bx instead of the model's own; mode 0:
+// the model is the box's fluid, 1: its fluid rows are exactly the box's
+// lattice (a Hex8 cavity beside a structure), 2: a stand-in box of the
+// model's one fluid (the solver maps the fluid rows onto its lattice)
+std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, const BoxInfo &bx, int mode);
 int64_t fdm_nodes(const FdmBox &f);
 const int *fdm_dims(const FdmBox &f);
 double fdm_rho(const FdmBox &f);

@@ -49,10 +49,10 @@ static void modes_1d(int n, double h, std::vector<double> &V, std::vector<double
 }
 
 std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, bool fluid_part) {
-  return fdm_build(m, m.box, fluid_part);
+  return fdm_build(m, m.box, fluid_part ? 1 : 0);
 }
 
-std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, const BoxInfo &bx, bool fluid_part) {
+std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, const BoxInfo &bx, int mode) {
   std::unique_ptr<FdmBox, void (*)(FdmBox *)> none(nullptr, fdm_free);
   if (!bx.set) return none;
   // the box's fluid: the model's one fluid material
@@ -63,13 +63,13 @@ std::unique_ptr<FdmBox, void (*)(FdmBox *)> fdm_build(const Model &m, const BoxI
       fluid = &kv.second;
       n_fluid++;
     }
-  if (n_fluid != 1 || (!fluid_part && m.mats.size() != 1)) return none;
-  if (!fluid_part) {
+  if (n_fluid != 1 || (mode == 0 && m.mats.size() != 1)) return none;
+  if (mode == 0) {
     // fluid rows, and rows of prescribed pressures (left out of the
     // correction by the interpolation)
     for (auto v : m.row_field)
       if (v == 0) return none;
-  } else {
+  } else if (mode == 1) {
     // the box lattice covers exactly the fluid rows (a constrained fluid row
     // is not a fluid row: the count then differs)
     int64_t nf = 0;

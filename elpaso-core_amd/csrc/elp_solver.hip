@@ -876,6 +876,14 @@ __global__ void cocg_beta_kernel(int B, const double2 *rho_new, double2 *rho, co
   beta_body(B, rho_new, rho, active, beta);
 }
 
+// x = 0 on the rows whose field is not `keep`
+template <typename V>
+__global__ void keep_field_kernel(int64_t nt, int B, const uint8_t *__restrict__ field, int keep, V *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt || field[i / B] == keep) return;
+  x[i] = V{0, 0};
+}
+
 // after p = z + beta p: columns that (re)started (2) iterate from now on (1)
 __global__ void clear_restart_kernel(int B, int *active) {
   const int k = threadIdx.x;
@@ -1760,7 +1768,8 @@ static bool fluid_mesh(const Model &m) {
   return true;
 }
 
-// The stand-in box of a fluid mesh: its bounding box with a uniform Hex8
+// The stand-in box of a fluid mesh (of a model's fluid): the bounding box of
+// its fluid nodes with a uniform Hex8
 // lattice of about the mesh's element size ((box volume / elements)^(1/3)),
 // at most kFdmMaxAxis points across x and y, Robin faces on its z sides
 // where the mesh has Robin faces lying on them (others are left to the
@@ -1769,7 +1778,7 @@ static BoxInfo standin_box(const Model &m) {
   BoxInfo b;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int64_t r = 0; r < m.n_nodes; r++) {
-    if (m.node_row[r] < 0) continue;
+    if (m.node_row[r] < 0 || m.row_field[m.node_row[r]] != 1) continue;  // fluid nodes
     const double *x = &m.xyz[(int64_t)m.rank_to_input[r] * 3];
     for (int d = 0; d < 3; d++) {
       lo[d] = std::min(lo[d], x[d]);
@@ -1777,7 +1786,8 @@ static BoxInfo standin_box(const Model &m) {
     }
   }
   int64_t ne = 0;
-  for (auto &kb : m.kinds) ne += kb.n_elem;
+  for (auto &kb : m.kinds)
+    if (is_fluid(kb.kind)) ne += kb.n_elem;
   double vol = 1;
   for (int d = 0; d < 3; d++) vol *= hi[d] - lo[d];
   if (!(vol > 0) || ne <= 0) return b;
@@ -1804,8 +1814,15 @@ static BoxInfo standin_box(const Model &m) {
   return b;
 }
 
-// a Hex8 fluid box (given or read from the mesh) coupled to structure rows
-static bool coupled_possible(const Model &m) { return m.box.set && m.box.order == 1 && two_fields(m); }
+// structure rows beside one fluid: the coupled preconditioner (the fluid
+// rows through the exact box solve of a Hex8 box given or read from the
+// mesh, or the two-level cycle on a stand-in box)
+static bool coupled_possible(const Model &m) {
+  if (!two_fields(m)) return false;
+  int n_fluid = 0;
+  for (auto &kv : m.mats) n_fluid += kv.second.fluid;
+  return n_fluid == 1;
+}
 
 static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_opts &o, hipStream_t s);
 
@@ -1833,11 +1850,11 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   st->levels.push_back(std::move(L0));
   st->box = m.box;
   if (want_mg) {
-    st->fdm = fdm_build(m, st->box, false);
+    st->fdm = fdm_build(m, st->box, 0);
   } else if (o.precond == ELP_PC_MG && fluid_mesh(m)) {
     // a fluid mesh that is no box: the two-level cycle on a stand-in box
     const BoxInfo b = standin_box(m);
-    if (b.set) st->fdm = fdm_build(m, b, false);
+    if (b.set) st->fdm = fdm_build(m, b, 0);
     if (st->fdm) {
       st->box = b;
       st->standin = true;
@@ -1846,11 +1863,20 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   st->order = st->box.order;
   if (want_cpl) {
     LevelData &L = *st->levels[0];
-    st->fdm = fdm_build(m, true);
+    if (m.box.set && m.box.order == 1) st->fdm = fdm_build(m, true);  // the cavity is the box
+    if (!st->fdm) {
+      const BoxInfo b = standin_box(m);
+      if (b.set) st->fdm = fdm_build(m, b, 2);
+      if (st->fdm) {
+        st->box = b;
+        st->standin = true;
+      }
+    }
+    st->order = st->box.order;
     if (st->fdm) {
       Model lattice;
-      lattice.box = m.box;
-      build_interp(L, lattice, m.box.nel, true);
+      lattice.box = st->box;
+      build_interp(L, lattice, st->box.nel, true);
     }
     permute_level_float(L, s);
     std::vector<int32_t> ip;
@@ -2223,6 +2249,13 @@ struct Solver {
       }
     }
     smooth(0, b, nullptr, nu, W.x.p);
+    if (st.coupled) {
+      // the fluid block alone: no share of the structure rows in A u (their
+      // part of z is the structure solve's)
+      hipLaunchKernelGGL((keep_field_kernel<V>), dim3(nblk(L.n * B, 256)), dim3(256), 0, s, L.n * B, B,
+                         st.levels[0]->template args<T>().field, 1, W.x.p);
+      ELP_HIP(hipGetLastError());
+    }
     op<1>(0, coefP.p, W.x.p, b, W.r.p);
     transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, fb.p);
     fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);

@@ -159,14 +159,17 @@ def steel_block(nx=200, ny=200, nz=4, L=(1.0, 1.0, 0.02), jitter=0.1, seed=0,
 
 
 def coupled_plate_cavity(n_xy=100, n_plate_z=2, n_cav_z=120, plate_t=0.005,
-                         cav_h=1.2, eta=0.01, seed=0):
+                         cav_h=1.2, eta=0.01, seed=0, jitter=0.0):
     """Steel plate (Brick8, clamped edges, loss factor eta) lying on top of an
     air cavity (Fluid8).  The plate's bottom face is the cavity's top face
     z = cav_h; the interface Quad4 elements pair coincident fluid and plate
     nodes (separate node ids); orientation +1 means the face normal a1 x a2
     points out of the fluid into the plate.  A unit force F_z = 1 acts at a
-    seeded random free node of the plate's top face."""
-    fid, fxyz, feid, fconn, _ = box(n_xy, n_xy, n_cav_z, (1.0, 1.0, cav_h))
+    seeded random free node of the plate's top face.  jitter > 0 moves the
+    cavity's interior nodes by U(-jitter, +jitter) element edges (its faces
+    stay planar): the cavity is then no uniform box and carries no box
+    description."""
+    fid, fxyz, feid, fconn, _ = box(n_xy, n_xy, n_cav_z, (1.0, 1.0, cav_h), jitter=jitter, seed=seed + 1)
     sid, sxyz, seid, sconn, _ = box(n_xy, n_xy, n_plate_z, (1.0, 1.0, plate_t),
                                     origin=(0.0, 0.0, cav_h),
                                     first_node_id=len(fid) + 1, first_elem_id=len(feid) + 1)
@@ -199,8 +202,9 @@ def coupled_plate_cavity(n_xy=100, n_plate_z=2, n_cav_z=120, plate_t=0.005,
     m.forces.append((int(cand[rng.integers(len(cand))]), 0.0, 0.0, 1.0))
     # the cavity is a uniform Hex8 box: the sweep's coupled preconditioner
     # solves its fluid rows exactly by fast diagonalization
-    m.box = dict(origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, cav_h), nel=(n_xy, n_xy, n_cav_z), order=1,
-                 robin_sides=0)
+    if jitter == 0:
+        m.box = dict(origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, cav_h), nel=(n_xy, n_xy, n_cav_z), order=1,
+                     robin_sides=0)
     return m
 
 
