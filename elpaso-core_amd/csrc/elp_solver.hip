@@ -1211,8 +1211,10 @@ struct elp::Model::SolverState {
   BoxInfo box;
   bool standin = false;
   int precond = -1;  // requested (the cache key)
-  int op_tile_rows = 0;  // requested (the cache key)
-  bool op_tiles_wanted = false;
+  // the 32-slot FP32 operator's variant, chosen at the first sweep that
+  // uses it (op_chosen) for the requested op_tile_rows
+  int op_tile_rows = 0;
+  bool op_chosen = false;
   // the last sweep's solver of a small model (work vectors, captured
   // iteration graph), kept for the next sweep with the same options;
   // declared last: destroyed before the levels it refers to
@@ -1831,7 +1833,6 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   const bool want_cpl = o.precond == ELP_PC_MG && !want_mg && coupled_possible(m);
   if (m.solver && m.solver->mg == want_mg && m.solver->mg_levels == o.mg_levels &&
       m.solver->mg_min_nodes == o.mg_min_nodes && m.solver->cpl_requested == want_cpl &&
-      m.solver->op_tile_rows == o.op_tile_rows && m.solver->op_tiles_wanted == tiles_wanted(o) &&
       m.solver->precond == o.precond)
     return *m.solver;
   m.solver.reset();
@@ -1842,8 +1843,6 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
   st->mg_levels = o.mg_levels;
   st->mg_min_nodes = o.mg_min_nodes;
   st->fp64_inner = two_fields(m);
-  st->op_tile_rows = o.op_tile_rows;
-  st->op_tiles_wanted = tiles_wanted(o) && !st->fp64_inner;
   auto L0 = std::make_unique<LevelData>();
   L0->model = &m;
   prepare_level(*L0, s);
@@ -1923,7 +1922,6 @@ static Model::SolverState &solver_state(Model &m, const elp_solve_opts &o, std::
       for (int d = 0; d < 3; d++) nel[d] = nc[d];
     }
   }
-  choose_operator(m, *st, o, s);
   ELP_HIP(hipStreamSynchronize(s));
   m.solver = st;
   return *st;
@@ -1974,7 +1972,17 @@ static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x
 // a vector of ones with the coefficients of a mid-range frequency.
 static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_opts &o, hipStream_t s) {
   LevelData &L = *st.levels[0];
-  if (!st.op_tiles_wanted || !L.pad_rp.p) return;
+  if (st.fp64_inner || !tiles_wanted(o) || !L.pad_rp.p) return;
+  if (st.op_chosen && st.op_tile_rows == o.op_tile_rows) return;
+  st.op_chosen = true;
+  st.op_tile_rows = o.op_tile_rows;
+  if (L.n_tiles > 0) {  // a variant chosen before under other options
+    L.n_tiles = 0;
+    L.t_row.release();
+    L.t_cptr.release();
+    L.t_col.release();
+    L.t_loff.release();
+  }
   std::vector<int32_t> perm, iperm, rp8;
   L.perm.download(perm);
   L.iperm.download(iperm);
@@ -3553,6 +3561,12 @@ template <typename T, int B>
 static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double *freqs, std::complex<double> Z,
                       const elp_solve_opts &o, double *X, double *relres, int32_t *iters, elp_sweep_stats *stats,
                       hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value && B == 32) {
+    if (!st.op_chosen || st.op_tile_rows != o.op_tile_rows) {
+      st.kept_solver.reset();  // its graph may hold the other variant's launches
+      choose_operator(m, st, o, s);
+    }
+  }
   // small models keep their solver (work vectors, captured graph) for the
   // next sweep with the same options: their sweeps are bound by host work
   const int kind = (std::is_same<T, float>::value ? 0 : 64) + B;
