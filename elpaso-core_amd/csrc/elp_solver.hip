@@ -306,8 +306,11 @@ __device__ inline double row_weight(const OpArgs<T> &a, const OpCoef *__restrict
   if (fld == 2) return 0.0;
   const OpCoef &cf = coef[f];
   const bool fl = fld == 1;
-  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
-  return d > 0 ? 1.0 / d : 0.0;
+  // a scale, not a value of the solution: FP32 and a hardware reciprocal
+  // (about 1 ulp) are enough, and keep the update kernel memory bound
+  const float d = (float)(fl ? cf.akf : cf.aks) * (float)a.rK[r] + (float)(fl ? cf.amf : cf.ams) * (float)a.rM[r] +
+                  (float)cf.ag * (float)a.rG[r];
+  return d > 0.f ? (double)__builtin_amdgcn_rcpf(d) : 0.0;
 }
 
 template <typename T>
