@@ -40,6 +40,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def operator_kernel(B, variant):
+    """Name of the level-0 FP32 operator kernel a sweep ran (the q = A p launch),
+    from the slot count and the operator variant in the sweep statistics."""
+    if B != 32:
+        return f"op_kernel<float, {B}, 0>"
+    if variant == -1:
+        return "row_op_kernel<0>"
+    if variant in (-2, -4):
+        return f"rgrp_op_kernel<0, {-variant}>"
+    if 0 < variant < 1000:
+        return "tile_op_kernel<0>"
+    return f"tgrp_op_kernel<0, {variant // 1000}>"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,7 +63,8 @@ def main():
     ap.add_argument("--nfreq", type=int, default=256)
     ap.add_argument("--block", type=int, default=32)
     ap.add_argument("--op-tile-rows", type=int, default=0,
-                    help="32-slot operator variant: 0 timed at set-up, -1 row kernel, 16/32/64 LDS row tiles")
+                    help="32-slot operator variant: 0 timed at set-up, -1 row kernel, -2/-4 rows grouped to a "
+                         "wavefront, t LDS tiles of t rows, 1000 R + t tiles with rows grouped R (e.g. 4032)")
     ap.add_argument("--fp64-sample", type=int, default=32,
                     help="frequencies timed with the FP64 solver (the error against FP64 covers all of them)")
     ap.add_argument("--no-config3", action="store_true", help="skip the coupled plate/cavity sweep")
@@ -169,12 +184,12 @@ def main():
 
     per_launch_ms = spmv_ms / max(1, spmv_calls)
     achieved = spmv_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
-    traffic = PMC_TRAFFIC.get(("row_op_kernel<0>", args.nel)) if B == 32 else None
+    op_kernel = operator_kernel(B, int(st.op_tile_rows))
+    traffic = PMC_TRAFFIC.get((op_kernel, args.nel))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": PMC_PROFILE if traffic else None,
-            "kernel": ("row_op_kernel<0> (fused 32-frequency FP32 operator, level 0)" if B == 32
-                       else f"op_kernel<float,{B},0> (fused multi-frequency FP32 operator, level 0)"),
+            "kernel": f"{op_kernel} (fused {B}-frequency FP32 operator, level 0, variant {int(st.op_tile_rows)})",
             "bytes_per_launch": int(spmv_bytes), "avg_launch_ms": round(per_launch_ms, 4),
             "launches": int(spmv_calls)}
 

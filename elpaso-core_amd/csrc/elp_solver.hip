@@ -509,6 +509,336 @@ __global__ __launch_bounds__(kTileWg) void tile_op_kernel(OpArgs<float> a, PadCs
     row_apply<MODE, true>(a, pc, coef, x, b, y, __builtin_amdgcn_readfirstlane(r), lane, tc.loff, lds);
 }
 
+// Broadcasts within each row of 16 lanes (DPP row_newbcast:J reads lane J of
+// the row) folded into the instruction that uses the value.  The callers
+// keep two conditions of the ISA's DPP hazards: every broadcast source (the
+// first operand) is written by a memory load, never by a vector ALU
+// instruction just before, and the whole wavefront is active with no EXEC
+// change around these instructions.
+template <int J>
+__device__ inline float fmac_bcast(float acc, float k, float x) {  // acc + bcast_J(k) * x, fused
+  asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(k), "v"(x), "n"(J));
+  return acc;
+}
+template <int J>
+__device__ inline uint32_t add_bcast(uint32_t o, uint32_t v) {  // bcast_J(o) + v
+  uint32_t r;
+  asm("v_add_u32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "=v"(r) : "v"(o), "v"(v), "n"(J));
+  return r;
+}
+template <int J0, int N, typename Fn>
+__device__ inline void static_for(Fn &&fn) {
+  if constexpr (N > 0) {
+    fn(std::integral_constant<int, J0>{});
+    static_for<J0 + 1, N - 1>(fn);
+  }
+}
+
+// x of F consecutive complex frequencies of one line: one 8- or 16-byte load
+// (the whole vector is bit-cast to floats: a bit cast of one element of an
+// integer vector reads element 0 with this compiler)
+typedef float elp_f2v __attribute__((ext_vector_type(2)));
+typedef float elp_f4v __attribute__((ext_vector_type(4)));
+template <int F> struct XLine;
+template <> struct XLine<1> {
+  using t = elp_f2v;
+  static __device__ inline t load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(t, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+};
+template <> struct XLine<2> {
+  using t = elp_f4v;
+  static __device__ inline t load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(t, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+};
+
+// One step's entry values of a group's row: lane j of each 16-lane row holds
+// entry e + j's x offset (or LDS offset), K and M, read through range-checked
+// loads (a step's second half may run past the last row: those values are
+// loaded, never used).
+struct GrpMeta {
+  uint32_t o;
+  float k, m;
+};
+struct GrpSrc {
+  __amdgpu_buffer_rsrc_t orr, kr, mr;
+  int32_t base;  // the row's first padded entry
+  int j16;
+};
+__device__ inline GrpMeta grp_meta(const GrpSrc &g, int32_t e) {
+  const uint32_t mo = (uint32_t)(g.base + e + g.j16) * 4u;
+  GrpMeta t;
+  t.o = __builtin_amdgcn_raw_buffer_load_b32(g.orr, mo, 0, 0);
+  t.k = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.kr, mo, 0, 0));
+  t.m = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.mr, mo, 0, 0));
+  return t;
+}
+
+// The summing loop of a group's rows (R = 2 or 4 rows to a wavefront, F = R /
+// 2 complex frequencies per lane), from the first step's entry values
+// `first` (grp_meta(g, 0)): entry j's values reach the other lanes of its
+// 16-lane row by row_newbcast, and LX(offset + lane bytes) loads the lane's
+// 2 F floats of that entry's column.  The x loads of both halves of a step
+// are in flight together (BOTH; else one half at a time, for x in LDS) and
+// the next step's entry values are loaded before this step's FMAs; the
+// steps alternate between two sets of those registers,
+// so that no copy (a vector ALU write) ever precedes a broadcast of them.
+// Entries are summed in row order.  len: the group's padded row length
+// (uniform).
+template <int F, bool BOTH, typename LX>
+__device__ inline void grp_sum(const GrpSrc &g, const GrpMeta &first, int32_t len, uint32_t qoff, LX &&lx,
+                               float (&aK)[2 * F], float (&aM)[2 * F]) {
+  using XT = typename XLine<F>::t;
+  auto step = [&](const GrpMeta &cur, int32_t e, GrpMeta &next) {
+    XT xv[BOTH ? 16 : 8];
+    const bool two = e + 8 < len;  // uniform: the step's second half is in the row
+    auto ld = [&](auto J) {
+      constexpr int j = decltype(J)::value;
+      xv[BOTH ? j : j % 8] = lx(add_bcast<j>(cur.o, qoff));
+    };
+    auto fm = [&](auto J) {
+      constexpr int j = decltype(J)::value;
+#pragma unroll
+      for (int c = 0; c < 2 * F; c++) {
+        aK[c] = fmac_bcast<j>(aK[c], cur.k, xv[BOTH ? j : j % 8][c]);
+        aM[c] = fmac_bcast<j>(aM[c], cur.m, xv[BOTH ? j : j % 8][c]);
+      }
+    };
+    static_for<0, 8>(ld);
+    if (BOTH && two) static_for<8, 8>(ld);
+    if (e + 16 < len) next = grp_meta(g, e + 16);
+    static_for<0, 8>(fm);
+    if (two) {
+      if (!BOTH) static_for<8, 8>(ld);
+      static_for<8, 8>(fm);
+    }
+  };
+  if (len > 0) {
+    GrpMeta ma = first, mb;
+    for (int32_t e = 0;;) {
+      step(ma, e, mb);
+      if ((e += 16) >= len) break;
+      step(mb, e, ma);
+      if ((e += 16) >= len) break;
+    }
+  }
+}
+
+// A group row's own values for the epilogue, loaded before its sum so that
+// their latency overlaps it: field, extra-entry range, b (MODE 1, 2), x
+// (MODE 2; MODE 0 and 1 read x in the epilogue for Dirichlet rows only).
+template <int MODE, int F>
+struct GrpRow {
+  uint8_t fld;
+  int32_t x0, x1;
+  float bv[2 * F], xo[2 * F];
+};
+template <int MODE, int F>
+__device__ inline GrpRow<MODE, F> grp_row(const OpArgs<float> &a, const float *__restrict__ x,
+                                          const float *__restrict__ b, int32_t r, int q) {
+  constexpr int W = 64;
+  GrpRow<MODE, F> w;
+  const int64_t o = (int64_t)r * W + q * 2 * F;
+  w.fld = a.field[r];
+  w.x0 = a.xrp[r];
+  w.x1 = a.xrp[r + 1];
+#pragma unroll
+  for (int c = 0; c < 2 * F; c++) {
+    w.bv[c] = MODE == 0 ? 0.f : b[o + c];
+    w.xo[c] = MODE == 2 ? x[o + c] : 0.f;
+  }
+  return w;
+}
+
+// Row epilogue of the grouped kernels (row_apply's, for the lane's F
+// frequencies q F .. q F + F - 1): extra entries (Robin face, FSI), the
+// coefficients, MODE, Dirichlet rows as identity.
+template <int MODE, int F>
+__device__ inline void grp_finish(const OpArgs<float> &a, const OpCoef *__restrict__ coef,
+                                  const float *__restrict__ x, float *__restrict__ y, int32_t r, int q,
+                                  const GrpRow<MODE, F> &w, const float (&aK)[2 * F], const float (&aM)[2 * F]) {
+  constexpr int W = 64;
+  const uint8_t fld = w.fld;
+  const int64_t o = (int64_t)r * W + q * 2 * F;
+  float aG[2 * F], aS[2 * F], xo[2 * F];
+  const float *bv = w.bv;
+#pragma unroll
+  for (int c = 0; c < 2 * F; c++) {
+    aG[c] = aS[c] = 0.f;
+    xo[c] = MODE == 2 ? w.xo[c] : ((MODE == 0 || MODE == 1) && fld == 2) ? x[o + c] : 0.f;
+  }
+  for (int32_t k = w.x0; k < w.x1; k++) {
+    const float *xl = x + (int64_t)(uint32_t)a.xcol[k] * W + q * 2 * F;
+    const float v = a.xval[k];
+    const bool g = a.xtype[k] == 0;
+#pragma unroll
+    for (int c = 0; c < 2 * F; c++) {
+      if (g) aG[c] = fmaf(v, xl[c], aG[c]);
+      else aS[c] = fmaf(v, xl[c], aS[c]);
+    }
+  }
+  const bool fl = fld == 1;
+  float res[2 * F];
+#pragma unroll
+  for (int t = 0; t < F; t++) {
+    const OpCoef &cf = coef[q * F + t];
+    const float2 K = {aK[2 * t], aK[2 * t + 1]}, M = {aM[2 * t], aM[2 * t + 1]};
+    const float2 Gs = {aG[2 * t], aG[2 * t + 1]};
+    const float2 ck = fl ? float2{(float)cf.kf.x, (float)cf.kf.y} : float2{(float)cf.ks.x, (float)cf.ks.y};
+    const float2 cm = fl ? float2{(float)cf.mf.x, (float)cf.mf.y} : float2{(float)cf.ms.x, (float)cf.ms.y};
+    const float2 cg = {(float)cf.g.x, (float)cf.g.y};
+    const float2 t1 = cmul(ck, K), t2 = cmul(cm, M), t3 = cmul(cg, Gs);
+    const float re = t1.x + t2.x + t3.x + aS[2 * t], im = t1.y + t2.y + t3.y + aS[2 * t + 1];
+    if (MODE == 0) {
+      res[2 * t] = re;
+      res[2 * t + 1] = im;
+    } else if (MODE == 1) {
+      res[2 * t] = bv[2 * t] - re;
+      res[2 * t + 1] = bv[2 * t + 1] - im;
+    } else {
+      const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+      const float inv = (float)(1.0 / d);
+      if (MODE == 3) {
+        res[2 * t] = bv[2 * t] * inv;
+        res[2 * t + 1] = bv[2 * t + 1] * inv;
+      } else {
+        res[2 * t] = xo[2 * t] + (bv[2 * t] - re) * inv;
+        res[2 * t + 1] = xo[2 * t + 1] + (bv[2 * t + 1] - im) * inv;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2 * F; c++) {
+    float v = res[c];
+    if (fld == 2) v = MODE == 0 ? xo[c] : MODE == 1 ? bv[c] - xo[c] : bv[c];  // Dirichlet row: identity
+    y[o + c] = v;
+  }
+}
+
+// The 32-slot FP32 operator with R rows to a wavefront (R = 2 or 4): each row
+// is taken by G = 64 / R lanes, lane q of a row holding F = R / 2 consecutive
+// complex frequencies (2 F floats, one 8- or 16-byte load of x per entry).
+// The rows of a wavefront (a group: rows of one window of 64 consecutive
+// rows taken by length) are padded to one length, so the wavefront walks
+// their entries in lock step with every lane active (grp_sum).  Per entry
+// and row: one load of x, 4 F FMAs and one address add for the wavefront's
+// R rows, against three v_readlane broadcasts and two FMAs per entry in
+// row_op_kernel.  Entries are summed in row order and the row epilogue is
+// row_apply's, so the results are the same bits.  `grows` lists each group's
+// R rows (-1 past the last row; a group's first row is always present).
+template <int MODE, int R>
+__global__ __launch_bounds__(kRowWg) void rgrp_op_kernel(OpArgs<float> a, PadCsr pc, int64_t n_pad,
+                                                       const int32_t *__restrict__ grows, int64_t waves,
+                                                       const OpCoef *__restrict__ coef, const float *__restrict__ x,
+                                                       const float *__restrict__ b, float *__restrict__ y) {
+  static_assert(R == 2 || R == 4, "2 or 4 rows per wavefront");
+  constexpr int G = 64 / R, F = R / 2, W = 64;
+  const int64_t wv = __builtin_amdgcn_readfirstlane(xcd_block() * (kRowWg / 64) + (int)(threadIdx.x >> 6));
+  if (wv >= waves) return;
+  const int lane = threadIdx.x & 63, q = lane % G;
+  const int32_t r0 = grows[wv * R + lane / G];
+  const int32_t rf = __builtin_amdgcn_readfirstlane(r0);
+  const int32_t r = r0 >= 0 ? r0 : rf;  // an absent row walks the first row's entries, unstored
+  const int32_t len = MODE == 3 ? 0 : __builtin_amdgcn_readfirstlane(pc.rp[rf + 1] - pc.rp[rf]);
+  const int xbytes = (int)(a.n * W * (int64_t)sizeof(float));
+  const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, xbytes, 0x00020000);
+  const int ebytes = (int)(n_pad * (int64_t)sizeof(float));
+  const __amdgpu_buffer_rsrc_t orr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.off, (short)0, ebytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.K, (short)0, ebytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t mr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.M, (short)0, ebytes, 0x00020000);
+  const GrpSrc gs{orr, kr, mr, pc.rp[r], lane & 15};
+  const GrpMeta first = grp_meta(gs, 0);
+  const GrpRow<MODE, F> w = grp_row<MODE, F>(a, x, b, r, q);
+  float aK[2 * F], aM[2 * F];
+#pragma unroll
+  for (int c = 0; c < 2 * F; c++) aK[c] = aM[c] = 0.f;
+  grp_sum<F, true>(gs, first, len, (uint32_t)q * (8 * F), [&](uint32_t off) { return XLine<F>::load(xr, off); },
+                   aK, aM);
+  if (r0 < 0) return;
+  grp_finish<MODE, F>(a, coef, x, y, r, q, w, aK, aM);
+}
+
+// The grouped operator on row tiles (tgrp_op_kernel): a workgroup of kTgWg
+// threads stages the x lines of its tile's distinct columns in LDS (line 0
+// zero), then each wavefront takes groups of R rows of the tile (rows of the
+// tile taken by length, padded to one length per group) and sums them with
+// grp_sum reading x from LDS: per padded entry the tile's LDS byte offset of
+// its column's line (256 (1 + position), 0 for padding) replaces the offset
+// in x.  Each x line crosses the texture path once per tile instead of once
+// per entry.  Same entries, same order: the same bits.
+constexpr int kTgWg = 512;
+template <int MODE, int R>
+__global__ __launch_bounds__(kTgWg) void tgrp_op_kernel(OpArgs<float> a, PadCsr pc, int64_t n_pad, TileCsr tc,
+                                                      const int32_t *__restrict__ tgrp,
+                                                      const int32_t *__restrict__ grows,
+                                                      const OpCoef *__restrict__ coef, const float *__restrict__ x,
+                                                      const float *__restrict__ b, float *__restrict__ y) {
+  static_assert(R == 2 || R == 4, "2 or 4 rows per wavefront");
+  constexpr int G = 64 / R, F = R / 2, ST = 8;  // ST float4 of x per thread per staging round
+  extern __shared__ float4 tile_lds[];
+  const int64_t t = xcd_block();
+  if (t >= tc.tiles) return;  // whole workgroup
+  const int32_t c0 = tc.cptr[t], nl = tc.cptr[t + 1] - c0;
+  const float4 *x4 = reinterpret_cast<const float4 *>(x);
+  const int nq = (nl + 1) * 16;
+  const int lane = threadIdx.x & 63, q = lane % G;
+  const int ebytes = (int)(n_pad * (int64_t)sizeof(float));
+  const __amdgpu_buffer_rsrc_t orr = __builtin_amdgcn_make_buffer_rsrc((void *)tc.loff, (short)0, ebytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.K, (short)0, ebytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t mr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.M, (short)0, ebytes, 0x00020000);
+  // the wavefront's first group: its rows, entry values and row values are
+  // loaded before the staging, so that their latency overlaps it
+  const int32_t g1 = tgrp[t + 1];
+  int32_t gi = tgrp[t] + (int)(threadIdx.x >> 6);
+  int32_t r0 = -1, r = 0, len = 0;
+  GrpSrc gs{orr, kr, mr, 0, lane & 15};
+  GrpMeta first{};
+  GrpRow<MODE, F> w{};
+  auto open_group = [&]() {
+    const int32_t g = __builtin_amdgcn_readfirstlane(gi);
+    r0 = grows[(int64_t)g * R + lane / G];
+    const int32_t rf = __builtin_amdgcn_readfirstlane(r0);
+    r = r0 >= 0 ? r0 : rf;
+    len = MODE == 3 ? 0 : __builtin_amdgcn_readfirstlane(pc.rp[rf + 1] - pc.rp[rf]);
+    gs.base = pc.rp[r];
+    first = grp_meta(gs, 0);
+    w = grp_row<MODE, F>(a, x, b, r, q);
+  };
+  if (gi < g1) open_group();
+  for (int i0 = 0; i0 < nq; i0 += ST * kTgWg) {  // one round for up to 255 lines
+    int32_t col[ST];
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      const int i = i0 + threadIdx.x + u * kTgWg, l = i >> 4;
+      col[u] = (i < nq && l > 0) ? tc.col[c0 + l - 1] : -1;
+    }
+    float4 v[ST];
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      const int i = i0 + threadIdx.x + u * kTgWg;
+      v[u] = col[u] >= 0 ? x4[(int64_t)col[u] * 16 + (i & 15)] : float4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      const int i = i0 + threadIdx.x + u * kTgWg;
+      if (i < nq) tile_lds[i] = v[u];
+    }
+  }
+  __syncthreads();
+  const char *lb = reinterpret_cast<const char *>(tile_lds);
+  while (gi < g1) {
+    float aK[2 * F], aM[2 * F];
+#pragma unroll
+    for (int c = 0; c < 2 * F; c++) aK[c] = aM[c] = 0.f;
+    grp_sum<F, false>(gs, first, len, (uint32_t)q * (8 * F),
+                      [&](uint32_t off) { return *reinterpret_cast<const typename XLine<F>::t *>(lb + off); }, aK,
+                      aM);
+    if (r0 >= 0) grp_finish<MODE, F>(a, coef, x, y, r, q, w, aK, aM);
+    if ((gi += kTgWg / 64) < g1) open_group();
+  }
+}
+
 // Post-smoothing of the two-level box cycle fused with the prolongation,
 // for 32 frequencies.  With u = D^-1 r (first sweep), v = r - A u and e the
 // box-lattice correction, the cycle's last step is
@@ -1156,6 +1486,13 @@ struct LevelData {
   int64_t n_tiles = 0;
   int t_rows = 0, t_lines = 0;  // rows per tile (before splits), most lines of a tile
   TileCsr tiles() const { return {t_row.p, t_cptr.p, t_col.p, t_loff.p, n_tiles}; }
+  // rows grouped g_R to a wavefront (rgrp_op_kernel), g_R = 0: none; the
+  // padded copy is then padded per group (pad_R = g_R), else to kRowPad
+  DArr<int32_t> g_rows;
+  int64_t g_waves = 0;
+  int g_R = 0, pad_R = 0;
+  // with tiles and groups (tgrp_op_kernel): each tile's first group
+  DArr<int32_t> t_grp;
   // box cycle: K P, M P, C_Gamma P (row-padded, permuted rows, lattice
   // columns as byte offsets) and P rows padded to 8 (box_post_kernel)
   DArr<int32_t> ap_rp;
@@ -1490,6 +1827,70 @@ static void build_box_products(LevelData &L, hipStream_t s) {
   L.pp_v.upload(c, pval);
 }
 
+// padded row pointer of the row kernel and the tiles: each row rounded up
+// to a multiple of kRowPad entries
+static std::vector<int32_t> row_padding(const std::vector<int32_t> &rp) {
+  const int64_t n = (int64_t)rp.size() - 1;
+  std::vector<int32_t> rp8(n + 1, 0);
+  for (int64_t r = 0; r < n; r++) {
+    const int32_t len = rp[r + 1] - rp[r];
+    rp8[r + 1] = rp8[r] + (len + kRowPad - 1) / kRowPad * kRowPad;
+  }
+  return rp8;
+}
+
+// Rows grouped R to a wavefront for rgrp_op_kernel: in each window of 64
+// consecutive rows (operator order) the rows sorted by length, longest
+// first (ties in row order), and taken R at a time; every row of a group is
+// padded to the group's longest row rounded up to kRowPad entries.  Returns
+// the padded row pointer; grows gets each group's rows (-1 past the end).
+static std::vector<int32_t> group_padding(const std::vector<int32_t> &rp, int R, std::vector<int32_t> &grows) {
+  constexpr int kWin = 64;
+  const int64_t n = (int64_t)rp.size() - 1;
+  std::vector<int32_t> plen(n, 0), idx;
+  grows.clear();
+  for (int64_t w0 = 0; w0 < n; w0 += kWin) {
+    const int64_t w1 = std::min<int64_t>(n, w0 + kWin);
+    idx.resize(w1 - w0);
+    for (int64_t r = w0; r < w1; r++) idx[r - w0] = (int32_t)r;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int32_t u, int32_t v) { return rp[u + 1] - rp[u] > rp[v + 1] - rp[v]; });
+    for (size_t k = 0; k < idx.size(); k += R) {
+      const int32_t longest = rp[idx[k] + 1] - rp[idx[k]];
+      const int32_t glen = (longest + kRowPad - 1) / kRowPad * kRowPad;
+      for (int u = 0; u < R; u++) {
+        if (k + u < idx.size()) {
+          plen[idx[k + u]] = glen;
+          grows.push_back(idx[k + u]);
+        } else {
+          grows.push_back(-1);
+        }
+      }
+    }
+  }
+  std::vector<int32_t> rp8(n + 1, 0);
+  for (int64_t r = 0; r < n; r++) rp8[r + 1] = rp8[r] + plen[r];
+  return rp8;
+}
+
+// (re)build the padded copy of the level's FP32 operator for the padded row
+// pointer rp8 (rows padded per group of pad_R rows, or pad_R = 0)
+static void pad_operator(LevelData &L, const std::vector<int32_t> &rp8, int pad_R, hipStream_t s) {
+  Model &m = *L.model;
+  Ctx *c = m.ctx;
+  ELP_REQUIRE((int64_t)rp8.size() == m.n + 1, ELP_ESTATE, "padded row pointer size");
+  L.pad_rp.upload(c, rp8);
+  L.pad_off.alloc(c, rp8[m.n]);
+  L.pad_K.alloc(c, rp8[m.n]);
+  L.pad_M.alloc(c, rp8[m.n]);
+  const uint32_t line = 32 * sizeof(float2);
+  hipLaunchKernelGGL(pad_rows_kernel, dim3(nblk(m.n, 256)), dim3(256), 0, s, m.n, L.rp32.p, L.ci32.p, L.K32.p,
+                     L.M32.p, L.pad_rp.p, L.pad_off.p, L.pad_K.p, L.pad_M.p, line, (uint32_t)(m.n * line));
+  ELP_HIP(hipGetLastError());
+  ELP_HIP(hipStreamSynchronize(s));
+  L.pad_R = pad_R;
+}
+
 static void permute_level_float(LevelData &L, hipStream_t s) {
   Model &m = *L.model;
   Ctx *c = m.ctx;
@@ -1591,32 +1992,22 @@ static void permute_level_float(LevelData &L, hipStream_t s) {
   L.M32 = std::move(Mn);
   L.X32 = std::move(Xn);
   L.permuted = true;
-  if (m.n <= kRowOpMaxRows) {
-    std::vector<int32_t> rp8(m.n + 1, 0);
-    for (int64_t r = 0; r < m.n; r++) {
-      const int32_t len = rpn[r + 1] - rpn[r];
-      rp8[r + 1] = rp8[r] + (len + kRowPad - 1) / kRowPad * kRowPad;
-    }
-    L.pad_rp.upload(c, rp8);
-    L.pad_off.alloc(c, rp8[m.n]);
-    L.pad_K.alloc(c, rp8[m.n]);
-    L.pad_M.alloc(c, rp8[m.n]);
-    const uint32_t line = 32 * sizeof(float2);
-    hipLaunchKernelGGL(pad_rows_kernel, dim3(g), dim3(256), 0, s, m.n, L.rp32.p, L.ci32.p, L.K32.p, L.M32.p,
-                       L.pad_rp.p, L.pad_off.p, L.pad_K.p, L.pad_M.p, line, (uint32_t)(m.n * line));
-    ELP_HIP(hipGetLastError());
-    ELP_HIP(hipStreamSynchronize(s));
-  }
+  if (m.n <= kRowOpMaxRows) pad_operator(L, row_padding(rpn), 0, s);
 }
 
-// Row tiles for tile_op_kernel over the padded FP32 operator (rows in the
-// operator's order): chunks of `rows` consecutive rows, each halved until its
-// distinct columns number at most max_lines (a single row always fits: rows
-// hold at most 256 columns).  Per padded entry the 1-based position of its
-// column in the tile's sorted column list times 256 (its line's byte offset
-// in LDS), 0 for padding.
+
+// Row tiles over the padded FP32 operator (rows in the operator's order):
+// chunks of `rows` consecutive rows, each halved until its distinct columns
+// number at most max_lines (a single row always fits: rows hold at most 256
+// columns).  Per padded entry the 1-based position of its column in the
+// tile's sorted column list times 256 (its line's byte offset in LDS), 0 for
+// padding.  R = 0: tiles of tile_op_kernel over the row padding; R = 2 or 4:
+// tiles of tgrp_op_kernel, the rows of each tile taken by length (longest
+// first, ties in row order) R at a time and padded to one length per group;
+// the padded copy is rebuilt for that padding, tgrp gets each tile's first
+// group and g_rows each group's rows (-1 past the tile's last row).
 static void build_tiles(LevelData &L, const std::vector<int32_t> &perm, const std::vector<int32_t> &iperm,
-                        const std::vector<int32_t> &rp8, int rows, int max_lines) {
+                        const std::vector<int32_t> &rp, int rows, int max_lines, int R, hipStream_t s) {
   const Model &m = *L.model;
   const int64_t n = m.n, nch = (n + rows - 1) / rows;
   struct Sub {
@@ -1624,7 +2015,6 @@ static void build_tiles(LevelData &L, const std::vector<int32_t> &perm, const st
     std::vector<int32_t> cols;
   };
   std::vector<std::vector<Sub>> per(nch);
-  std::vector<uint32_t> idx(rp8[n], 0);
   parallel_for(nch, [&](int64_t a, int64_t b) {
     std::vector<int32_t> u;
     for (int64_t ch = a; ch < b; ch++) {
@@ -1643,33 +2033,73 @@ static void build_tiles(LevelData &L, const std::vector<int32_t> &perm, const st
           todo.push_back({rr.first, mid});
           continue;
         }
-        for (int32_t r = rr.first; r < rr.second; r++) {
-          int32_t q = rp8[r];
-          for (int32_t p = m.rowptr[perm[r]]; p < m.rowptr[perm[r] + 1]; p++, q++)
-            idx[q] = 256u * (uint32_t)(1 + (std::lower_bound(u.begin(), u.end(), iperm[m.colind[p]]) - u.begin()));
-        }
         per[ch].push_back({rr.first, rr.second, u});
+      }
+    }
+  });
+  std::vector<const Sub *> tl;
+  for (auto &v : per)
+    for (auto &t : v) tl.push_back(&t);
+  // padding: per row, or per group of R rows of a tile
+  std::vector<int32_t> rp8, tgrp{0}, grows;
+  if (R > 0) {
+    std::vector<int32_t> plen(n, 0), idx;
+    for (const Sub *t : tl) {
+      idx.resize(t->r1 - t->r0);
+      for (int32_t r = t->r0; r < t->r1; r++) idx[r - t->r0] = r;
+      std::stable_sort(idx.begin(), idx.end(),
+                       [&](int32_t u, int32_t v) { return rp[u + 1] - rp[u] > rp[v + 1] - rp[v]; });
+      for (size_t k = 0; k < idx.size(); k += R) {
+        const int32_t longest = rp[idx[k] + 1] - rp[idx[k]];
+        const int32_t glen = (longest + kRowPad - 1) / kRowPad * kRowPad;
+        for (int u = 0; u < R; u++) {
+          if (k + u < idx.size()) plen[idx[k + u]] = glen;
+          grows.push_back(k + u < idx.size() ? idx[k + u] : -1);
+        }
+      }
+      tgrp.push_back((int32_t)(grows.size() / R));
+    }
+    rp8.assign(n + 1, 0);
+    for (int64_t r = 0; r < n; r++) rp8[r + 1] = rp8[r] + plen[r];
+  } else {
+    rp8 = row_padding(rp);
+  }
+  std::vector<uint32_t> loff(rp8[n], 0);
+  parallel_for((int64_t)tl.size(), [&](int64_t a, int64_t b) {
+    for (int64_t k = a; k < b; k++) {
+      const Sub &t = *tl[k];
+      for (int32_t r = t.r0; r < t.r1; r++) {
+        int32_t q = rp8[r];
+        for (int32_t p = m.rowptr[perm[r]]; p < m.rowptr[perm[r] + 1]; p++, q++)
+          loff[q] = 256u * (uint32_t)(1 + (std::lower_bound(t.cols.begin(), t.cols.end(), iperm[m.colind[p]]) -
+                                           t.cols.begin()));
       }
     }
   });
   std::vector<int32_t> trow{0}, tcp{0}, tcol;
   int most = 0;
-  for (auto &v : per)
-    for (auto &t : v) {
-      trow.back() = t.r0;
-      trow.push_back(t.r1);
-      tcol.insert(tcol.end(), t.cols.begin(), t.cols.end());
-      tcp.push_back((int32_t)tcol.size());
-      most = std::max(most, (int)t.cols.size());
-    }
+  for (const Sub *t : tl) {
+    trow.back() = t->r0;
+    trow.push_back(t->r1);
+    tcol.insert(tcol.end(), t->cols.begin(), t->cols.end());
+    tcp.push_back((int32_t)tcol.size());
+    most = std::max(most, (int)t->cols.size());
+  }
   Ctx *c = m.ctx;
+  if (R > 0 || L.pad_R != 0) pad_operator(L, rp8, R, s);
   L.t_row.upload(c, trow);
   L.t_cptr.upload(c, tcp);
   L.t_col.upload(c, tcol);
-  L.t_loff.upload(c, idx);
+  L.t_loff.upload(c, loff);
   L.n_tiles = (int64_t)trow.size() - 1;
   L.t_rows = rows;
   L.t_lines = most;
+  if (R > 0) {
+    L.t_grp.upload(c, tgrp);
+    L.g_rows.upload(c, grows);
+    L.g_waves = (int64_t)grows.size() / R;
+    L.g_R = R;
+  }
   if (most > max_lines) L.n_tiles = 0;  // a row alone has more columns than fit: no tiles
 }
 
@@ -1749,10 +2179,8 @@ static bool mg_possible(const Model &m) {
   return true;
 }
 
-// the 32-slot FP32 operator may run on row tiles
-static bool tiles_wanted(const elp_solve_opts &o) {
-  return o.precision != ELP_PREC_FP64 && o.block == 32 && o.op_tile_rows != -1;
-}
+// the 32-slot FP32 operator has variants (row kernel, row groups, tiles)
+static bool variants_wanted(const elp_solve_opts &o) { return o.precision != ELP_PREC_FP64 && o.block == 32; }
 
 // structure and fluid rows in one model
 static bool two_fields(const Model &m) {
@@ -1952,46 +2380,104 @@ static bool same_opts(const elp_solve_opts &a, const elp_solve_opts &b) {
 // HIP graph by default (their kernels are short next to launch overhead)
 constexpr int64_t kGraphAutoRows = 100000;
 
-// Launch of the 32-slot FP32 operator of a level: on its row tiles when it
-// has them, else the row kernel (both give the same bits)
+// Launch of the 32-slot FP32 operator of a level: on its row tiles or row
+// groups when it has them, else the row kernel (all give the same bits)
 constexpr int kTileMaxLines = kTileStage * kTileWg / 16 - 1;  // 64 KB of LDS per tile with the zero line
+constexpr int kTgMaxLines = kTileMaxLines;  // 191 or 127 lines (more workgroups per CU) measured slower
 template <int MODE>
 static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x, const float *b, float *y,
                           hipStream_t s) {
-  if (L.n_tiles > 0) {
+  const OpArgs<float> a = L.template args<float>();
+  if (L.n_tiles > 0 && (L.g_R == 2 || L.g_R == 4)) {
+    const dim3 grid(xcd_grid(L.n_tiles));
+    const size_t lds = (size_t)(L.t_lines + 1) * 256;
+    if (L.g_R == 2)
+      hipLaunchKernelGGL((tgrp_op_kernel<MODE, 2>), grid, dim3(kTgWg), lds, s, a, L.padded(), (int64_t)L.pad_K.n,
+                         L.tiles(), L.t_grp.p, L.g_rows.p, coef, x, b, y);
+    else
+      hipLaunchKernelGGL((tgrp_op_kernel<MODE, 4>), grid, dim3(kTgWg), lds, s, a, L.padded(), (int64_t)L.pad_K.n,
+                         L.tiles(), L.t_grp.p, L.g_rows.p, coef, x, b, y);
+  } else if (L.n_tiles > 0) {
     hipLaunchKernelGGL((tile_op_kernel<MODE>), dim3(xcd_grid(L.n_tiles)), dim3(kTileWg),
-                       (size_t)(L.t_lines + 1) * 256, s, L.template args<float>(), L.padded(), L.tiles(), coef, x, b,
-                       y);
+                       (size_t)(L.t_lines + 1) * 256, s, a, L.padded(), L.tiles(), coef, x, b, y);
+  } else if (L.g_R == 2 || L.g_R == 4) {
+    const dim3 grid(xcd_grid(nblk(L.g_waves, kRowWg / 64)));
+    if (L.g_R == 2)
+      hipLaunchKernelGGL((rgrp_op_kernel<MODE, 2>), grid, dim3(kRowWg), 0, s, a, L.padded(), (int64_t)L.pad_K.n,
+                         L.g_rows.p, L.g_waves, coef, x, b, y);
+    else
+      hipLaunchKernelGGL((rgrp_op_kernel<MODE, 4>), grid, dim3(kRowWg), 0, s, a, L.padded(), (int64_t)L.pad_K.n,
+                         L.g_rows.p, L.g_waves, coef, x, b, y);
   } else {
-    hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
-                       L.template args<float>(), L.padded(), coef, x, b, y);
+    hipLaunchKernelGGL((row_op_kernel<MODE>), dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s, a,
+                       L.padded(), coef, x, b, y);
   }
   ELP_HIP(hipGetLastError());
 }
 
-// The 32-slot FP32 operator of level 0: the row kernel or row tiles of 16,
-// 32 or 64 rows, as requested, or (op_tile_rows = 0) whichever is fastest on
-// this model: each candidate is timed over 3 launches (after one untimed) on
-// a vector of ones with the coefficients of a mid-range frequency.
+// the variants of the 32-slot FP32 operator (elp_solve_opts.op_tile_rows):
+// -1 the row kernel, -2 and -4 rows grouped 2 or 4 to a wavefront, t > 0 LDS
+// tiles of t rows, 1000 R + t tiles of t rows with rows grouped R (2, 4)
+static bool operator_variant_valid(int v) {
+  if (v == 0 || v == -1 || v == -2 || v == -4) return true;
+  if (v > 0 && v < 1000) return true;
+  const int R = v / 1000, t = v % 1000;
+  return (R == 2 || R == 4) && t > 0;
+}
+// the variant a level runs (the op_tile_rows encoding)
+static int operator_variant(const LevelData &L) {
+  if (L.n_tiles > 0) return L.g_R > 1 ? 1000 * L.g_R + L.t_rows : L.t_rows;
+  return L.g_R > 1 ? -L.g_R : -1;
+}
+
+// Set level 0 up for operator variant v: tiles and groups of another
+// variant released, the padded copy rebuilt when its padding changes.
+static void set_operator_variant(LevelData &L, int v, const std::vector<int32_t> &perm,
+                                 const std::vector<int32_t> &iperm, const std::vector<int32_t> &rp, hipStream_t s) {
+  L.n_tiles = 0;
+  L.t_rows = 0;
+  L.t_row.release();
+  L.t_cptr.release();
+  L.t_col.release();
+  L.t_loff.release();
+  L.t_grp.release();
+  L.g_R = 0;
+  L.g_waves = 0;
+  L.g_rows.release();
+  if (v <= -2) {
+    std::vector<int32_t> grows;
+    const std::vector<int32_t> rp8 = group_padding(rp, -v, grows);
+    pad_operator(L, rp8, -v, s);
+    L.g_rows.upload(L.model->ctx, grows);
+    L.g_waves = (int64_t)grows.size() / -v;
+    L.g_R = -v;
+    return;
+  }
+  if (v > 0) {
+    build_tiles(L, perm, iperm, rp, v % 1000, v >= 1000 ? kTgMaxLines : kTileMaxLines, v / 1000, s);
+    return;
+  }
+  if (L.pad_R != 0) pad_operator(L, row_padding(rp), 0, s);
+}
+
+// The 32-slot FP32 operator of level 0: the variant requested, or
+// (op_tile_rows = 0) whichever of the row kernel, rows grouped 4 to a
+// wavefront, row tiles of 32 rows and row tiles of 32 or 64 rows grouped 2
+// or 4 to a wavefront is fastest on this model:
+// each candidate is timed over 3 launches (after one untimed) on a vector of
+// ones with the coefficients of a mid-range frequency.
 static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_opts &o, hipStream_t s) {
   LevelData &L = *st.levels[0];
-  if (st.fp64_inner || !tiles_wanted(o) || !L.pad_rp.p) return;
+  if (st.fp64_inner || !variants_wanted(o) || !L.pad_rp.p) return;
   if (st.op_chosen && st.op_tile_rows == o.op_tile_rows) return;
   st.op_chosen = true;
   st.op_tile_rows = o.op_tile_rows;
-  if (L.n_tiles > 0) {  // a variant chosen before under other options
-    L.n_tiles = 0;
-    L.t_row.release();
-    L.t_cptr.release();
-    L.t_col.release();
-    L.t_loff.release();
-  }
-  std::vector<int32_t> perm, iperm, rp8;
+  std::vector<int32_t> perm, iperm, rp;
   L.perm.download(perm);
   L.iperm.download(iperm);
-  L.pad_rp.download(rp8);
-  if (o.op_tile_rows > 0) {
-    build_tiles(L, perm, iperm, rp8, o.op_tile_rows, kTileMaxLines);
+  L.rp32.download(rp);
+  if (o.op_tile_rows != 0) {
+    set_operator_variant(L, o.op_tile_rows, perm, iperm, rp, s);
     return;
   }
 
@@ -2015,25 +2501,19 @@ static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_op
     ELP_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
     return ms;
   };
-  float best = time_it();  // the row kernel
-  int best_rows = -1;
-  for (int rows : {16, 32, 64}) {
-    build_tiles(L, perm, iperm, rp8, rows, kTileMaxLines);
+  float best = 0;
+  int best_v = 0, last = 0;
+  for (int v : {-1, -4, 32, 2032, 4032, 4064}) {
+    set_operator_variant(L, v, perm, iperm, rp, s);
+    last = v;
+    if (v > 0 && L.n_tiles == 0) continue;  // rows too long for tiles
     const float ms = time_it();
-    if (ms < best) {
+    if (best_v == 0 || ms < best) {
       best = ms;
-      best_rows = rows;
+      best_v = v;
     }
   }
-  if (best_rows < 0) {
-    L.n_tiles = 0;
-    L.t_row.release();
-    L.t_cptr.release();
-    L.t_col.release();
-    L.t_loff.release();
-  } else if (best_rows != L.t_rows) {
-    build_tiles(L, perm, iperm, rp8, best_rows, kTileMaxLines);
-  }
+  if (best_v != last) set_operator_variant(L, best_v, perm, iperm, rp, s);
 }
 
 template <typename T, int B>
@@ -3528,6 +4008,7 @@ std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *fre
                                          hipStream_t setup, int *key) {
   ELP_REQUIRE(o.method == ELP_METHOD_COCG, ELP_EINVAL, "method must be ELP_METHOD_COCG");
   ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
+  ELP_REQUIRE(operator_variant_valid(o.op_tile_rows), ELP_EINVAL, "op_tile_rows must be 0, -1, -2, -4, a tile size below 1000 or 2000/4000 + a tile size");
   for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
   const int bi = slot_index(o.block);
   if (o.precision == ELP_PREC_FP64 || bi < 0) {  // one frequency at a time: runs alone
@@ -3610,7 +4091,7 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
     stats->refactor_ms = S.refactor_ms;
     const LevelData &L0 = *st.levels[0];
     stats->op_tile_rows = (std::is_same<T, float>::value && B == 32 && L0.pad_rp.p)
-                              ? (L0.n_tiles > 0 ? L0.t_rows : -1)
+                              ? operator_variant(L0)
                               : 0;
   }
   if (keep) {
@@ -3649,6 +4130,7 @@ int64_t sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z,
   std::vector<double> own_rel(relres_out ? 0 : nf);
   double *relres = relres_out ? relres_out : own_rel.data();
   ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
+  ELP_REQUIRE(operator_variant_valid(o.op_tile_rows), ELP_EINVAL, "op_tile_rows must be 0, -1, -2, -4, a tile size below 1000 or 2000/4000 + a tile size");
   for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
   auto t0 = std::chrono::steady_clock::now();
   Model::SolverState &st = solver_state(m, o, Z, s);

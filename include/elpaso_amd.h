@@ -96,9 +96,12 @@ typedef struct elp_solve_opts {
   int coarse_sweeps; /* MG coarsest-level smoothing sweeps                */
   int mg_levels;     /* MG: maximum number of levels (0 = automatic)      */
   int mg_min_nodes;  /* MG: stop coarsening below this many nodes         */
-  int op_tile_rows;  /* 32-slot FP32 operator: rows per LDS tile (16, 32, */
-                     /* 64), -1 = row kernel, 0 = chosen at set-up by     */
-                     /* timing the candidates on the model                */
+  int op_tile_rows;  /* 32-slot FP32 operator variant: -1 one row per    */
+                     /* wavefront, -2 / -4 rows grouped 2 / 4 to a        */
+                     /* wavefront, t (< 1000) LDS tiles of t rows,        */
+                     /* 1000 R + t LDS tiles of t rows with rows grouped  */
+                     /* R = 2 or 4 to a wavefront (e.g. 4032); 0 = chosen */
+                     /* at set-up by timing candidates on the model       */
   int graph;         /* FP32 sweeps: each Krylov iteration replayed from  */
                      /* a captured HIP graph: 1 on, -1 off, 0 for models  */
                      /* up to 100,000 rows (launch-bound)                 */
@@ -114,8 +117,8 @@ typedef struct elp_sweep_stats {
   int unconverged;     /* frequencies above rtol (then ELP_ENOCONV)       */
   int refactors;       /* coupled: factorizations of the structure block  */
   double refactor_ms;  /* coupled: time spent in them                     */
-  int op_tile_rows;    /* fine-level operator used: rows per tile, -1 row */
-                       /* kernel, 0 another operator (other slot counts)  */
+  int op_tile_rows;    /* fine-level operator used (the op_tile_rows      */
+                       /* encoding), 0 another operator (other slots)     */
   int64_t graph_replays; /* iterations replayed from the captured graph   */
 } elp_sweep_stats;
 
