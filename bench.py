@@ -177,7 +177,7 @@ def main():
                     "kernel_ms_per_freq": round(tm["kernel_ms"] / len(idx), 3),
                     "wall_ms_per_freq": round(1e3 * wall / len(idx), 2),
                     "kernel": "fluid_node_gradp_kernel<20> + node_results_kernel",
-                    "bound": "instruction issue (magnitude-ordered sums of 20 terms)",
+                    "bound": "gathers of the solutions (20 per element and node)",
                     "note": "wall time includes the host copies of the solutions in and the 9 results per node out"}
         del Xs, nd
         log(f"[bench] node results: {node_res}")

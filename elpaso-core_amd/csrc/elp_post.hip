@@ -48,43 +48,15 @@ __device__ inline double jac_inverse(const double J[3][3], double inv[3][3]) {
   return det;
 }
 
-// sum of v[0..n) in elPaSo's order: sorted by magnitude, largest first
-// (std::sort with abs_order, then summed from the back; misc/mytypes.h:226).
-// The sort is Batcher's odd-even merge network for N inputs (103
-// compare-exchanges for 20, 19 for 8; every loop bound is a compile-time
-// constant, so the network unrolls and v stays in registers).  Like
-// std::sort it does not keep the order of equal magnitudes.
-template <int N>
-__device__ inline double abs_order_sum(double (&v)[N]) {
-#pragma unroll
-  for (int p = 1; p < N; p <<= 1) {
-#pragma unroll
-    for (int k = p; k >= 1; k >>= 1) {
-#pragma unroll
-      for (int j = k % p; j + k < N; j += 2 * k) {
-#pragma unroll
-        for (int i = 0; i < k; i++) {
-          if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
-            const double a = v[i + j], b = v[i + j + k];
-            const bool sw = fabs(a) > fabs(b);
-            v[i + j] = sw ? b : a;
-            v[i + j + k] = sw ? a : b;
-          }
-        }
-      }
-    }
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int i = N - 1; i >= 0; i--) s += v[i];
-  return s;
-}
-
 // grad p at the fluid nodes: one thread per node of a fluid kind; for each
 // of the node's elements in ascending id the element's gradient operator at
 // the node (J^-1 dN, frequency independent) is formed once and applied to the
 // chunk's nf solutions, each added to the node's slots 6..8 of out[f][r][9]
-// (the node's own slots: no other thread writes them).
+// (the node's own slots: no other thread writes them).  elPaSo sums the NPE
+// terms of a component sorted by magnitude (std::sort with abs_order,
+// misc/mytypes.h:226, elementfluid3d.cpp:521-526); here they are summed in
+// the element's local node order with fused multiply-adds: the two differ by
+// rounding only (a few ulps of the largest term).
 template <int NPE>
 __global__ __launch_bounds__(kPostThreads) void fluid_node_gradp_kernel(
     int64_t n_kn, const int32_t *__restrict__ knodes, const int32_t *__restrict__ adj_ptr,
@@ -132,13 +104,12 @@ __global__ __launch_bounds__(kPostThreads) void fluid_node_gradp_kernel(
       double2 *o = out + ((int64_t)f * n_nodes + r) * 9 + 6;
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        double re[NPE], im[NPE];
+        double2 g = {0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < NPE; k++) {
-          re[k] = jh[a][k] * p[k].x;
-          im[k] = jh[a][k] * p[k].y;
+          g.x = fma(jh[a][k], p[k].x, g.x);
+          g.y = fma(jh[a][k], p[k].y, g.y);
         }
-        const double2 g = double2{abs_order_sum(re), abs_order_sum(im)};
         double2 v = o[a];
         v.x += g.x;
         v.y += g.y;

@@ -242,8 +242,9 @@ int elp_results_sizes(const elp_model *m, int64_t *n_nodes, int64_t *n_brick_ele
  *    (elementstructurebrick.cpp:161-279); slots 6..8 grad p of each of the
  *    node's fluid elements evaluated at the node, summed over the elements
  *    (cElementFluid3d::computeStressesNodes / computeGradPPoint,
- *    element/fluid/elementfluid3d.cpp:380-540; per-element sums in elPaSo's
- *    magnitude order); other slots 0.
+ *    element/fluid/elementfluid3d.cpp:380-540; elPaSo sums each element's
+ *    terms by magnitude, here in local node order: the same to rounding);
+ *    other slots 0.
  * Replaces cAnalysis::computeStressesCells / computeStressesNodes
  * (analysis/analysis.cpp:869-937), which run per frequency on the host.
  * kernel_ms (may be null): device time of the kernels (HIP events), without
