@@ -54,6 +54,8 @@ def operator_kernel(B, variant):
         return f"rgrp_op_kernel<0, {-variant}>"
     if 0 < variant < 1000:
         return "tile_op_kernel<0>"
+    if variant // 1000 == 6:
+        return "thgrp_op_kernel<0>"
     return f"tgrp_op_kernel<0, {variant // 1000}>"
 
 
@@ -67,7 +69,8 @@ def main():
     ap.add_argument("--block", type=int, default=32)
     ap.add_argument("--op-tile-rows", type=int, default=0,
                     help="32-slot operator variant: 0 timed at set-up, -1 row kernel, -2/-4 rows grouped to a "
-                         "wavefront, t LDS tiles of t rows, 1000 R + t tiles with rows grouped R (e.g. 4032)")
+                         "wavefront, t LDS tiles of t rows, 1000 R + t tiles with rows grouped R (e.g. 4032), "
+                         "6000 + t tiles grouped 4 with their frequencies split over two workgroups")
     ap.add_argument("--fp64-sample", type=int, default=32,
                     help="frequencies timed with the FP64 solver (the error against FP64 covers all of them)")
     ap.add_argument("--no-config3", action="store_true", help="skip the coupled plate/cavity sweep")

@@ -839,6 +839,84 @@ __global__ __launch_bounds__(kTgWg) void tgrp_op_kernel(OpArgs<float> a, PadCsr 
   }
 }
 
+// tgrp_op_kernel with each tile's 32 frequencies split over two workgroups
+// (thgrp_op_kernel): workgroup 2 t + h stages the half-lines (128 bytes,
+// frequencies 16 h .. 16 h + 15) of tile t's columns, half the LDS of a
+// whole-line tile, so twice as many workgroups fit a CU; each 16-lane row of
+// a wavefront holds one of its row's 16 frequencies per lane (8-byte LDS
+// reads), and the tile's entries are read by both workgroups (the second
+// read from L2); the entries' LDS offsets count 128-byte lines.  Same
+// entries, same order: the same bits.
+template <int MODE>
+__global__ __launch_bounds__(kTgWg) void thgrp_op_kernel(OpArgs<float> a, PadCsr pc, int64_t n_pad, TileCsr tc,
+                                                       const int32_t *__restrict__ tgrp,
+                                                       const int32_t *__restrict__ grows,
+                                                       const OpCoef *__restrict__ coef, const float *__restrict__ x,
+                                                       const float *__restrict__ b, float *__restrict__ y) {
+  constexpr int R = 4, G = 16, F = 1, ST = 4;  // ST float4 of x per thread per staging round
+  extern __shared__ float4 tile_lds[];
+  const int64_t tb = xcd_block();
+  const int64_t t = tb >> 1;
+  const int h = (int)(tb & 1);
+  if (t >= tc.tiles) return;  // whole workgroup
+  const int32_t c0 = tc.cptr[t], nl = tc.cptr[t + 1] - c0;
+  const float4 *x4 = reinterpret_cast<const float4 *>(x);
+  const int nq = (nl + 1) * 8;
+  const int lane = threadIdx.x & 63, q = 16 * h + lane % G;  // the lane's frequency
+  const int ebytes = (int)(n_pad * (int64_t)sizeof(float));
+  const __amdgpu_buffer_rsrc_t orr = __builtin_amdgcn_make_buffer_rsrc((void *)tc.loff, (short)0, ebytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.K, (short)0, ebytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t mr = __builtin_amdgcn_make_buffer_rsrc((void *)pc.M, (short)0, ebytes, 0x00020000);
+  const int32_t g1 = tgrp[t + 1];
+  int32_t gi = tgrp[t] + (int)(threadIdx.x >> 6);
+  int32_t r0 = -1, r = 0, len = 0;
+  GrpSrc gs{orr, kr, mr, 0, lane & 15};
+  GrpMeta first{};
+  GrpRow<MODE, F> w{};
+  auto open_group = [&]() {
+    const int32_t g = __builtin_amdgcn_readfirstlane(gi);
+    r0 = grows[(int64_t)g * R + lane / G];
+    const int32_t rf = __builtin_amdgcn_readfirstlane(r0);
+    r = r0 >= 0 ? r0 : rf;
+    len = MODE == 3 ? 0 : __builtin_amdgcn_readfirstlane(pc.rp[rf + 1] - pc.rp[rf]);
+    gs.base = pc.rp[r];
+    first = grp_meta(gs, 0);
+    w = grp_row<MODE, F>(a, x, b, r, q);
+  };
+  if (gi < g1) open_group();
+  for (int i0 = 0; i0 < nq; i0 += ST * kTgWg) {  // one round for up to 255 lines
+    int32_t col[ST];
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      const int i = i0 + threadIdx.x + u * kTgWg, l = i >> 3;
+      col[u] = (i < nq && l > 0) ? tc.col[c0 + l - 1] : -1;
+    }
+    float4 v[ST];
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      const int i = i0 + threadIdx.x + u * kTgWg;
+      v[u] = col[u] >= 0 ? x4[(int64_t)col[u] * 16 + 8 * h + (i & 7)] : float4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      const int i = i0 + threadIdx.x + u * kTgWg;
+      if (i < nq) tile_lds[i] = v[u];
+    }
+  }
+  __syncthreads();
+  const char *lb = reinterpret_cast<const char *>(tile_lds);
+  while (gi < g1) {
+    float aK[2 * F], aM[2 * F];
+#pragma unroll
+    for (int c = 0; c < 2 * F; c++) aK[c] = aM[c] = 0.f;
+    grp_sum<F, false>(gs, first, len, (uint32_t)(lane % G) * 8u,
+                      [&](uint32_t off) { return *reinterpret_cast<const typename XLine<F>::t *>(lb + off); }, aK,
+                      aM);
+    if (r0 >= 0) grp_finish<MODE, F>(a, coef, x, y, r, q, w, aK, aM);
+    if ((gi += kTgWg / 64) < g1) open_group();
+  }
+}
+
 // Post-smoothing of the two-level box cycle fused with the prolongation,
 // for 32 frequencies.  With u = D^-1 r (first sweep), v = r - A u and e the
 // box-lattice correction, the cycle's last step is
@@ -1491,8 +1569,10 @@ struct LevelData {
   DArr<int32_t> g_rows;
   int64_t g_waves = 0;
   int g_R = 0, pad_R = 0;
-  // with tiles and groups (tgrp_op_kernel): each tile's first group
+  // with tiles and groups (tgrp_op_kernel): each tile's first group;
+  // t_half: each tile's frequencies split over two workgroups (thgrp_op_kernel)
   DArr<int32_t> t_grp;
+  bool t_half = false;
   // box cycle: K P, M P, C_Gamma P (row-padded, permuted rows, lattice
   // columns as byte offsets) and P rows padded to 8 (box_post_kernel)
   DArr<int32_t> ap_rp;
@@ -2000,14 +2080,15 @@ static void permute_level_float(LevelData &L, hipStream_t s) {
 // chunks of `rows` consecutive rows, each halved until its distinct columns
 // number at most max_lines (a single row always fits: rows hold at most 256
 // columns).  Per padded entry the 1-based position of its column in the
-// tile's sorted column list times 256 (its line's byte offset in LDS), 0 for
-// padding.  R = 0: tiles of tile_op_kernel over the row padding; R = 2 or 4:
+// tile's sorted column list times line_bytes (its line's byte offset in LDS:
+// 256, or 128 for the half-lines of thgrp_op_kernel), 0 for padding.  R = 0: tiles of tile_op_kernel over the row padding; R = 2 or 4:
 // tiles of tgrp_op_kernel, the rows of each tile taken by length (longest
 // first, ties in row order) R at a time and padded to one length per group;
 // the padded copy is rebuilt for that padding, tgrp gets each tile's first
 // group and g_rows each group's rows (-1 past the tile's last row).
 static void build_tiles(LevelData &L, const std::vector<int32_t> &perm, const std::vector<int32_t> &iperm,
-                        const std::vector<int32_t> &rp, int rows, int max_lines, int R, hipStream_t s) {
+                        const std::vector<int32_t> &rp, int rows, int max_lines, int R, hipStream_t s,
+                        uint32_t line_bytes = 256) {
   const Model &m = *L.model;
   const int64_t n = m.n, nch = (n + rows - 1) / rows;
   struct Sub {
@@ -2071,8 +2152,9 @@ static void build_tiles(LevelData &L, const std::vector<int32_t> &perm, const st
       for (int32_t r = t.r0; r < t.r1; r++) {
         int32_t q = rp8[r];
         for (int32_t p = m.rowptr[perm[r]]; p < m.rowptr[perm[r] + 1]; p++, q++)
-          loff[q] = 256u * (uint32_t)(1 + (std::lower_bound(t.cols.begin(), t.cols.end(), iperm[m.colind[p]]) -
-                                           t.cols.begin()));
+          loff[q] = line_bytes * (uint32_t)(1 + (std::lower_bound(t.cols.begin(), t.cols.end(),
+                                                                   iperm[m.colind[p]]) -
+                                                  t.cols.begin()));
       }
     }
   });
@@ -2388,7 +2470,11 @@ template <int MODE>
 static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x, const float *b, float *y,
                           hipStream_t s) {
   const OpArgs<float> a = L.template args<float>();
-  if (L.n_tiles > 0 && (L.g_R == 2 || L.g_R == 4)) {
+  if (L.n_tiles > 0 && L.t_half) {
+    hipLaunchKernelGGL((thgrp_op_kernel<MODE>), dim3(xcd_grid(2 * L.n_tiles)), dim3(kTgWg),
+                       (size_t)(L.t_lines + 1) * 128, s, a, L.padded(), (int64_t)L.pad_K.n, L.tiles(), L.t_grp.p,
+                       L.g_rows.p, coef, x, b, y);
+  } else if (L.n_tiles > 0 && (L.g_R == 2 || L.g_R == 4)) {
     const dim3 grid(xcd_grid(L.n_tiles));
     const size_t lds = (size_t)(L.t_lines + 1) * 256;
     if (L.g_R == 2)
@@ -2418,14 +2504,17 @@ static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x
 // the variants of the 32-slot FP32 operator (elp_solve_opts.op_tile_rows):
 // -1 the row kernel, -2 and -4 rows grouped 2 or 4 to a wavefront, t > 0 LDS
 // tiles of t rows, 1000 R + t tiles of t rows with rows grouped R (2, 4)
+// 6000 + t: tiles of t rows grouped 4 with each tile's frequencies split
+// over two workgroups
 static bool operator_variant_valid(int v) {
   if (v == 0 || v == -1 || v == -2 || v == -4) return true;
   if (v > 0 && v < 1000) return true;
   const int R = v / 1000, t = v % 1000;
-  return (R == 2 || R == 4) && t > 0;
+  return (R == 2 || R == 4 || R == 6) && t > 0;
 }
 // the variant a level runs (the op_tile_rows encoding)
 static int operator_variant(const LevelData &L) {
+  if (L.n_tiles > 0 && L.t_half) return 6000 + L.t_rows;
   if (L.n_tiles > 0) return L.g_R > 1 ? 1000 * L.g_R + L.t_rows : L.t_rows;
   return L.g_R > 1 ? -L.g_R : -1;
 }
@@ -2441,9 +2530,15 @@ static void set_operator_variant(LevelData &L, int v, const std::vector<int32_t>
   L.t_col.release();
   L.t_loff.release();
   L.t_grp.release();
+  L.t_half = false;
   L.g_R = 0;
   L.g_waves = 0;
   L.g_rows.release();
+  if (v >= 6000) {
+    build_tiles(L, perm, iperm, rp, v % 1000, kTgMaxLines, 4, s, 128);
+    L.t_half = L.n_tiles > 0;
+    return;
+  }
   if (v <= -2) {
     std::vector<int32_t> grows;
     const std::vector<int32_t> rp8 = group_padding(rp, -v, grows);
@@ -2462,8 +2557,9 @@ static void set_operator_variant(LevelData &L, int v, const std::vector<int32_t>
 
 // The 32-slot FP32 operator of level 0: the variant requested, or
 // (op_tile_rows = 0) whichever of the row kernel, rows grouped 4 to a
-// wavefront, row tiles of 32 rows and row tiles of 32 or 64 rows grouped 2
-// or 4 to a wavefront is fastest on this model:
+// wavefront, row tiles of 32 rows, row tiles of 32 or 64 rows grouped 2 or 4
+// to a wavefront and those grouped 4 with their frequencies split over two
+// workgroups is fastest on this model:
 // each candidate is timed over 3 launches (after one untimed) on a vector of
 // ones with the coefficients of a mid-range frequency.
 static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_opts &o, hipStream_t s) {
@@ -2503,7 +2599,7 @@ static void choose_operator(Model &m, Model::SolverState &st, const elp_solve_op
   };
   float best = 0;
   int best_v = 0, last = 0;
-  for (int v : {-1, -4, 32, 2032, 4032, 4064}) {
+  for (int v : {-1, -4, 32, 2032, 4032, 4064, 6032, 6064}) {
     set_operator_variant(L, v, perm, iperm, rp, s);
     last = v;
     if (v > 0 && L.n_tiles == 0) continue;  // rows too long for tiles
@@ -4008,7 +4104,7 @@ std::unique_ptr<GroupItem> group_prepare(Model &m, int64_t nf, const double *fre
                                          hipStream_t setup, int *key) {
   ELP_REQUIRE(o.method == ELP_METHOD_COCG, ELP_EINVAL, "method must be ELP_METHOD_COCG");
   ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
-  ELP_REQUIRE(operator_variant_valid(o.op_tile_rows), ELP_EINVAL, "op_tile_rows must be 0, -1, -2, -4, a tile size below 1000 or 2000/4000 + a tile size");
+  ELP_REQUIRE(operator_variant_valid(o.op_tile_rows), ELP_EINVAL, "op_tile_rows must be 0, -1, -2, -4, a tile size below 1000 or 2000/4000/6000 + a tile size");
   for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
   const int bi = slot_index(o.block);
   if (o.precision == ELP_PREC_FP64 || bi < 0) {  // one frequency at a time: runs alone
@@ -4130,7 +4226,7 @@ int64_t sweep(Model &m, int64_t nf, const double *freqs, std::complex<double> Z,
   std::vector<double> own_rel(relres_out ? 0 : nf);
   double *relres = relres_out ? relres_out : own_rel.data();
   ELP_REQUIRE(o.rtol > 0 && o.max_iters > 0, ELP_EINVAL, "invalid tolerance / iteration limit");
-  ELP_REQUIRE(operator_variant_valid(o.op_tile_rows), ELP_EINVAL, "op_tile_rows must be 0, -1, -2, -4, a tile size below 1000 or 2000/4000 + a tile size");
+  ELP_REQUIRE(operator_variant_valid(o.op_tile_rows), ELP_EINVAL, "op_tile_rows must be 0, -1, -2, -4, a tile size below 1000 or 2000/4000/6000 + a tile size");
   for (int64_t k = 0; k < nf; k++) ELP_REQUIRE(freqs[k] > 0, ELP_EINVAL, "frequencies must be positive");
   auto t0 = std::chrono::steady_clock::now();
   Model::SolverState &st = solver_state(m, o, Z, s);

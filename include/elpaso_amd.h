@@ -100,8 +100,10 @@ typedef struct elp_solve_opts {
                      /* wavefront, -2 / -4 rows grouped 2 / 4 to a        */
                      /* wavefront, t (< 1000) LDS tiles of t rows,        */
                      /* 1000 R + t LDS tiles of t rows with rows grouped  */
-                     /* R = 2 or 4 to a wavefront (e.g. 4032); 0 = chosen */
-                     /* at set-up by timing candidates on the model       */
+                     /* R = 2 or 4 to a wavefront (e.g. 4032), 6000 + t   */
+                     /* those grouped 4 with each tile's frequencies      */
+                     /* split over two workgroups; 0 = chosen at set-up   */
+                     /* by timing candidates on the model                 */
   int graph;         /* FP32 sweeps: each Krylov iteration replayed from  */
                      /* a captured HIP graph: 1 on, -1 off, 0 for models  */
                      /* up to 100,000 rows (launch-bound)                 */

@@ -296,17 +296,18 @@ def test_coupled_sweep_block_preconditioner(ctx, block):
 def test_operator_variants_same_bits(ctx):
     """The 32-slot FP32 operator's variants (row kernel, rows grouped 2 or 4
     to a wavefront, LDS row tiles of 16, 32 and 64 rows, tiles with rows
-    grouped 2 or 4) sum the same entries in the same order: the sweeps are
-    the same bits, and the set-up's timing choice (op_tile_rows = 0) is one
-    of them.  The row kernel runs again after the others (its padding
+    grouped 2 or 4, tiles with rows grouped 4 and their frequencies split
+    over two workgroups) sum the same entries in the same order: the sweeps
+    are the same bits, and the set-up's timing choice (op_tile_rows = 0) is
+    one of them.  The row kernel runs again after the others (its padding
     rebuilt)."""
     m = MG.acoustic_cavity(16, 12, 10, quadratic=True)
     g = GpuModel(ctx, m)
     g.assemble()
     freqs = MG.linear_frequencies(10, 2000, 48)
     ref = None
-    variants = (-1, -4, 16, 32, 2016, 2032, 4016, 4032, 4064)
-    for rows in (-1, -2, -4, 16, 32, 64, 2016, 2032, 4016, 4032, 4064, 0, -1):
+    variants = (-1, -4, 32, 2032, 4032, 4064, 6032, 6064)
+    for rows in (-1, -2, -4, 16, 32, 64, 2016, 2032, 4016, 4032, 4064, 6016, 6032, 6064, 0, -1):
         X, rel, it, st = g.sweep(freqs, opts=_lib.default_opts(block=32, op_tile_rows=rows))
         print(f"[sweep] operator variant {rows}: used {st.op_tile_rows}", flush=True)
         assert (rel <= 1e-8).all()
