@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # on the default workloads, scaled by the factors a 1 GiB copy gives (x2, x1);
 # the commands and counter values are in the profile below.
 PMC_PROFILE = "profiles/pmc_traffic_operator_variants_and_assembly_kernels.md"
-PMC_TRAFFIC = {("tgrp_op_kernel<0, 4>", "100,80,60"): 2.933e9, ("row_op_kernel<0>", "100,80,60"): 2.790e9,
-               ("rgrp_op_kernel<0, 4>", "100,80,60"): 2.835e9, ("tile_op_kernel<0>", "100,80,60"): 2.884e9,
-               ("tgrp_op_kernel<0, 2>", "100,80,60"): 2.900e9,
-               ("hex8_node_kernel<3, false, true>", "200,200,4"): 1.117e9}
+PMC_TRAFFIC = {("thgrp_op_kernel<0>", "100,80,60"): 2.934e9, ("tgrp_op_kernel<0, 4>", "100,80,60"): 2.933e9,
+               ("row_op_kernel<0>", "100,80,60"): 2.790e9, ("rgrp_op_kernel<0, 4>", "100,80,60"): 2.833e9,
+               ("tile_op_kernel<0>", "100,80,60"): 2.884e9, ("tgrp_op_kernel<0, 2>", "100,80,60"): 2.900e9,
+               ("hex8_node_kernel<3, false, true>", "200,200,4"): 1.116e9}
 
 
 def log(*a):
