@@ -277,22 +277,39 @@ struct TileCsr {
   int64_t tiles;
 };
 
+// A row's l1 bound inputs (field, row sums) and a frequency's bound
+// coefficients, loaded once and passed to the value functions below.
+struct RowL1 {
+  uint8_t fld;
+  double rK, rM, rG;
+};
+template <typename T>
+__device__ inline RowL1 row_l1(const OpArgs<T> &a, int64_t r) {
+  return {a.field[r], a.rK[r], a.rM[r], a.rG[r]};
+}
+struct CoefL1 {
+  double akf, amf, aks, ams, ag;
+};
+__device__ inline CoefL1 coef_l1(const OpCoef &cf) { return {cf.akf, cf.amf, cf.aks, cf.ams, cf.ag}; }
+
 // First l1-Jacobi sweep from a zero guess, y = b / D, one thread per
 // (row, frequency): the same values as op_kernel's MODE 3.
 template <typename T>
-__device__ inline typename VT<T>::t jacobi0_value(const OpArgs<T> &a, const OpCoef *__restrict__ coef, int64_t r,
-                                                  int f, typename VT<T>::t bv) {
+__device__ inline typename VT<T>::t jacobi0_value(const RowL1 &w, const CoefL1 &cf, typename VT<T>::t bv) {
   using V = typename VT<T>::t;
-  const uint8_t fld = a.field[r];
-  if (fld == 2) return bv;
-  const OpCoef &cf = coef[f];
-  const bool fl = fld == 1;
-  const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+  if (w.fld == 2) return bv;
+  const bool fl = w.fld == 1;
+  const double d = (fl ? cf.akf : cf.aks) * w.rK + (fl ? cf.amf : cf.ams) * w.rM + cf.ag * w.rG;
   const T inv = (T)(1.0 / d);
   V t;
   t.x = bv.x * inv;
   t.y = bv.y * inv;
   return t;
+}
+template <typename T>
+__device__ inline typename VT<T>::t jacobi0_value(const OpArgs<T> &a, const OpCoef *__restrict__ coef, int64_t r,
+                                                  int f, typename VT<T>::t bv) {
+  return jacobi0_value<T>(row_l1<T>(a, r), coef_l1(coef[f]), bv);
 }
 
 // 1 / (l1 row bound of A(w_f)) on free rows, 0 on Dirichlet rows: the
@@ -300,17 +317,18 @@ __device__ inline typename VT<T>::t jacobi0_value(const OpArgs<T> &a, const OpCo
 // field share a scale (fluid rows divided by rho w^2, structure rows of order
 // E h), and fields can differ by 15 orders: a plain norm of a coupled
 // residual sees the structure rows only.
-template <typename T>
-__device__ inline double row_weight(const OpArgs<T> &a, const OpCoef *__restrict__ coef, int64_t r, int f) {
-  const uint8_t fld = a.field[r];
-  if (fld == 2) return 0.0;
-  const OpCoef &cf = coef[f];
-  const bool fl = fld == 1;
+__device__ inline double row_weight(const RowL1 &w, const CoefL1 &cf) {
+  if (w.fld == 2) return 0.0;
+  const bool fl = w.fld == 1;
   // a scale, not a value of the solution: FP32 and a hardware reciprocal
   // (about 1 ulp) are enough, and keep the update kernel memory bound
-  const float d = (float)(fl ? cf.akf : cf.aks) * (float)a.rK[r] + (float)(fl ? cf.amf : cf.ams) * (float)a.rM[r] +
-                  (float)cf.ag * (float)a.rG[r];
+  const float d = (float)(fl ? cf.akf : cf.aks) * (float)w.rK + (float)(fl ? cf.amf : cf.ams) * (float)w.rM +
+                  (float)cf.ag * (float)w.rG;
   return d > 0.f ? (double)__builtin_amdgcn_rcpf(d) : 0.0;
+}
+template <typename T>
+__device__ inline double row_weight(const OpArgs<T> &a, const OpCoef *__restrict__ coef, int64_t r, int f) {
+  return row_weight(row_l1<T>(a, r), coef_l1(coef[f]));
 }
 
 template <typename T>
@@ -1148,11 +1166,13 @@ __device__ inline void wnorm_body(int64_t n, int B, const V *__restrict__ v, con
   const int64_t chunk = (n + nblocks - 1) / nblocks;
   const int64_t r0 = blk * chunk, r1 = min(n, r0 + chunk);
   double2 acc = {0, 0};
+  const CoefL1 ca = coef_l1(coefa[f]);  // the column's bound coefficients, once
   for (int64_t r = r0 + rl; r < r1; r += R) {
     const V a = v[r * B + f];
+    const RowL1 w = row_l1<T>(la, r);
     const double q = (double)a.x * a.x + (double)a.y * a.y;
     acc.x += q;
-    acc.y += q * row_weight<T>(la, coefa, r, f);
+    acc.y += q * row_weight(w, ca);
   }
   red_block_store(acc, B, partial, blk);
 }
@@ -1181,24 +1201,54 @@ __device__ inline void update_norm_body(int64_t n, int B, const double2 *__restr
   const int64_t r0 = blk * chunk, r1 = min(n, r0 + chunk);
   const double2 a = alpha[f];
   const bool upd = !(a.x == 0 && a.y == 0);  // frozen column: x, r untouched
+  // the column's bound coefficients, once; every load of a row is issued
+  // before its first use (stores count against the same wait counter as
+  // loads here, so a load after a store waits for the store too)
+  const CoefL1 cp = coef_l1(coefp[f]), ca = coef_l1(coefa[f]);
   double2 acc = {0, 0};
-  for (int64_t row = r0 + rl; row < r1; row += R) {
+  // the next row's loads are issued before this row's stores
+  struct In {
+    V rv, pv, qv, xv;
+    RowL1 w;
+  };
+  auto load = [&](int64_t row) {
     const int64_t i = row * B + f;
-    V rv = r[i];
+    In t;
+    t.rv = r[i];
+    t.pv = t.qv = t.xv = V{};
     if (upd) {
-      const V pv = p[i], qv = q[i];
-      V xv = x[i];
+      t.pv = p[i];
+      t.qv = q[i];
+      t.xv = x[i];
+    }
+    t.w = row_l1<T>(la, row);
+    return t;
+  };
+  int64_t row = r0 + rl;
+  In nx{};
+  if (row < r1) nx = load(row);
+  for (; row < r1; row += R) {
+    const int64_t i = row * B + f;
+    const In c = nx;
+    if (row + R < r1) nx = load(row + R);
+    V rv = c.rv, xv = c.xv;
+    const V pv = c.pv, qv = c.qv;
+    const RowL1 &w = c.w;
+    if (upd) {
       xv.x += a.x * pv.x - a.y * pv.y;
       xv.y += a.x * pv.y + a.y * pv.x;
       rv.x -= a.x * qv.x - a.y * qv.y;
       rv.y -= a.x * qv.y + a.y * qv.x;
+    }
+    const V uv = jacobi0_value<T>(w, cp, rv);
+    const double q2 = (double)rv.x * rv.x + (double)rv.y * rv.y;
+    acc.x += q2;
+    acc.y += q2 * row_weight(w, ca);
+    if (upd) {
       x[i] = xv;
       r[i] = rv;
     }
-    if (u) u[i] = jacobi0_value<T>(la, coefp, row, f, rv);
-    const double q2 = (double)rv.x * rv.x + (double)rv.y * rv.y;
-    acc.x += q2;
-    acc.y += q2 * row_weight<T>(la, coefa, row, f);
+    if (u) u[i] = uv;
   }
   red_block_store(acc, B, partial, blk);
 }
