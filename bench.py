@@ -303,7 +303,8 @@ def main():
                    "precond": "two-level cycle: damped l1-Jacobi around the exact Hex8 box solve "
                               "(fast diagonalization)",
                    "method": "COCG (FP32) + FP64 iterative refinement",
-                   "operator_variant": int(st.op_tile_rows), "parallelism": "single GPU"},
+                   "operator_variant": int(st.op_tile_rows), "post_smoothing_tile_rows": int(st.box_tile_rows),
+                   "parallelism": "single GPU"},
         "converged": f"{converged}/{len(rels)}", "relres_max": float(rels.max()),
         "iters_mean": float(iters.mean()), "iters_max": int(iters.max()),
         "sweep_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},

@@ -491,6 +491,7 @@ __global__ __launch_bounds__(kRowWg) void row_op_kernel(OpArgs<float> a, PadCsr 
 // Dynamic LDS: 256 bytes per line (32 complex FP32 frequencies).
 constexpr int kTileWg = 1024;
 constexpr int kTileStage = 4;  // float4 of x per thread in the staging: (255 + 1) lines x 16 / 1024
+constexpr int kTileMaxLines = kTileStage * kTileWg / 16 - 1;  // 64 KB of LDS per tile with the zero line
 template <int MODE>
 __global__ __launch_bounds__(kTileWg) void tile_op_kernel(OpArgs<float> a, PadCsr pc, TileCsr tc,
                                                           const OpCoef *__restrict__ coef,
@@ -1024,6 +1025,169 @@ __global__ __launch_bounds__(kRowWg) void box_post_kernel(OpArgs<float> a, PadCs
   const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
   const float inv = (float)(1.0 / d);
   z[o] = uu + pe + (vv - ax) * inv;
+}
+
+// box_post_kernel on grouped half-line tiles (thbox_post_kernel), the
+// scheme of thgrp_op_kernel: a row's entries are its (A P) row with the P
+// value of each column beside K P, M P and C_Gamma P (0 where P has no
+// entry: every column of a P row is a column of its (A P) row), so P e and
+// (A P) e are summed in one walk over the lattice lines of the tile staged in
+// LDS, in the column order of both rows: the same bits as box_post_kernel.
+struct BoxTiles {
+  const int32_t *rp;     // padded rows (per group of 4 of a tile)
+  const uint32_t *loff;  // LDS offset of the entry's 128-byte half-line
+  const float *K, *M, *G, *P;
+  int64_t n_pad;
+  TileCsr tc;            // tiles: rows, distinct lattice columns
+  const int32_t *tgrp, *grows;
+};
+struct GrpMeta4 {
+  uint32_t o;
+  float k, m, g, p;
+};
+struct GrpSrc4 {
+  __amdgpu_buffer_rsrc_t orr, kr, mr, gr, pr;
+  int32_t base;
+  int j16;
+};
+__device__ inline GrpMeta4 grp_meta4(const GrpSrc4 &g, int32_t e) {
+  const uint32_t mo = (uint32_t)(g.base + e + g.j16) * 4u;
+  GrpMeta4 t;
+  t.o = __builtin_amdgcn_raw_buffer_load_b32(g.orr, mo, 0, 0);
+  t.k = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.kr, mo, 0, 0));
+  t.m = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.mr, mo, 0, 0));
+  t.g = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.gr, mo, 0, 0));
+  t.p = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.pr, mo, 0, 0));
+  return t;
+}
+// grp_sum with four values per entry (K P, M P, C_Gamma P, P), one complex
+// frequency per lane, one half of a step's x loads in flight at a time
+template <typename LX>
+__device__ inline void grp_sum4(const GrpSrc4 &g, const GrpMeta4 &first, int32_t len, uint32_t qoff, LX &&lx,
+                                float (&aK)[2], float (&aM)[2], float (&aG)[2], float (&aP)[2]) {
+  using XT = typename XLine<1>::t;
+  auto step = [&](const GrpMeta4 &cur, int32_t e, GrpMeta4 &next) {
+    XT xv[8];
+    const bool two = e + 8 < len;  // uniform
+    auto ld = [&](auto J) {
+      constexpr int j = decltype(J)::value;
+      xv[j % 8] = lx(add_bcast<j>(cur.o, qoff));
+    };
+    auto fm = [&](auto J) {
+      constexpr int j = decltype(J)::value;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        aK[c] = fmac_bcast<j>(aK[c], cur.k, xv[j % 8][c]);
+        aM[c] = fmac_bcast<j>(aM[c], cur.m, xv[j % 8][c]);
+        aG[c] = fmac_bcast<j>(aG[c], cur.g, xv[j % 8][c]);
+        aP[c] = fmac_bcast<j>(aP[c], cur.p, xv[j % 8][c]);
+      }
+    };
+    static_for<0, 8>(ld);
+    if (e + 16 < len) next = grp_meta4(g, e + 16);
+    static_for<0, 8>(fm);
+    if (two) {
+      static_for<8, 8>(ld);
+      static_for<8, 8>(fm);
+    }
+  };
+  if (len > 0) {
+    GrpMeta4 ma = first, mb;
+    for (int32_t e = 0;;) {
+      step(ma, e, mb);
+      if ((e += 16) >= len) break;
+      step(mb, e, ma);
+      if ((e += 16) >= len) break;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTgWg) void thbox_post_kernel(OpArgs<float> a, BoxTiles bt,
+                                                         const OpCoef *__restrict__ coef,
+                                                         const float *__restrict__ e, const float *__restrict__ u,
+                                                         const float *__restrict__ v, float *__restrict__ z) {
+  constexpr int R = 4, G = 16, ST = 4, W = 64;
+  extern __shared__ float4 tile_lds[];
+  const TileCsr &tc = bt.tc;
+  const int64_t tb = xcd_block();
+  const int64_t t = tb >> 1;
+  const int h = (int)(tb & 1);
+  if (t >= tc.tiles) return;  // whole workgroup
+  const int32_t c0 = tc.cptr[t], nl = tc.cptr[t + 1] - c0;
+  const float4 *e4 = reinterpret_cast<const float4 *>(e);
+  const int nq = (nl + 1) * 8;
+  const int lane = threadIdx.x & 63, q = 16 * h + lane % G;  // the lane's frequency
+  const int ebytes = (int)(bt.n_pad * (int64_t)sizeof(float));
+  GrpSrc4 gs;
+  gs.orr = __builtin_amdgcn_make_buffer_rsrc((void *)bt.loff, (short)0, ebytes, 0x00020000);
+  gs.kr = __builtin_amdgcn_make_buffer_rsrc((void *)bt.K, (short)0, ebytes, 0x00020000);
+  gs.mr = __builtin_amdgcn_make_buffer_rsrc((void *)bt.M, (short)0, ebytes, 0x00020000);
+  gs.gr = __builtin_amdgcn_make_buffer_rsrc((void *)bt.G, (short)0, ebytes, 0x00020000);
+  gs.pr = __builtin_amdgcn_make_buffer_rsrc((void *)bt.P, (short)0, ebytes, 0x00020000);
+  gs.base = 0;
+  gs.j16 = lane & 15;
+  const int32_t g1 = bt.tgrp[t + 1];
+  int32_t gi = bt.tgrp[t] + (int)(threadIdx.x >> 6);
+  int32_t r0 = -1, r = 0, len = 0;
+  GrpMeta4 first{};
+  float2 uu{}, vv{};
+  auto open_group = [&]() {
+    const int32_t g = __builtin_amdgcn_readfirstlane(gi);
+    r0 = bt.grows[(int64_t)g * R + lane / G];
+    const int32_t rf = __builtin_amdgcn_readfirstlane(r0);
+    r = r0 >= 0 ? r0 : rf;
+    len = __builtin_amdgcn_readfirstlane(bt.rp[rf + 1] - bt.rp[rf]);
+    gs.base = bt.rp[r];
+    first = grp_meta4(gs, 0);
+    const int64_t o = (int64_t)r * W + 2 * q;
+    uu = float2{u[o], u[o + 1]};
+    vv = float2{v[o], v[o + 1]};
+  };
+  if (gi < g1) open_group();
+  for (int i0 = 0; i0 < nq; i0 += ST * kTgWg) {  // one round for up to 255 lines
+    int32_t col[ST];
+#pragma unroll
+    for (int k = 0; k < ST; k++) {
+      const int i = i0 + threadIdx.x + k * kTgWg, l = i >> 3;
+      col[k] = (i < nq && l > 0) ? tc.col[c0 + l - 1] : -1;
+    }
+    float4 x[ST];
+#pragma unroll
+    for (int k = 0; k < ST; k++) {
+      const int i = i0 + threadIdx.x + k * kTgWg;
+      x[k] = col[k] >= 0 ? e4[(int64_t)col[k] * 16 + 8 * h + (i & 7)] : float4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int k = 0; k < ST; k++) {
+      const int i = i0 + threadIdx.x + k * kTgWg;
+      if (i < nq) tile_lds[i] = x[k];
+    }
+  }
+  __syncthreads();
+  const char *lb = reinterpret_cast<const char *>(tile_lds);
+  while (gi < g1) {
+    float aK[2] = {0.f, 0.f}, aM[2] = {0.f, 0.f}, aG[2] = {0.f, 0.f}, aP[2] = {0.f, 0.f};
+    grp_sum4(gs, first, len, (uint32_t)(lane % G) * 8u,
+             [&](uint32_t off) { return *reinterpret_cast<const typename XLine<1>::t *>(lb + off); }, aK, aM, aG,
+             aP);
+    if (r0 >= 0) {
+      // box_post_kernel's epilogue for the lane's frequency q
+      const float2 K = {aK[0], aK[1]}, M = {aM[0], aM[1]}, Gs = {aG[0], aG[1]};
+      const OpCoef &cf = coef[q];
+      const bool fl = a.field[r] == 1;
+      const float2 ck = fl ? float2{(float)cf.kf.x, (float)cf.kf.y} : float2{(float)cf.ks.x, (float)cf.ks.y};
+      const float2 cm = fl ? float2{(float)cf.mf.x, (float)cf.mf.y} : float2{(float)cf.ms.x, (float)cf.ms.y};
+      const float2 cg = {(float)cf.g.x, (float)cf.g.y};
+      const float2 t1 = cmul(ck, K), t2 = cmul(cm, M), t3 = cmul(cg, Gs);
+      const float axr = t1.x + t2.x + t3.x, axi = t1.y + t2.y + t3.y;
+      const double d = (fl ? cf.akf : cf.aks) * a.rK[r] + (fl ? cf.amf : cf.ams) * a.rM[r] + cf.ag * a.rG[r];
+      const float inv = (float)(1.0 / d);
+      const int64_t o = (int64_t)r * W + 2 * q;
+      z[o] = uu.x + aP[0] + (vv.x - axr) * inv;
+      z[o + 1] = uu.y + aP[1] + (vv.y - axi) * inv;
+    }
+    if ((gi += kTgWg / 64) < g1) open_group();
+  }
 }
 
 // y = P x (ADD = false) or y += P x (ADD = true) for a real CSR P (same
@@ -1629,6 +1793,17 @@ struct LevelData {
   DArr<uint32_t> ap_off, pp_off;
   DArr<float> ap_K, ap_M, ap_G, pp_v;
   PadCsr3 ap() const { return {ap_rp.p, ap_off.p, ap_K.p, ap_M.p, ap_G.p}; }
+  // the same on grouped half-line tiles (thbox_post_kernel; bt_on: chosen)
+  DArr<int32_t> bp_rp, bt_row, bt_cptr, bt_col, bt_grp, bg_rows;
+  DArr<uint32_t> bp_loff;
+  DArr<float> bp_K, bp_M, bp_G, bp_P;
+  int64_t n_btiles = 0;
+  int bt_lines = 0, bt_rows = 0;  // most lines of a tile, rows per tile (before splits)
+  bool bt_on = false;
+  BoxTiles box_tiles() const {
+    return {bp_rp.p, bp_loff.p, bp_K.p, bp_M.p, bp_G.p, bp_P.p, (int64_t)bp_K.n,
+            TileCsr{bt_row.p, bt_cptr.p, bt_col.p, nullptr, n_btiles}, bt_grp.p, bg_rows.p};
+  }
   template <typename T> OpArgs<T> args() const;
   struct Csr {
     const int32_t *rp, *ci;
@@ -1685,6 +1860,7 @@ struct elp::Model::SolverState {
   // uses it (op_chosen) for the requested op_tile_rows
   int op_tile_rows = 0;
   bool op_chosen = false;
+  bool box_chosen = false;  // the post-smoothing kernel was chosen (thbox or box_post)
   // the last sweep's solver of a small model (work vectors, captured
   // iteration graph), kept for the next sweep with the same options;
   // declared last: destroyed before the levels it refers to
@@ -1955,6 +2131,171 @@ static void build_box_products(LevelData &L, hipStream_t s) {
   L.ap_G.upload(c, vG);
   L.pp_off.upload(c, poff);
   L.pp_v.upload(c, pval);
+}
+
+// The box products and the P rows merged for thbox_post_kernel: per row its
+// (A P) entries in column order with the P value of each column (0 where P
+// has none), in tiles of `rows` consecutive rows (halved until their
+// distinct lattice columns number at most 255), the rows of a tile taken by
+// length 4 at a time and padded to one length per group; per entry the
+// LDS offset of its column's 128-byte half-line in the tile.  Returns false
+// (nothing built) when a P column is missing from its (A P) row.
+static bool build_box_tiles(LevelData &L, int rows) {
+  Model &m = *L.model;
+  const int64_t n = L.n, nc = L.nc;
+  constexpr uint32_t line = 32 * sizeof(float2);
+  std::vector<int32_t> arp;
+  std::vector<uint32_t> aoff, poff;
+  std::vector<float> aK, aM, aG, pv;
+  L.ap_rp.download(arp);
+  L.ap_off.download(aoff);
+  L.ap_K.download(aK);
+  L.ap_M.download(aM);
+  L.ap_G.download(aG);
+  L.pp_off.download(poff);
+  L.pp_v.download(pv);
+  // merged rows: lattice columns and P values
+  std::vector<int32_t> mrp(n + 1, 0), mcol;
+  std::vector<float> mP;
+  mcol.reserve(aoff.size());
+  mP.reserve(aoff.size());
+  for (int64_t r = 0; r < n; r++) {
+    const int32_t a0 = (int32_t)mcol.size();
+    for (int32_t p = arp[r]; p < arp[r + 1]; p++) {
+      if (aoff[p] >= (uint32_t)(nc * line)) continue;  // padding
+      mcol.push_back((int32_t)(aoff[p] / line));
+      mP.push_back(0.f);
+    }
+    for (int t = 0; t < 8; t++) {
+      const uint32_t o = poff[r * 8 + t];
+      if (o >= (uint32_t)(nc * line)) continue;
+      const int32_t c = (int32_t)(o / line);
+      const auto it = std::lower_bound(mcol.begin() + a0, mcol.end(), c);
+      if (it == mcol.end() || *it != c) return false;
+      mP[it - mcol.begin()] = pv[r * 8 + t];
+    }
+    mrp[r + 1] = (int32_t)mcol.size();
+  }
+  // the K, M, G values of the merged entries: the unpadded (A P) entries
+  std::vector<float> vK(mcol.size()), vM(mcol.size()), vG(mcol.size());
+  for (int64_t r = 0, q = 0; r < n; r++)
+    for (int32_t p = arp[r]; p < arp[r + 1]; p++) {
+      if (aoff[p] >= (uint32_t)(nc * line)) continue;
+      vK[q] = aK[p];
+      vM[q] = aM[p];
+      vG[q] = aG[p];
+      q++;
+    }
+  // tiles
+  struct Sub {
+    int32_t r0, r1;
+    std::vector<int32_t> cols;
+  };
+  const int64_t nch = (n + rows - 1) / rows;
+  std::vector<std::vector<Sub>> per(nch);
+  parallel_for(nch, [&](int64_t a, int64_t b) {
+    std::vector<int32_t> u;
+    for (int64_t ch = a; ch < b; ch++) {
+      std::vector<std::pair<int32_t, int32_t>> todo{{(int32_t)(ch * rows), (int32_t)std::min<int64_t>(n, (ch + 1) * rows)}};
+      while (!todo.empty()) {
+        const auto rr = todo.back();
+        todo.pop_back();
+        u.assign(mcol.begin() + mrp[rr.first], mcol.begin() + mrp[rr.second]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if ((int)u.size() > kTileMaxLines && rr.second - rr.first > 1) {
+          const int32_t mid = rr.first + (rr.second - rr.first) / 2;
+          todo.push_back({mid, rr.second});
+          todo.push_back({rr.first, mid});
+          continue;
+        }
+        per[ch].push_back({rr.first, rr.second, u});
+      }
+    }
+  });
+  std::vector<const Sub *> tl;
+  int most = 0;
+  for (auto &v : per)
+    for (auto &t : v) {
+      tl.push_back(&t);
+      most = std::max(most, (int)t.cols.size());
+    }
+  if (most > kTileMaxLines) return false;
+  // groups of 4 rows per tile, padded per group
+  std::vector<int32_t> plen(n, 0), idx, tgrp{0}, grows;
+  for (const Sub *t : tl) {
+    idx.resize(t->r1 - t->r0);
+    for (int32_t r = t->r0; r < t->r1; r++) idx[r - t->r0] = r;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int32_t a, int32_t b) { return mrp[a + 1] - mrp[a] > mrp[b + 1] - mrp[b]; });
+    for (size_t k = 0; k < idx.size(); k += 4) {
+      const int32_t longest = mrp[idx[k] + 1] - mrp[idx[k]];
+      const int32_t glen = (longest + kRowPad - 1) / kRowPad * kRowPad;
+      for (int u = 0; u < 4; u++) {
+        if (k + u < idx.size()) plen[idx[k + u]] = glen;
+        grows.push_back(k + u < idx.size() ? idx[k + u] : -1);
+      }
+    }
+    tgrp.push_back((int32_t)(grows.size() / 4));
+  }
+  std::vector<int32_t> brp(n + 1, 0);
+  for (int64_t r = 0; r < n; r++) brp[r + 1] = brp[r] + plen[r];
+  const int64_t np = brp[n];
+  std::vector<uint32_t> loff(np, 0);
+  std::vector<float> bK(np, 0.f), bM(np, 0.f), bG(np, 0.f), bP(np, 0.f);
+  parallel_for((int64_t)tl.size(), [&](int64_t a, int64_t b) {
+    for (int64_t k = a; k < b; k++) {
+      const Sub &t = *tl[k];
+      for (int32_t r = t.r0; r < t.r1; r++) {
+        int32_t q = brp[r];
+        for (int32_t p = mrp[r]; p < mrp[r + 1]; p++, q++) {
+          loff[q] = 128u * (uint32_t)(1 + (std::lower_bound(t.cols.begin(), t.cols.end(), mcol[p]) - t.cols.begin()));
+          bK[q] = vK[p];
+          bM[q] = vM[p];
+          bG[q] = vG[p];
+          bP[q] = mP[p];
+        }
+      }
+    }
+  });
+  std::vector<int32_t> trow{0}, tcp{0}, tcol;
+  for (const Sub *t : tl) {
+    trow.back() = t->r0;
+    trow.push_back(t->r1);
+    tcol.insert(tcol.end(), t->cols.begin(), t->cols.end());
+    tcp.push_back((int32_t)tcol.size());
+  }
+  Ctx *c = m.ctx;
+  L.bp_rp.upload(c, brp);
+  L.bp_loff.upload(c, loff);
+  L.bp_K.upload(c, bK);
+  L.bp_M.upload(c, bM);
+  L.bp_G.upload(c, bG);
+  L.bp_P.upload(c, bP);
+  L.bt_row.upload(c, trow);
+  L.bt_cptr.upload(c, tcp);
+  L.bt_col.upload(c, tcol);
+  L.bt_grp.upload(c, tgrp);
+  L.bg_rows.upload(c, grows);
+  L.n_btiles = (int64_t)trow.size() - 1;
+  L.bt_lines = most;
+  return true;
+}
+
+static void release_box_tiles(LevelData &L) {
+  L.bp_rp.release();
+  L.bp_loff.release();
+  L.bp_K.release();
+  L.bp_M.release();
+  L.bp_G.release();
+  L.bp_P.release();
+  L.bt_row.release();
+  L.bt_cptr.release();
+  L.bt_col.release();
+  L.bt_grp.release();
+  L.bg_rows.release();
+  L.n_btiles = 0;
+  L.bt_on = false;
 }
 
 // padded row pointer of the row kernel and the tiles: each row rounded up
@@ -2514,7 +2855,6 @@ constexpr int64_t kGraphAutoRows = 100000;
 
 // Launch of the 32-slot FP32 operator of a level: on its row tiles or row
 // groups when it has them, else the row kernel (all give the same bits)
-constexpr int kTileMaxLines = kTileStage * kTileWg / 16 - 1;  // 64 KB of LDS per tile with the zero line
 constexpr int kTgMaxLines = kTileMaxLines;  // 191 or 127 lines (more workgroups per CU) measured slower
 template <int MODE>
 static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x, const float *b, float *y,
@@ -2549,6 +2889,75 @@ static void launch_row_op(const LevelData &L, const OpCoef *coef, const float *x
                        L.padded(), coef, x, b, y);
   }
   ELP_HIP(hipGetLastError());
+}
+
+// the box cycle's fused post-smoothing: on the grouped half-line tiles when
+// the set-up chose them, else one row per wavefront (the same bits)
+static void launch_box_post(const LevelData &L, const OpCoef *coef, const float *e, int64_t ne, const float *u,
+                            const float *v, float *z, hipStream_t s, bool row_kernels = false) {
+  if (L.bt_on && !row_kernels) {
+    hipLaunchKernelGGL(thbox_post_kernel, dim3(xcd_grid(2 * L.n_btiles)), dim3(kTgWg), (size_t)(L.bt_lines + 1) * 128,
+                       s, L.template args<float>(), L.box_tiles(), coef, e, u, v, z);
+  } else {
+    hipLaunchKernelGGL(box_post_kernel, dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
+                       L.template args<float>(), L.ap(), L.pp_off.p, L.pp_v.p, coef, e, ne, u, v, z);
+  }
+  ELP_HIP(hipGetLastError());
+}
+
+// The post-smoothing kernel of level 0 (Hex20 box cycle): the grouped
+// half-line tiles (of 32, 64 or 128 rows) when they build and are faster on
+// this model than one row per wavefront, each timed over 3 launches (after
+// one untimed) on vectors of ones with the coefficients of a mid-range
+// frequency.
+static void choose_box_post(Model &m, Model::SolverState &st, hipStream_t s) {
+  LevelData &L = *st.levels[0];
+  if (st.box_chosen || !L.ap_rp.p || !st.fdm) return;
+  st.box_chosen = true;
+  if (!build_box_tiles(L, 32)) return;
+  const int64_t ne = fdm_nodes(*st.fdm);
+  DArr<float> ev, uv, vv, zv;
+  ev.alloc(m.ctx, ne * 64);
+  uv.alloc(m.ctx, m.n * 64);
+  vv.alloc(m.ctx, m.n * 64);
+  zv.alloc(m.ctx, m.n * 64);
+  std::vector<float> ones((size_t)std::max<int64_t>(ne, m.n) * 64, 1.0f);
+  ELP_HIP(hipMemcpyAsync(ev.p, ones.data(), sizeof(float) * ne * 64, hipMemcpyHostToDevice, s));
+  ELP_HIP(hipMemcpyAsync(uv.p, ones.data(), sizeof(float) * m.n * 64, hipMemcpyHostToDevice, s));
+  ELP_HIP(hipMemcpyAsync(vv.p, ones.data(), sizeof(float) * m.n * 64, hipMemcpyHostToDevice, s));
+  std::vector<OpCoef> hc(32, make_coef(m, 2 * M_PI * 500.0, m.Z, 0.0));
+  DArr<OpCoef> dc;
+  dc.upload(m.ctx, hc);
+  HipEvent e0, e1;
+  auto time_it = [&]() {
+    launch_box_post(L, dc.p, ev.p, ne, uv.p, vv.p, zv.p, s);
+    ELP_HIP(hipEventRecord(e0.e, s));
+    for (int k = 0; k < 3; k++) launch_box_post(L, dc.p, ev.p, ne, uv.p, vv.p, zv.p, s);
+    ELP_HIP(hipEventRecord(e1.e, s));
+    ELP_HIP(hipEventSynchronize(e1.e));
+    float ms = 0;
+    ELP_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+    return ms;
+  };
+  L.bt_on = false;
+  float best = time_it();  // one row per wavefront
+  int best_rows = 0, built = 32;
+  for (int rows : {32, 64, 128}) {
+    if (rows != 32) {
+      if (!build_box_tiles(L, rows)) break;
+      built = rows;
+    }
+    L.bt_on = true;
+    const float ms = time_it();
+    if (ms < best) {
+      best = ms;
+      best_rows = rows;
+    }
+  }
+  if (best_rows == 0) release_box_tiles(L);
+  else if (best_rows != built && !build_box_tiles(L, best_rows)) release_box_tiles(L);
+  L.bt_on = best_rows != 0 && L.n_btiles > 0;
+  L.bt_rows = best_rows;
 }
 
 // the variants of the 32-slot FP32 operator (elp_solve_opts.op_tile_rows):
@@ -2877,11 +3286,10 @@ struct Solver {
         op<1>(0, coefP.p, W.x.p, b, W.r.p);
         transfer<false>(nf, L.Rcsr<T>().rp, L.Rcsr<T>().ci, Rv(L), W.r.p, fb.p);
         fdm_solve(*st.fdm, B, coefF.p, fb.p, fe.p, fs1.p, fs2.p, s);
-        hipLaunchKernelGGL(box_post_kernel, dim3(xcd_grid(nblk(L.n, kRowWg / 64))), dim3(kRowWg), 0, s,
-                           L.template args<float>(), L.ap(), L.pp_off.p, L.pp_v.p, coefP.p,
-                           reinterpret_cast<const float *>(fe.p), nf, reinterpret_cast<const float *>(W.x.p),
-                           reinterpret_cast<const float *>(W.r.p), reinterpret_cast<float *>(z));
-        ELP_HIP(hipGetLastError());
+        // op_tile_rows = -1: one row per wavefront here too
+        launch_box_post(L, coefP.p, reinterpret_cast<const float *>(fe.p), nf, reinterpret_cast<const float *>(W.x.p),
+                        reinterpret_cast<const float *>(W.r.p), reinterpret_cast<float *>(z), s,
+                        o.op_tile_rows == -1);
         return;
       }
     }
@@ -4187,6 +4595,8 @@ std::unique_ptr<GroupDriver> group_driver(int key, Ctx *ctx, hipStream_t s) {
   }
 }
 
+static const LevelData &L0b(const Model::SolverState &st) { return *st.levels[0]; }
+
 template <typename T, int B>
 static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double *freqs, std::complex<double> Z,
                       const elp_solve_opts &o, double *X, double *relres, int32_t *iters, elp_sweep_stats *stats,
@@ -4196,6 +4606,7 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
       st.kept_solver.reset();  // its graph may hold the other variant's launches
       choose_operator(m, st, o, s);
     }
+    if (!st.box_chosen && !st.fp64_inner) choose_box_post(m, st, s);
   }
   // small models keep their solver (work vectors, captured graph) for the
   // next sweep with the same options: their sweeps are bound by host work
@@ -4233,6 +4644,7 @@ static void run_sweep(Model &m, Model::SolverState &st, int64_t nf, const double
                         nx * (int64_t)(4 + 1 + sizeof(T)) + 2 * m.n * B * (int64_t)sizeof(V);
     stats->refinements = refinements;
     stats->graph_replays = S.graph_replays;
+    stats->box_tile_rows = (S.fused_box() && L0b(st).bt_on && o.op_tile_rows != -1) ? L0b(st).bt_rows : 0;
     stats->refactors = S.refactors;
     stats->refactor_ms = S.refactor_ms;
     const LevelData &L0 = *st.levels[0];

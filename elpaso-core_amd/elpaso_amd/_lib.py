@@ -76,7 +76,7 @@ class SweepStats(C.Structure):
     _fields_ = [("setup_ms", C.c_double), ("solve_ms", C.c_double), ("spmv_calls", C.c_int64),
                 ("spmv_ms", C.c_double), ("spmv_bytes", C.c_int64), ("refinements", C.c_int),
                 ("unconverged", C.c_int), ("refactors", C.c_int), ("refactor_ms", C.c_double),
-                ("op_tile_rows", C.c_int), ("graph_replays", C.c_int64)]
+                ("op_tile_rows", C.c_int), ("graph_replays", C.c_int64), ("box_tile_rows", C.c_int)]
 
 
 def library_path():

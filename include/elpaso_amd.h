@@ -97,7 +97,8 @@ typedef struct elp_solve_opts {
   int mg_levels;     /* MG: maximum number of levels (0 = automatic)      */
   int mg_min_nodes;  /* MG: stop coarsening below this many nodes         */
   int op_tile_rows;  /* 32-slot FP32 operator variant: -1 one row per    */
-                     /* wavefront, -2 / -4 rows grouped 2 / 4 to a        */
+                     /* wavefront (also for the box cycle's fused post-   */
+                     /* smoothing), -2 / -4 rows grouped 2 / 4 to a       */
                      /* wavefront, t (< 1000) LDS tiles of t rows,        */
                      /* 1000 R + t LDS tiles of t rows with rows grouped  */
                      /* R = 2 or 4 to a wavefront (e.g. 4032), 6000 + t   */
@@ -122,6 +123,9 @@ typedef struct elp_sweep_stats {
   int op_tile_rows;    /* fine-level operator used (the op_tile_rows      */
                        /* encoding), 0 another operator (other slots)     */
   int64_t graph_replays; /* iterations replayed from the captured graph   */
+  int box_tile_rows;     /* Hex20 box cycle's post-smoothing: rows per     */
+                         /* grouped half-line tile, 0 one row per         */
+                         /* wavefront or no fused post-smoothing          */
 } elp_sweep_stats;
 
 /* ---- context -------------------------------------------------------------- */

@@ -300,7 +300,9 @@ def test_operator_variants_same_bits(ctx):
     over two workgroups) sum the same entries in the same order: the sweeps
     are the same bits, and the set-up's timing choice (op_tile_rows = 0) is
     one of them.  The row kernel runs again after the others (its padding
-    rebuilt)."""
+    rebuilt).  Variant -1 also runs the box cycle's post-smoothing one row
+    per wavefront; the others use the post-smoothing kernel the set-up chose
+    (grouped half-line tiles where faster): the same bits as well."""
     m = MG.acoustic_cavity(16, 12, 10, quadratic=True)
     g = GpuModel(ctx, m)
     g.assemble()
