@@ -182,8 +182,8 @@ def main():
         node_res = {"workload": f"grad p at the nodes of the Hex20 cavity, {len(idx)} of the sweep's solutions",
                     "kernel_ms_per_freq": round(tm["kernel_ms"] / len(idx), 3),
                     "wall_ms_per_freq": round(1e3 * wall / len(idx), 2),
-                    "kernel": "fluid_node_gradp_kernel<20> + node_results_kernel",
-                    "bound": "gathers of the solutions (20 per element and node)",
+                    "kernel": "fluid_item_gradp_kernel<20> + node_results_kernel",
+                    "bound": "latency of the gathers of the solutions (20 per element and node)",
                     "note": "wall time includes the host copies of the solutions in and the 9 results per node out"}
         del Xs, nd
         log(f"[bench] node results: {node_res}")

@@ -188,6 +188,11 @@ struct KindBlock {
   // matrices (records in node order, so the gather reads them contiguously)
   std::vector<int32_t> row_at;
   DArr<int32_t> d_row_at;
+  // fluid kinds: runs of whole entries of `nodes` for the node-results
+  // gradient kernel (block b: entries grad_blk[b] .. grad_blk[b + 1]); empty
+  // if a node is in too many elements for one block
+  std::vector<int32_t> grad_blk;
+  DArr<int32_t> d_grad_blk;
   DArr<int32_t> d_conn, d_nodes, d_adj_ptr, d_adj_elem, d_h8_desc;
   DArr<uint8_t> d_adj_local, d_adj_slot;
   DArr<double> d_params;

@@ -9,8 +9,8 @@
 //
 // Layout: solutions X[f][row] (complex), f over a chunk of frequencies.  Per
 // chunk: one thread per (brick, frequency) for the centroid stresses, one per
-// (node, frequency) summing them over the node's bricks, and one per fluid
-// node applying each of its elements' gradient operators to all of the
+// (node, frequency) summing them over the node's bricks, and one per (fluid
+// node, element) applying the element's gradient at the node to all of the
 // chunk's frequencies.
 #include <algorithm>
 #include <array>
@@ -48,7 +48,8 @@ __device__ inline double jac_inverse(const double J[3][3], double inv[3][3]) {
   return det;
 }
 
-// grad p at the fluid nodes: one thread per node of a fluid kind; for each
+// grad p at the fluid nodes: one thread per node of a fluid kind (kinds that
+// fluid_item_gradp_kernel below does not take); for each
 // of the node's elements in ascending id the element's gradient operator at
 // the node (J^-1 dN, frequency independent) is formed once and applied to the
 // chunk's nf solutions, each added to the node's slots 6..8 of out[f][r][9]
@@ -116,6 +117,108 @@ __global__ __launch_bounds__(kPostThreads) void fluid_node_gradp_kernel(
         o[a] = v;
       }
     }
+  }
+}
+
+// grad p at the fluid nodes, one thread per (node, element) entry of the
+// kind's adjacency.  The kind's nodes are cut into runs of whole nodes with
+// at most kGradItems entries (run r: entries blk[r] .. blk[r + 1] of
+// `knodes`), one block of kGradItems threads each.  Each thread forms its
+// element's inverse Jacobian at the node once (the table T of shape-function
+// derivatives at the element's nodes staged in LDS) and keeps it and the
+// byte offsets of the element's rows across the chunk's frequencies; per
+// frequency it gathers the element's nodal pressures, takes the derivatives
+// along the element's axes, r_c = sum_k dN_k/dxi_c p_k, in local node order
+// and maps them to grad p = J^-1 r (one FMA chain per component;
+// J(c, d) = dx_d / dxi_c).  Thread t < (nodes of the run) then adds the
+// gradients of node t's elements to its slots 6..8 of out in ascending
+// element id (loaded before the gathers, so that their latency overlaps), as
+// the per-node kernel above does; the two differ only in the association of
+// the same products.  Needs n * 16 < 2^31 (32-bit buffer offsets into a
+// frequency's solution).
+constexpr int kGradItems = 256;
+template <int NPE>
+__global__ __launch_bounds__(kGradItems) void fluid_item_gradp_kernel(
+    const int32_t *__restrict__ blk, const int32_t *__restrict__ knodes, const int32_t *__restrict__ adj_ptr,
+    const int32_t *__restrict__ adj_elem, const uint8_t *__restrict__ adj_local, const int32_t *__restrict__ conn,
+    const double *__restrict__ xyz, const int32_t *__restrict__ node_row, const double *__restrict__ T, int nf,
+    int64_t n, int64_t n_nodes, const double2 *__restrict__ X, double2 *__restrict__ out) {
+  constexpr int kTa = 3 * NPE + 1;
+  __shared__ double sT[NPE * kTa];
+  __shared__ double2 sG[kGradItems][3];
+  __shared__ int32_t sSeg[kGradItems + 1];
+  const int t = threadIdx.x;
+  const int32_t nb0 = blk[blockIdx.x], nb1 = blk[blockIdx.x + 1];
+  const int32_t ip0 = adj_ptr[nb0], ni = adj_ptr[nb1] - ip0;
+  for (int u = t; u < NPE * 3 * NPE; u += kGradItems) sT[(u / (3 * NPE)) * kTa + u % (3 * NPE)] = T[u];
+  for (int u = t; u <= nb1 - nb0; u += kGradItems) sSeg[u] = adj_ptr[nb0 + u] - ip0;
+  __syncthreads();
+  const bool has = t < ni;
+  const int32_t j = ip0 + min(t, max(ni - 1, 0));
+  const int64_t e = adj_elem[j];
+  const double *Ta = sT + (int)adj_local[j] * kTa;
+  uint32_t ro[NPE];
+  double inv[3][3];
+  {
+    double J[3][3] = {};
+#pragma unroll
+    for (int k = 0; k < NPE; k++) {
+      const int32_t q = conn[e * NPE + k];
+      const double x0 = xyz[(int64_t)q * 3], x1 = xyz[(int64_t)q * 3 + 1], x2 = xyz[(int64_t)q * 3 + 2];
+      ro[k] = (uint32_t)node_row[q] * 16u;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double d = Ta[c * NPE + k];
+        J[c][0] += d * x0;
+        J[c][1] += d * x1;
+        J[c][2] += d * x2;
+      }
+    }
+    jac_inverse(J, inv);
+  }
+  const int32_t nn = nb1 - nb0;
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  for (int f = 0; f < nf; f++) {
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(X + (int64_t)f * n), (short)0, (int)(n * 16), 0x00020000);
+    double2 *o = out + ((int64_t)f * n_nodes + knodes[nb0 + min(t, nn - 1)]) * 9 + 6;
+    double2 v[3] = {};
+    if (t < nn) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) v[a] = o[a];
+    }
+    if (has) {
+      double2 r[3] = {};
+#pragma unroll
+      for (int k = 0; k < NPE; k++) {
+        const d2v p = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(xr, ro[k], 0, 0));
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double d = Ta[c * NPE + k];
+          r[c].x = fma(d, p.x, r[c].x);
+          r[c].y = fma(d, p.y, r[c].y);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        double2 g;
+        g.x = fma(inv[a][2], r[2].x, fma(inv[a][1], r[1].x, inv[a][0] * r[0].x));
+        g.y = fma(inv[a][2], r[2].y, fma(inv[a][1], r[1].y, inv[a][0] * r[0].y));
+        sG[t][a] = g;
+      }
+    }
+    __syncthreads();
+    if (t < nn) {
+      for (int32_t u = sSeg[t]; u < sSeg[t + 1]; u++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          v[a].x += sG[u][a].x;
+          v[a].y += sG[u][a].y;
+        }
+#pragma unroll
+      for (int a = 0; a < 3; a++) o[a] = v[a];
+    }
+    __syncthreads();
   }
 }
 
@@ -278,6 +381,25 @@ void post_prepare(Model &m) {
     }
   m.d_post_slot.upload(c, slot);
   m.d_post_pos.upload(c, pos);
+  // runs of whole nodes with at most kGradItems adjacency entries, one block
+  // of fluid_item_gradp_kernel each (none if a node has more, or none, or the
+  // solution of one frequency is too long for 32-bit offsets)
+  for (auto &kb : m.kinds) {
+    kb.grad_blk.clear();
+    if (!is_fluid(kb.kind) || kb.nodes.empty()) continue;
+    const int32_t nk = (int32_t)kb.nodes.size();
+    std::vector<int32_t> b{0};
+    bool fits = m.n * 16 < (int64_t)INT32_MAX;
+    for (int32_t i = 0; i < nk && fits; i++) {
+      fits = kb.adj_ptr[i + 1] - kb.adj_ptr[i] >= 1 && kb.adj_ptr[i + 1] - kb.adj_ptr[i] <= kGradItems;
+      if (kb.adj_ptr[i + 1] - kb.adj_ptr[b.back()] > kGradItems || i - b.back() == kGradItems) b.push_back(i);
+    }
+    b.push_back(nk);
+    if (fits) {
+      kb.grad_blk = b;
+      kb.d_grad_blk.upload(c, b);
+    }
+  }
   m.d_T8.upload(c, hex_nodal_derivs(8));
   m.d_T20.upload(c, hex_nodal_derivs(20));
   const std::vector<std::array<double, 3>> centroid(1, std::array<double, 3>{0.0, 0.0, 0.0});
@@ -315,6 +437,20 @@ void launch_post(Model &m, int nc, const double *w, const double2 *dX, double2 *
   for (auto &kb : m.kinds) {
     const int64_t nk = (int64_t)kb.nodes.size();
     if (!is_fluid(kb.kind) || nk == 0) continue;
+    if (!kb.grad_blk.empty()) {
+      const int32_t nr = (int32_t)kb.grad_blk.size() - 1;
+      const dim3 gb((unsigned)nr);
+      if (kb.kind == ELP_FLUID8)
+        hipLaunchKernelGGL(fluid_item_gradp_kernel<8>, gb, dim3(kGradItems), 0, s, kb.d_grad_blk.p,
+                           kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p,
+                           m.d_node_row.p, m.d_T8.p, nc, n, N, dX, dN);
+      else
+        hipLaunchKernelGGL(fluid_item_gradp_kernel<20>, gb, dim3(kGradItems), 0, s, kb.d_grad_blk.p,
+                           kb.d_nodes.p, kb.d_adj_ptr.p, kb.d_adj_elem.p, kb.d_adj_local.p, kb.d_conn.p, m.d_xyz.p,
+                           m.d_node_row.p, m.d_T20.p, nc, n, N, dX, dN);
+      ELP_HIP(hipGetLastError());
+      continue;
+    }
     dim3 g1((unsigned)((nk + kPostThreads - 1) / kPostThreads));
     if (kb.kind == ELP_FLUID8)
       hipLaunchKernelGGL(fluid_node_gradp_kernel<8>, g1, dim3(kPostThreads), 0, s, nk, kb.d_nodes.p, kb.d_adj_ptr.p,
