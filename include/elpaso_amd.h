@@ -91,7 +91,10 @@ typedef struct elp_solve_opts {
   int max_iters;     /* Krylov iterations per frequency                   */
   double inner_rtol; /* mixed: FP32 inner reduction per refinement step   */
                      /* (the last step aims just past rtol when it can)   */
-  double shift;      /* shifted-Laplacian damping beta (MG)               */
+  double shift;      /* shifted-Laplacian damping beta (MG); < 0 (the     */
+                     /* default) automatic: 0 where the box solve is    */
+                     /* exact (Hex8 box, coupled cavity), growing with  */
+                     /* k h on Hex20 and stand-in boxes, 0.5 otherwise  */
   int smooth;        /* MG pre/post smoothing sweeps                      */
   int coarse_sweeps; /* MG coarsest-level smoothing sweeps                */
   int mg_levels;     /* MG: maximum number of levels (0 = automatic)      */
