@@ -3822,7 +3822,12 @@ struct SlotRun {
   // window of frequencies: the sweep's range cut into as many equal windows
   // as it has slot-sized groups of frequencies, each factored at its centre.
   // Frequencies are taken in descending order, so the windows are used one
-  // after the other and the next one is factored in the background.
+  // after the other and the next one is factored in the background.  A slot
+  // takes a frequency of the next window only when the current window's
+  // frequencies are done: switched in the middle of a window, the slots'
+  // frequencies iterated under a factorization a window away (on a 152k-DOF
+  // coupled model over 10-2000 Hz 44 iterations per frequency instead of 25,
+  // at most 354 instead of 46).
   bool cpl = false;
   double ftop = 0, fbot = 0, wwidth = 0;
   int64_t nwin = 1, cur_win = -1, want_win = -1;
@@ -3937,8 +3942,17 @@ struct SlotRun {
     q.n0 = {1.0, q.w > 0 ? rs * q.refs / q.w : 0.0};
   }
   // assign the next frequency to slot k (stream idle)
+  // a slot of the current window still iterating (not counting slot k)
+  bool window_busy(int k) const {
+    for (int j = 0; j < B; j++)
+      if (j != k && sl[j].state != 0 && window_of(freqs[sl[j].f]) == cur_win) return true;
+    return false;
+  }
   bool assign(int k) {
-    if (next >= nf) {
+    if (next >= nf || (cpl && cur_win >= 0 && window_of(freqs[order[next]]) != cur_win && window_busy(k))) {
+      // coupled: a frequency of the next window waits until the current
+      // window's frequencies are done, so that every frequency iterates
+      // under its own window's structure factorization
       sl[k] = Slot();
       act[k] = 0;
       return false;
@@ -4021,6 +4035,7 @@ struct SlotRun {
   }
   void start() {
     if (done) return;
+    if (cpl) cur_win = 0;
     for (int k = 0; k < B; k++) assign(k);
     upload_coefs();
     if (cpl) use_window(0);
@@ -4156,6 +4171,9 @@ void SlotRun<T, B>::after_norms() {
       scale[k] = 1.0 / q.w;
     }
   }
+  if (cpl)
+    for (int k = 0; k < B; k++)
+      if (sl[k].state == 0) any_new |= assign(k);
   if (need_refactor) {
     // slots in the middle of an inner solve keep their progress (x64 +=
     // w d) and restart on the new residual under the new factorization
