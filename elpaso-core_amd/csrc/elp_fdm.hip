@@ -190,56 +190,80 @@ __global__ __launch_bounds__(256) void axis_transform_kernel(int64_t outer, int 
   }
 }
 
-// FP64 transforms on the matrix cores, out[o][p][c] = sum_l Mat[p][l]
-// in[o][l][c] with v_mfma_f64_16x16x4f64: one persistent 512-thread
-// workgroup per CU stages Mat once in LDS (rows padded with zeros to RB 16,
-// columns to a multiple of 4, row stride KP + 1), and each wavefront walks
-// strips of 16 columns (o, c0 .. c0+15) on its own: lane j + 16 k loads the B
-// fragment element in[o][4 ks + k][c0 + j] straight from memory, 8 reduction
-// steps ahead of the matrix cores (the next strip's first steps while the
-// current one finishes), and accumulates the RB row blocks of the strip.
-// Fragment layout of the f64 16x16x4 product: A element (i, k) from lane
-// i + 16 k, B element (k, j) from lane j + 16 k, result (l / 16 + 4 r, l % 16)
-// in element r of lane l.  Each product adds its 4 terms in order and the
-// padded terms are exact zeros: the same bits as axis_transform_kernel.
-// On the 101 x 101 x 121 lattice of a 1.2 m cavity with 32 frequencies
-// (630 MB per vector) 0.47 ms per transform against 1.21 ms.
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
-constexpr int kMfma64MaxAxis = 128;  // RB <= 8: 128 x 129 doubles of LDS
-template <int RB>
-__global__ __launch_bounds__(512) void axis_mfma64_kernel(int64_t outer, int L, int64_t inner, int64_t nstrips,
-                                                          const double *__restrict__ Mat,
-                                                          const double *__restrict__ in, double *__restrict__ out) {
-  extern __shared__ double sA[];
+// Transforms on the matrix cores, out[o][p][c] = sum_l Mat[p][l] in[o][l][c]
+// (v_mfma_f32_16x16x4f32 for FP32, v_mfma_f64_16x16x4f64 for FP64): one
+// persistent 512-thread workgroup per CU stages Mat once in LDS (rows padded
+// with zeros to RB 16, columns to a multiple of 4, row stride KP + 1), and
+// each wavefront walks strips of 16 columns (o, c0 .. c0+15) on its own:
+// lane j + 16 k loads the B fragment element in[o][4 ks + k][c0 + j] straight
+// from memory, CH reduction steps ahead of the matrix cores (the next
+// strip's first steps while the current one finishes), and accumulates the
+// RB row blocks of the strip.  Fragment layouts (checked on the device): A
+// element (i, k) from lane i + 16 k, B element (k, j) from lane j + 16 k;
+// the result (4 (l / 16) + r, l % 16) in element r of lane l for FP32,
+// (l / 16 + 4 r, l % 16) for FP64.  Each product adds its 4 terms in order
+// and the padded terms are exact zeros: the same bits as
+// axis_transform_kernel and as the earlier FP32 kernel that staged 64-column
+// tiles in LDS beside Mat.  Measured per transform: FP64 on the 101 x 101 x
+// 121 lattice of a 1.2 m cavity with 32 frequencies (630 MB per vector)
+// 0.44 ms against 1.21 ms on the vector units; FP32 on the 101 x 81 x 61
+// lattice of the Hex20 cavity (128 MB) 0.14 / 0.11 ms (x / y) against
+// 0.20 / 0.16 ms with LDS-staged tiles.
+template <typename T> struct MfmaStrip;
+template <> struct MfmaStrip<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static constexpr int kMaxAxis = 160;  // RB <= 10: 160 x 161 floats of LDS
+  static constexpr int kChunk = 8;
+  __device__ static acc_t mfma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int k, int r) { return 4 * k + r; }
+};
+template <> struct MfmaStrip<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static constexpr int kMaxAxis = 128;  // RB <= 8: 128 x 129 doubles of LDS
+  static constexpr int kChunk = 16;
+  __device__ static acc_t mfma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int k, int r) { return k + 4 * r; }
+};
+template <typename T, int RB>
+__global__ __launch_bounds__(512) void axis_strip_kernel(int64_t outer, int L, int64_t inner, int64_t nstrips,
+                                                         const T *__restrict__ Mat, const T *__restrict__ in,
+                                                         T *__restrict__ out) {
+  using MS = MfmaStrip<T>;
+  using acc_t = typename MS::acc_t;
+  constexpr int CH = MS::kChunk;
+  extern __shared__ unsigned char smem_strip[];
+  T *sA = reinterpret_cast<T *>(smem_strip);
   const int KP = (L + 3) / 4 * 4, S = KP + 1;
   for (int idx = threadIdx.x; idx < RB * 16 * KP; idx += 512) {
     const int p = idx / KP, l = idx % KP;
-    sA[p * S + l] = (p < L && l < L) ? Mat[(int64_t)p * L + l] : 0.0;
+    sA[p * S + l] = (p < L && l < L) ? Mat[(int64_t)p * L + l] : T(0);
   }
   __syncthreads();
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int i = lane % 16, k = lane / 16;
   const int64_t spl = inner / 16;
-  const int nks = KP / 4, nch = (nks + 7) / 8;
+  const int nks = KP / 4, nch = (nks + CH - 1) / CH;
   const int64_t stride = (int64_t)gridDim.x * 8;
-  // B fragments of chunk c (reduction steps 8 c .. 8 c + 7) of strip st; 0
-  // past the last strip or the last row
-  auto load = [&](int64_t st, int c, double *bb) {
+  // B fragments of chunk c (reduction steps CH c .. CH c + CH - 1) of strip
+  // st; 0 past the last strip or the last row
+  auto load = [&](int64_t st, int c, T *bb) {
     const int64_t o = st / spl, c0 = (st % spl) * 16;
-    const double *src = in + o * L * inner + c0 + i;
+    const T *src = in + o * L * inner + c0 + i;
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int l = (c * 8 + u) * 4 + k;
-      bb[u] = (st < nstrips && l < L) ? src[(int64_t)l * inner] : 0.0;
+    for (int u = 0; u < CH; u++) {
+      const int l = (c * CH + u) * 4 + k;
+      bb[u] = (st < nstrips && l < L) ? src[(int64_t)l * inner] : T(0);
     }
   };
   int64_t strip = (int64_t)blockIdx.x * 8 + wave;
   int ch = 0;
-  double bc[8], bn[8];
+  T bc[CH], bn[CH];
   load(strip, 0, bc);
-  mfma_d4 acc[RB];
+  acc_t acc[RB];
 #pragma unroll
-  for (int pb = 0; pb < RB; pb++) acc[pb] = mfma_d4{0, 0, 0, 0};
+  for (int pb = 0; pb < RB; pb++) acc[pb] = acc_t{0, 0, 0, 0};
   while (strip < nstrips) {
     int64_t ns = strip;
     int nc = ch + 1;
@@ -249,111 +273,30 @@ __global__ __launch_bounds__(512) void axis_mfma64_kernel(int64_t outer, int L, 
     }
     load(ns, nc, bn);
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int ks = ch * 8 + u;
+    for (int u = 0; u < CH; u++) {
+      const int ks = ch * CH + u;
       if (ks < nks) {
 #pragma unroll
-        for (int pb = 0; pb < RB; pb++) {
-          const double a = sA[(pb * 16 + i) * S + ks * 4 + k];
-          acc[pb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bc[u], acc[pb], 0, 0, 0);
-        }
+        for (int pb = 0; pb < RB; pb++) acc[pb] = MS::mfma(sA[(pb * 16 + i) * S + ks * 4 + k], bc[u], acc[pb]);
       }
     }
     if (ch == nch - 1) {
       const int64_t o = strip / spl, c0 = (strip % spl) * 16;
-      double *dst = out + o * L * inner + c0 + i;
+      T *dst = out + o * L * inner + c0 + i;
 #pragma unroll
       for (int pb = 0; pb < RB; pb++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int p = pb * 16 + k + 4 * r;
+          const int p = pb * 16 + MS::row(k, r);
           if (p < L) dst[(int64_t)p * inner] = acc[pb][r];
         }
-        acc[pb] = mfma_d4{0, 0, 0, 0};
+        acc[pb] = acc_t{0, 0, 0, 0};
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++) bc[u] = bn[u];
+    for (int u = 0; u < CH; u++) bc[u] = bn[u];
     strip = ns;
     ch = nc;
-  }
-}
-
-// FP32 transforms on the matrix cores: out[o][p][c] = sum_l Mat[p][l] in[o][l][c]
-// as GEMMs Mat (L x L) x tile (L x 64) with v_mfma_f32_16x16x4f32.  A
-// workgroup stages Mat once in LDS (row stride KP + 1 words: conflict-free
-// reads of the A fragments) and walks the 64-column tiles t = blockIdx.x,
-// blockIdx.x + gridDim.x, ...: each tile (columns as in axis_transform_kernel)
-// is staged in LDS with a row stride of 80 words (conflict-free B fragments),
-// wave w computes output columns 16w .. 16w+15 for all RT row tiles.
-// Fragment layouts (checked on the device): A element (i, k) from lane
-// i + 16 k, B element (k, j) from lane j + 16 k, result (4 (l / 16) + r, l % 16)
-// in element r of lane l.
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-constexpr int kMfmaTileStride = 80;
-// L (L + 5) + 80 (L + 3) words of LDS must fit the 160 KB of a CU
-constexpr int kMfmaMaxAxis = 160;
-template <int RT>
-__global__ __launch_bounds__(256) void axis_mfma_kernel(int64_t outer, int L, int64_t inner, int64_t ntiles,
-                                                        const float *__restrict__ Mat, const float *__restrict__ in,
-                                                        float *__restrict__ out) {
-  constexpr int CW = 64;
-  constexpr int NPRE = ((kMfmaMaxAxis + 3) / 4 * 4 * (CW / 4) + 255) / 256;  // float4 per thread per tile
-  extern __shared__ float sm[];
-  const int KP = (L + 3) / 4 * 4, KS = KP + 1;
-  float *ma = sm;                   // [L][KS]: ma[p * KS + l] = Mat[p][l], zero for l >= L
-  float *tb = sm + (size_t)L * KS;  // [KP][kMfmaTileStride]
-  for (int idx = threadIdx.x; idx < L * KP; idx += 256) {
-    const int p = idx / KP, l = idx % KP;
-    ma[p * KS + l] = l < L ? Mat[(int64_t)p * L + l] : 0.f;
-  }
-  // inner is a multiple of CW here: tile t = columns (t % tpl) CW .. + CW of
-  // line t / tpl, loaded as float4 rows
-  const int64_t tpl = inner / CW;
-  const int nv = KP * (CW / 4);
-  auto load = [&](int64_t t, float4 *v) {
-    const float *src = in + (t / tpl) * L * inner + (t % tpl) * CW;
-#pragma unroll
-    for (int u = 0; u < NPRE; u++) {
-      const int idx = threadIdx.x + u * 256;
-      const int l = idx / (CW / 4), q = idx % (CW / 4);
-      v[u] = (idx < nv && l < L) ? *reinterpret_cast<const float4 *>(src + (int64_t)l * inner + 4 * q)
-                                 : float4{0.f, 0.f, 0.f, 0.f};
-    }
-  };
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = lane & 15, lk = lane >> 4;
-  float4 pre[NPRE];
-  if ((int64_t)blockIdx.x < ntiles) load(blockIdx.x, pre);
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    __syncthreads();  // the previous tile's reads of tb are done (and ma is staged)
-#pragma unroll
-    for (int u = 0; u < NPRE; u++) {
-      const int idx = threadIdx.x + u * 256;
-      if (idx < nv) *reinterpret_cast<float4 *>(tb + (idx / (CW / 4)) * kMfmaTileStride + 4 * (idx % (CW / 4))) = pre[u];
-    }
-    __syncthreads();
-    if (t + gridDim.x < ntiles) load(t + gridDim.x, pre);  // in flight during the products
-    mfma_f4 acc[RT];
-#pragma unroll
-    for (int rt = 0; rt < RT; rt++) acc[rt] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < KP; k0 += 4) {
-      const float bv = tb[(k0 + lk) * kMfmaTileStride + 16 * w + li];
-#pragma unroll
-      for (int rt = 0; rt < RT; rt++) {
-        const int row = 16 * rt + li;
-        const float av = row < L ? ma[row * KS + k0 + lk] : 0.f;
-        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[rt], 0, 0, 0);
-      }
-    }
-    float *dst = out + (t / tpl) * L * inner + (t % tpl) * CW + 16 * w + li;
-#pragma unroll
-    for (int rt = 0; rt < RT; rt++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int p = 16 * rt + 4 * lk + r;
-        if (p < L) dst[(int64_t)p * inner] = acc[rt][r];
-      }
   }
 }
 
@@ -449,65 +392,38 @@ static void launch_axis(int64_t outer, int L, int64_t inner, const T *Mat, const
   ELP_HIP(hipGetLastError());
 }
 
-template <int RT>
-static void launch_axis_mfma(int64_t outer, int L, int64_t inner, const float *Mat, const float *in, float *out,
-                             hipStream_t s) {
-  constexpr int CW = 64;
+template <typename T, int RB>
+static void launch_axis_strip(int64_t outer, int L, int64_t inner, const T *Mat, const T *in, T *out, hipStream_t s) {
   const int KP = (L + 3) / 4 * 4;
-  const size_t lds = ((size_t)L * (KP + 1) + (size_t)KP * kMfmaTileStride) * sizeof(float);
-  const int64_t ntiles = outer * (inner / CW);
-  const auto kern = axis_mfma_kernel<RT>;
-  ELP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 1024);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, outer, L, inner, ntiles, Mat, in, out);
-  ELP_HIP(hipGetLastError());
-}
-
-template <int RB>
-static void launch_axis_mfma64(int64_t outer, int L, int64_t inner, const double *Mat, const double *in, double *out,
-                               hipStream_t s) {
-  const int KP = (L + 3) / 4 * 4;
-  const size_t lds = (size_t)RB * 16 * (KP + 1) * sizeof(double);
+  const size_t lds = (size_t)RB * 16 * (KP + 1) * sizeof(T);
   const int64_t nstrips = outer * (inner / 16);
-  const auto kern = axis_mfma64_kernel<RB>;
+  const auto kern = axis_strip_kernel<T, RB>;
   ELP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const unsigned grid = (unsigned)std::min<int64_t>((nstrips + 7) / 8, 256);  // one workgroup per CU
   hipLaunchKernelGGL(kern, dim3(grid), dim3(512), lds, s, outer, L, inner, nstrips, Mat, in, out);
   ELP_HIP(hipGetLastError());
 }
 
+
 // transform along an axis of L <= kFdmMaxAxis points; inner counts reals
 template <typename T, typename V>
 static void transform(int64_t outer, int L, int64_t inner, const T *Mat, const V *in, V *out, hipStream_t s) {
-  if constexpr (std::is_same<T, float>::value) {
-    if (L <= kMfmaMaxAxis && inner % 64 == 0) {  // whole 64-column tiles (B = 32 frequencies)
-      const float *i = reinterpret_cast<const float *>(in);
-      float *o = reinterpret_cast<float *>(out);
-      const int rt = (L + 15) / 16;
-      if (rt <= 2) launch_axis_mfma<2>(outer, L, inner, Mat, i, o, s);
-      else if (rt <= 4) launch_axis_mfma<4>(outer, L, inner, Mat, i, o, s);
-      else if (rt <= 6) launch_axis_mfma<6>(outer, L, inner, Mat, i, o, s);
-      else if (rt <= 8) launch_axis_mfma<8>(outer, L, inner, Mat, i, o, s);
-      else launch_axis_mfma<12>(outer, L, inner, Mat, i, o, s);
-      return;
+  if (L <= MfmaStrip<T>::kMaxAxis && inner % 16 == 0) {  // whole 16-column strips (B >= 8 frequencies)
+    const T *i = reinterpret_cast<const T *>(in);
+    T *o = reinterpret_cast<T *>(out);
+    switch ((L + 15) / 16) {
+      case 1: launch_axis_strip<T, 1>(outer, L, inner, Mat, i, o, s); break;
+      case 2: launch_axis_strip<T, 2>(outer, L, inner, Mat, i, o, s); break;
+      case 3: launch_axis_strip<T, 3>(outer, L, inner, Mat, i, o, s); break;
+      case 4: launch_axis_strip<T, 4>(outer, L, inner, Mat, i, o, s); break;
+      case 5: launch_axis_strip<T, 5>(outer, L, inner, Mat, i, o, s); break;
+      case 6: launch_axis_strip<T, 6>(outer, L, inner, Mat, i, o, s); break;
+      case 7: launch_axis_strip<T, 7>(outer, L, inner, Mat, i, o, s); break;
+      case 8: launch_axis_strip<T, 8>(outer, L, inner, Mat, i, o, s); break;
+      case 9: launch_axis_strip<T, std::is_same<T, float>::value ? 9 : 8>(outer, L, inner, Mat, i, o, s); break;
+      default: launch_axis_strip<T, std::is_same<T, float>::value ? 10 : 8>(outer, L, inner, Mat, i, o, s); break;
     }
-  }
-  if constexpr (std::is_same<T, double>::value) {
-    if (L <= kMfma64MaxAxis && inner % 16 == 0) {  // whole 16-column strips (B >= 8 frequencies)
-      const double *i = reinterpret_cast<const double *>(in);
-      double *o = reinterpret_cast<double *>(out);
-      switch ((L + 15) / 16) {
-        case 1: launch_axis_mfma64<1>(outer, L, inner, Mat, i, o, s); break;
-        case 2: launch_axis_mfma64<2>(outer, L, inner, Mat, i, o, s); break;
-        case 3: launch_axis_mfma64<3>(outer, L, inner, Mat, i, o, s); break;
-        case 4: launch_axis_mfma64<4>(outer, L, inner, Mat, i, o, s); break;
-        case 5: launch_axis_mfma64<5>(outer, L, inner, Mat, i, o, s); break;
-        case 6: launch_axis_mfma64<6>(outer, L, inner, Mat, i, o, s); break;
-        case 7: launch_axis_mfma64<7>(outer, L, inner, Mat, i, o, s); break;
-        default: launch_axis_mfma64<8>(outer, L, inner, Mat, i, o, s); break;
-      }
-      return;
-    }
+    return;
   }
   constexpr int CW = sizeof(T) == 4 ? 64 : 32;
   ELP_REQUIRE(L <= kFdmMaxAxis && inner > 0, ELP_EINVAL, "box transform shape");
