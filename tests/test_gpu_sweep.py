@@ -380,3 +380,20 @@ def test_exact_box_sweeps_unshifted_to_2khz(ctx):
           f"refactors {st.refactors}", flush=True)
     assert (rel <= 1e-8).all() and st.unconverged == 0
     assert it.mean() < 25 and it.max() <= 50, it
+
+
+def test_long_box_axis_sweep(ctx):
+    """A Hex8 duct 140 elements long (2.8 x 0.06 x 0.04 m): the box solve's
+    transform along x has 141 points, 9 row blocks of the matrix-core strip
+    kernel in FP32 (the Hex20 cavity of the benchmark uses 7); mixed sweep
+    on 32 slots against elPaSo's system and SuperLU."""
+    m = MG.acoustic_cavity(140, 3, 2, L=(2.8, 0.06, 0.04))
+    g = GpuModel(ctx, m)
+    g.assemble()
+    assert g.get_box() is not None
+    freqs = MG.linear_frequencies(10, 2000, 24)
+    X, rel, it, st = g.sweep(freqs, block=32)
+    print(f"[sweep] duct 140x3x2: iterations mean {it.mean():.1f} max {it.max()}", flush=True)
+    assert (rel <= 1e-8).all() and st.unconverged == 0, (rel, it)
+    assert it.mean() < 20, it
+    _check_forward_error(m, freqs[::6], X[::6])
