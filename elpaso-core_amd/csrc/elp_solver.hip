@@ -110,6 +110,39 @@ template <int B> struct OpShape {
   static constexpr int RPB = 256 / G;        // rows per block
 };
 
+// (A x)[r][f] from a row's K, M and extra sums and the frequency's
+// coefficients (fl: a fluid row)
+template <typename T>
+__device__ inline typename VT<T>::t op_value(const OpCoef &cf, bool fl, typename VT<T>::t accK,
+                                             typename VT<T>::t accM, typename VT<T>::t accX) {
+  using V = typename VT<T>::t;
+  const V ck = fl ? V{(T)cf.kf.x, (T)cf.kf.y} : V{(T)cf.ks.x, (T)cf.ks.y};
+  const V cm = fl ? V{(T)cf.mf.x, (T)cf.mf.y} : V{(T)cf.ms.x, (T)cf.ms.y};
+  V ax = cmul(ck, accK), bx = cmul(cm, accM);
+  V res;
+  res.x = ax.x + bx.x + accX.x;
+  res.y = ax.y + bx.y + accX.y;
+  return res;
+}
+
+// FP64 residual of a few frozen columns on a narrow block of CB columns
+// (MODE 4): x holds their x64 columns gathered, xn[r][c] = x64[r][map[c]],
+// and r64[r][map[c]] = b64[r][map[c]] - (A x)[r][c] is stored for the nfz
+// frozen columns c < nfz (the padding columns c >= nfz store nothing).
+// b64 and r64 are [row][ld].
+struct NarrowOut {
+  const int *map;
+  int nfz, ld;
+  const double2 *b64;
+  double2 *r64;
+};
+__device__ inline void narrow_store(const NarrowOut &nw, int64_t r, int c, double2 v) {
+  if (c >= nw.nfz) return;
+  const int64_t o = r * nw.ld + nw.map[c];
+  const double2 bv = nw.b64[o];
+  nw.r64[o] = {bv.x - v.x, bv.y - v.y};
+}
+
 // Row epilogue shared by the operator kernels: combine the K, M and extra
 // sums with the frequency's coefficients and apply MODE.
 template <typename T, int B, int MODE>
@@ -120,12 +153,7 @@ __device__ inline void op_finish(const OpArgs<T> &a, const OpCoef &cf, int64_t r
   using V = typename VT<T>::t;
   const int64_t o = r * B + f;
   const bool fl = fld == 1;
-  const V ck = fl ? V{(T)cf.kf.x, (T)cf.kf.y} : V{(T)cf.ks.x, (T)cf.ks.y};
-  const V cm = fl ? V{(T)cf.mf.x, (T)cf.mf.y} : V{(T)cf.ms.x, (T)cf.ms.y};
-  V ax = cmul(ck, accK), bx = cmul(cm, accM);
-  V res;
-  res.x = ax.x + bx.x + accX.x;
-  res.y = ax.y + bx.y + accX.y;
+  const V res = op_value<T>(cf, fl, accK, accM, accX);
   if (MODE == 0) {
     y[o] = res;
     return;
@@ -184,7 +212,7 @@ __device__ inline typename VT<T>::t op_extras(const OpArgs<T> &a, const OpCoef &
 template <typename T, int B, int MODE>
 __device__ inline void op_body(const OpArgs<T> &a, const OpCoef *__restrict__ coef,
                                const typename VT<T>::t *__restrict__ x, const typename VT<T>::t *__restrict__ b,
-                               typename VT<T>::t *__restrict__ y, int blk) {
+                               typename VT<T>::t *__restrict__ y, int blk, NarrowOut nw = {}) {
   using V = typename VT<T>::t;
   using SH = OpShape<B>;
   constexpr int G = SH::G, S = SH::S, U = SH::U, CH = SH::CH;
@@ -199,7 +227,10 @@ __device__ inline void op_body(const OpArgs<T> &a, const OpCoef *__restrict__ co
   const uint8_t fld = a.field[r];
   const OpCoef &cf = coef[f];
   if (fld == 2) {  // Dirichlet row: identity
-    if (sub == 0) op_identity_row<T, B, MODE>(r, f, x, b, y);
+    if (sub == 0) {
+      if constexpr (MODE == 4) narrow_store(nw, r, f, x[r * B + f]);
+      else op_identity_row<T, B, MODE>(r, f, x, b, y);
+    }
     return;
   }
   V accK = {0, 0}, accM = {0, 0}, accX = {0, 0};
@@ -238,7 +269,8 @@ __device__ inline void op_body(const OpArgs<T> &a, const OpCoef *__restrict__ co
     if (sub == 0) accX = op_extras<T, B>(a, cf, r, f, x);
   }
   if (sub != 0) return;
-  op_finish<T, B, MODE>(a, cf, r, f, fld, accK, accM, accX, x, b, y);
+  if constexpr (MODE == 4) narrow_store(nw, r, f, op_value<T>(cf, fld == 1, accK, accM, accX));
+  else op_finish<T, B, MODE>(a, cf, r, f, fld, accK, accM, accX, x, b, y);
 }
 
 template <typename T, int B, int MODE>
@@ -247,6 +279,12 @@ __global__ __launch_bounds__(256) void op_kernel(OpArgs<T> a, const OpCoef *__re
                                                  const typename VT<T>::t *__restrict__ b,
                                                  typename VT<T>::t *__restrict__ y) {
   op_body<T, B, MODE>(a, coef, x, b, y, xcd_block());
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void narrow_residual_kernel(OpArgs<double> a, const OpCoef *__restrict__ coef,
+                                                              const double2 *__restrict__ xn, NarrowOut nw) {
+  op_body<double, CB, 4>(a, coef, xn, nullptr, nullptr, xcd_block(), nw);
 }
 
 constexpr int kRowPad = 8;  // padded rows hold a multiple of this many entries
@@ -1591,25 +1629,15 @@ __global__ void unpack_col_kernel(int64_t n, int B, int c, const double2 *__rest
 }
 
 // FP64 refinement of a few frozen columns: their x64 columns gathered into a
-// narrow [row][CB] block (xc[r][f] = x64[r][map[f]]), and r64[r][map[f]] =
-// F[r] - (A x)[r][f] scattered back for the nfz frozen columns (the same
-// values as b - A x computed on all B columns: each column is summed alone)
+// narrow [row][CB] block (xc[r][f] = x64[r][map[f]]); narrow_residual_kernel
+// then stores r64[r][map[f]] = b64[r][map[f]] - (A xc)[r][f] for the nfz
+// frozen columns (the same values as b - A x computed on all B columns: each
+// column is summed alone)
 __global__ void gather_cols_kernel(int64_t n, int B, int CB, const int *__restrict__ map,
                                    const double2 *__restrict__ x, double2 *__restrict__ xc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * CB) return;
   xc[i] = x[(i / CB) * B + map[i % CB]];
-}
-__global__ void scatter_residual_kernel(int64_t n, int B, int CB, int nfz, const int *__restrict__ map,
-                                        const double2 *__restrict__ b64, const double2 *__restrict__ y,
-                                        double2 *__restrict__ r64) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * CB) return;
-  const int f = (int)(i % CB);
-  if (f >= nfz) return;
-  const int64_t r = i / CB;
-  const double2 bv = b64[r * B + map[f]], yv = y[i];
-  r64[r * B + map[f]] = {bv.x - yv.x, bv.y - yv.y};
 }
 
 // columns taking a new frequency: x64 = g (the prescribed values on
@@ -3166,12 +3194,13 @@ struct Solver {
     hipLaunchKernelGGL((op_kernel<TT, B, MODE>), dim3(grid), dim3(256), 0, s, L.template args<TT>(), coef, x, b, y);
     ELP_HIP(hipGetLastError());
   }
-  // y = A x in FP64 on CB columns (x, y: [row][CB]); coef holds CB entries
+  // r64 = b64 - A x in FP64 on the nw.nfz frozen columns, x: their [row][CB]
+  // gather; coef holds CB entries
   template <int CB>
-  void op_fp64_narrow(const OpCoef *coef, const double2 *x, double2 *y) {
+  void residual_fp64_narrow(const OpCoef *coef, const double2 *x, const NarrowOut &nw) {
     const LevelData &L = *st.levels[0];
-    hipLaunchKernelGGL((op_kernel<double, CB, 0>), dim3(xcd_grid(nblk(L.n, OpShape<CB>::RPB))), dim3(256), 0, s,
-                       L.template args<double>(), coef, x, nullptr, y);
+    hipLaunchKernelGGL((narrow_residual_kernel<CB>), dim3(xcd_grid(nblk(L.n, OpShape<CB>::RPB))), dim3(256), 0, s,
+                       L.template args<double>(), coef, x, nw);
     ELP_HIP(hipGetLastError());
   }
   template <bool ADD>
@@ -3784,7 +3813,7 @@ struct SlotRun {
   std::vector<OpCoef> cAn;
   DArr<int> d_fmap, d_fresh;
   DArr<OpCoef> d_cAn;
-  DArr<double2> xn, yn;
+  DArr<double2> xn;
   DArr<double> d_scale, d_wgt;
   struct Slot {
     int64_t f = -1;   // frequency index, -1 = empty
@@ -4113,19 +4142,14 @@ void SlotRun<T, B>::after_norms() {
     const int CB = nfz <= 8 ? 8 : 16;
     for (int f = nfz; f < CB; f++) fmap[f] = fmap[0];  // padding columns repeat a frozen one
     for (int f = 0; f < CB; f++) cAn[f] = cA[fmap[f]];
-    if (!xn.p) {
-      xn.alloc(m.ctx, m.n * 16);
-      yn.alloc(m.ctx, m.n * 16);
-    }
+    if (!xn.p) xn.alloc(m.ctx, m.n * 16);
     ELP_HIP(hipMemcpyAsync(d_fmap.p, fmap.data(), sizeof(int) * CB, hipMemcpyHostToDevice, s));
     ELP_HIP(hipMemcpyAsync(d_cAn.p, cAn.data(), sizeof(OpCoef) * CB, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(gather_cols_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, d_fmap.p, x64.p,
                        xn.p);
-    if (CB == 8) S.template op_fp64_narrow<8>(d_cAn.p, xn.p, yn.p);
-    else S.template op_fp64_narrow<16>(d_cAn.p, xn.p, yn.p);
-    hipLaunchKernelGGL(scatter_residual_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, nfz,
-                       d_fmap.p, b64.p, yn.p, r64.p);
-    ELP_HIP(hipGetLastError());
+    const NarrowOut nw = {d_fmap.p, nfz, B, b64.p, r64.p};
+    if (CB == 8) S.template residual_fp64_narrow<8>(d_cAn.p, xn.p, nw);
+    else S.template residual_fp64_narrow<16>(d_cAn.p, xn.p, nw);
   } else {
     S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
   }
