@@ -1628,16 +1628,37 @@ __global__ void unpack_col_kernel(int64_t n, int B, int c, const double2 *__rest
   out[r] = x[r * B + c];
 }
 
-// FP64 refinement of a few frozen columns: their x64 columns gathered into a
-// narrow [row][CB] block (xc[r][f] = x64[r][map[f]]); narrow_residual_kernel
-// then stores r64[r][map[f]] = b64[r][map[f]] - (A xc)[r][f] for the nfz
-// frozen columns (the same values as b - A x computed on all B columns: each
-// column is summed alone)
-__global__ void gather_cols_kernel(int64_t n, int B, int CB, const int *__restrict__ map,
-                                   const double2 *__restrict__ x, double2 *__restrict__ xc) {
+// FP64 refinement of a few frozen columns: add_scaled_kernel's update
+// x64[r][map[c]] += w[map[c]] * d[iperm[r]][map[c]] on the nfz frozen
+// columns c < nfz (w = 0: unchanged), with the result also gathered into a
+// narrow [row][CB] block xc[r][c]; the padding columns c >= nfz of xc are 0.
+// narrow_residual_kernel then stores r64[r][map[c]] = b64[r][map[c]] -
+// (A xc)[r][c] for c < nfz (the same values as b - A x computed on all B
+// columns: each column is summed alone)
+template <typename V>
+__global__ void add_gather_kernel(int64_t n, int B, int CB, int nfz, const int *__restrict__ map,
+                                  const double *__restrict__ w, const V *__restrict__ d,
+                                  const int32_t *__restrict__ iperm, double2 *__restrict__ x,
+                                  double2 *__restrict__ xc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * CB) return;
-  xc[i] = x[(i / CB) * B + map[i % CB]];
+  const int c = (int)(i % CB);
+  if (c >= nfz) {
+    xc[i] = {0, 0};
+    return;
+  }
+  const int64_t r = i / CB;
+  const int f = map[c];
+  const int64_t o = r * B + f;
+  double2 v = x[o];
+  const double sc = w[f];
+  if (sc != 0.0) {
+    const V dv = d[(iperm ? iperm[r] : r) * B + f];
+    v.x += sc * (double)dv.x;
+    v.y += sc * (double)dv.y;
+    x[o] = v;
+  }
+  xc[i] = v;
 }
 
 // columns taking a new frequency: x64 = g (the prescribed values on
@@ -4131,26 +4152,28 @@ void SlotRun<T, B>::after_norms() {
   for (int k = 0; k < B; k++) was_iter[k] = sl[k].state == 1;
   for (int k = 0; k < B; k++) wgt[k] = (sl[k].state == 2 && !sl[k].broken) ? sl[k].w : 0.0;
   ELP_HIP(hipMemcpyAsync(d_wgt.p, wgt.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
-                     iperm);
-  // r64 = b - A x64 on the frozen columns: on a narrow block of 8 or 16
-  // columns when few are frozen (the gathers of x shrink with the block)
+  // x64 += w x_in on the frozen columns, then r64 = b - A x64 on them: on a
+  // narrow block of 8 or 16 columns when few are frozen (the gathers of x
+  // shrink with the block; the update and the gather into the block are one
+  // pass)
   int nfz = 0;
   for (int k = 0; k < B; k++)
     if (sl[k].state == 2) fmap[nfz++] = k;
   if (B == 32 && nfz <= 16) {
     const int CB = nfz <= 8 ? 8 : 16;
-    for (int f = nfz; f < CB; f++) fmap[f] = fmap[0];  // padding columns repeat a frozen one
+    for (int f = nfz; f < CB; f++) fmap[f] = fmap[0];  // padding columns take a frozen one's coefficients
     for (int f = 0; f < CB; f++) cAn[f] = cA[fmap[f]];
     if (!xn.p) xn.alloc(m.ctx, m.n * 16);
     ELP_HIP(hipMemcpyAsync(d_fmap.p, fmap.data(), sizeof(int) * CB, hipMemcpyHostToDevice, s));
     ELP_HIP(hipMemcpyAsync(d_cAn.p, cAn.data(), sizeof(OpCoef) * CB, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(gather_cols_kernel, dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, d_fmap.p, x64.p,
-                       xn.p);
+    hipLaunchKernelGGL((add_gather_kernel<V>), dim3(nblk(m.n * CB, tpb)), dim3(tpb), 0, s, m.n, B, CB, nfz,
+                       d_fmap.p, d_wgt.p, S.xi.p, iperm, x64.p, xn.p);
     const NarrowOut nw = {d_fmap.p, nfz, B, b64.p, r64.p};
     if (CB == 8) S.template residual_fp64_narrow<8>(d_cAn.p, xn.p, nw);
     else S.template residual_fp64_narrow<16>(d_cAn.p, xn.p, nw);
   } else {
+    hipLaunchKernelGGL((add_scaled_kernel<V>), dim3(nblk(nt, tpb)), dim3(tpb), 0, s, nt, B, d_wgt.p, S.xi.p, x64.p,
+                       iperm);
     S.template op<1, double>(0, S.coefA.p, x64.p, b64.p, r64.p);
   }
   S.wreduce64(r64.p, S.red_out.p);
